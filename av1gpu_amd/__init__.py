@@ -1,0 +1,322 @@
+"""av1gpu_amd -- MI355X-native AV1 intra rate-distortion core.
+
+Python host mirror of the reference's encoder API (maddog107/av1gpu encoder/src/pipeline.h:191-216): ``GpEncoder``
+with ``init(config)`` / ``encode(frames)`` / ``config()``, ``GpConfig`` / ``GpParams`` knobs, ``GpFrameIn`` /
+``GpFrameOut`` / ``GpTimes``, over the C ABI of ``include/av1mi.h`` (``libav1mi.so``, built in-tree by
+``av1gpu_amd/csrc/Makefile``).  The HIP library is the only compute path: if it is missing, every call raises.
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libav1mi.so")
+
+BLOCK_DTYPE = np.dtype([("mi_row", "<u2"), ("mi_col", "<u2"), ("bsize", "u1"), ("skip", "u1"), ("tx_depth", "u1"),
+                        ("y_mode", "u1"), ("angle_delta_y", "i1"), ("uv_mode", "u1"), ("angle_delta_uv", "i1"),
+                        ("cfl_alpha_u", "i1"), ("cfl_alpha_v", "i1"), ("filter_intra", "u1"), ("cdef_idx", "i1"),
+                        ("sb_qindex", "u1")])
+TXB_DTYPE = np.dtype([("coeff_off", "<u4"), ("x4", "<u2"), ("y4", "<u2"), ("plane", "u1"), ("tx_size", "u1"),
+                      ("tx_type", "u1"), ("flags", "u1"), ("eob", "<u2"), ("reserved", "<u2")])
+
+_PARAM_FLOATS = ("lambdaMul", "satdLambda", "rnd", "zbin", "rndC", "zbinC")
+_PARAM_INTS = ("K", "Krecheck", "txSearch", "txSearchCL", "txAllCL", "deltas", "uvDeltas", "cfl", "uvK", "shapeMask",
+               "partMask8", "partMaskN", "maxPart")
+_PARAM_FLOATS2 = ("uvOverheadBits", "splitBias", "lambdaCMul")
+
+
+class _Params(C.Structure):
+    _fields_ = [(n, C.c_float) for n in _PARAM_FLOATS] + [(n, C.c_int) for n in _PARAM_INTS] + \
+               [(n, C.c_float) for n in _PARAM_FLOATS2]
+
+
+class _Config(C.Structure):
+    _fields_ = [("qindex", C.c_int), ("edgeFilter", C.c_int), ("P", _Params)]
+
+
+class _FrameIn(C.Structure):
+    _fields_ = [("W", C.c_int), ("H", C.c_int), ("mono", C.c_int), ("yuv444", C.c_int), ("bitDepth", C.c_int),
+                ("dsrc", C.c_void_p * 3), ("spitch", C.c_int * 3), ("aw", C.c_int), ("ah", C.c_int), ("qindex", C.c_int)]
+
+
+class _FrameOut(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("base_q_idx", C.c_uint32), ("bit_depth", C.c_uint32),
+                ("chroma_format", C.c_uint32), ("monochrome", C.c_uint32), ("n_blocks", C.c_uint32), ("n_txb", C.c_uint32),
+                ("n_coeffs", C.c_uint64), ("blocks", C.c_void_p), ("block_txb", C.c_void_p), ("txb", C.c_void_p),
+                ("coeffs", C.c_void_p), ("recon", C.c_void_p * 3), ("recon_pitch", C.c_int * 3), ("aw", C.c_int),
+                ("ah", C.c_int), ("tile_cols", C.c_int), ("tile_rows", C.c_int)]
+
+
+class _Times(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("setup", "analysis", "dp", "recon", "pack", "download", "total")] + \
+               [(n, C.c_uint32) for n in ("steps", "sbs", "blocks", "skipBlocks", "cflBlocks", "rangeFixes")] + \
+               [("deviceBytes", C.c_size_t)]
+
+
+_lib = None
+
+
+def lib():
+    """The HIP library (loaded once).  Raises if it is not built: there is no other compute path."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libav1mi.so is not built (run __graft_entry__.build() or make -f av1gpu_amd/csrc/Makefile)")
+        L = C.CDLL(LIB_PATH)
+        L.av1mi_default_params.argtypes = [C.POINTER(_Params)]
+        L.av1mi_check_partitions.argtypes = [C.POINTER(_Params), C.c_char_p, C.c_int]
+        L.av1mi_create.argtypes = [C.POINTER(_Config), C.POINTER(C.c_void_p), C.c_char_p, C.c_int]
+        L.av1mi_destroy.argtypes = [C.c_void_p]
+        L.av1mi_get_config.argtypes = [C.c_void_p, C.POINTER(_Config)]
+        L.av1mi_encode.argtypes = [C.c_void_p, C.POINTER(_FrameIn), C.c_int, C.POINTER(_FrameOut), C.POINTER(_Times),
+                                   C.c_void_p, C.c_int, C.c_char_p, C.c_int]
+        L.av1mi_frame_out_free.argtypes = [C.POINTER(_FrameOut)]
+        L.av1mi_dev_alloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        L.av1mi_dev_free.argtypes = [C.c_void_p]
+        L.av1mi_copy_h2d.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.av1mi_synth_image.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.POINTER(C.c_int)]
+        L.av1mi_fwd_matrix.argtypes = [C.c_int, C.c_int, C.c_void_p]
+        L.av1mi_misc_tables.argtypes = [C.c_void_p] * 7
+        L.av1mi_cost_table.argtypes = [C.c_int, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+@dataclass
+class GpParams:
+    """Search knobs (the reference's GpParams, encoder/src/gp.h:29-68); defaults = the plain maximum-effort search."""
+    lambdaMul: float = 1.0
+    satdLambda: float = 1.0
+    rnd: float = 0.375
+    zbin: float = 0.656
+    rndC: float = 0.375
+    zbinC: float = 0.656
+    K: int = 6
+    Krecheck: int = 6
+    txSearch: int = 1
+    txSearchCL: int = 1
+    txAllCL: int = 1
+    deltas: int = 1
+    uvDeltas: int = 1
+    cfl: int = 1
+    uvK: int = 6
+    shapeMask: int = 0x3F1FFF
+    partMask8: int = 0xF
+    partMaskN: int = 0x3FF
+    maxPart: int = 64
+    uvOverheadBits: float = 1.5
+    splitBias: float = 1.0
+    lambdaCMul: float = 1.0
+
+    def _c(self):
+        p = _Params()
+        for n in _PARAM_FLOATS + _PARAM_INTS + _PARAM_FLOATS2:
+            setattr(p, n, getattr(self, n))
+        return p
+
+
+@dataclass
+class GpConfig:
+    """GpConfig (encoder/src/pipeline.h:44-105) subset for the intra RD core."""
+    qindex: int = 120
+    edgeFilter: int = 1
+    P: GpParams = field(default_factory=GpParams)
+
+
+@dataclass
+class GpFrameIn:
+    """One input frame (pipeline.h:149-163): planes padded by edge replication to aw x ah (multiples of 64).
+    Planes are device pointers (ints, e.g. torch tensors' data_ptr()) or host numpy arrays (uploaded by encode)."""
+    W: int
+    H: int
+    planes: tuple = ()
+    spitch: tuple = ()
+    aw: int = 0
+    ah: int = 0
+    mono: int = 0
+    yuv444: int = 0
+    bitDepth: int = 8
+    qindex: int = 0
+
+
+@dataclass
+class GpFrameOut:
+    """DecisionRecord + reconstruction of one frame (pipeline.h:165-178)."""
+    width: int
+    height: int
+    base_q_idx: int
+    bit_depth: int
+    blocks: np.ndarray
+    block_txb: np.ndarray
+    txb: np.ndarray
+    coeffs: np.ndarray
+    recon: list
+    tile_cols: int
+    tile_rows: int
+
+
+@dataclass
+class GpTimes:
+    setup: float = 0
+    analysis: float = 0
+    dp: float = 0
+    recon: float = 0
+    pack: float = 0
+    download: float = 0
+    total: float = 0
+    steps: int = 0
+    sbs: int = 0
+    blocks: int = 0
+    skipBlocks: int = 0
+    cflBlocks: int = 0
+    rangeFixes: int = 0
+    deviceBytes: int = 0
+
+
+def check_partitions(P: GpParams):
+    """gp_check_partitions (pipeline.h:216): raises ValueError for options without a finite partition everywhere."""
+    err = C.create_string_buffer(256)
+    p = P._c()
+    if lib().av1mi_check_partitions(C.byref(p), err, 256) != 0:
+        raise ValueError(err.value.decode())
+
+
+def pad_planes(planes, W, H, ssx=1):
+    """Edge-replicate visible planes to the 64-aligned coded area (the layout GpFrameIn expects)."""
+    aw, ah = ((W + 63) // 64) * 64, ((H + 63) // 64) * 64
+    out = []
+    for p, a in enumerate(planes):
+        sx = ssx if p else 0
+        vw, vh = (W + sx) >> sx, (H + sx) >> sx
+        a = a[:vh, :vw]
+        out.append(np.ascontiguousarray(np.pad(a, ((0, (ah >> sx) - vh), (0, (aw >> sx) - vw)), mode="edge")))
+    return out, aw, ah
+
+
+def synth_image(seed, W, H, ssx=1, bd=8):
+    """The benchmark's seeded synthetic image (visible planes, host numpy)."""
+    dt = np.uint8 if bd == 8 else np.uint16
+    cw, ch = (W + ssx) >> ssx, (H + ssx) >> ssx
+    y, u, v = np.zeros((H, W), dt), np.zeros((ch, cw), dt), np.zeros((ch, cw), dt)
+    st = (C.c_int * 3)(W, cw, cw)
+    rc = lib().av1mi_synth_image(seed, W, H, ssx, bd, y.ctypes.data, u.ctypes.data, v.ctypes.data, st)
+    if rc != 0:
+        raise ValueError("synth_image: bad arguments")
+    return [y, u, v]
+
+
+class DeviceFrame:
+    """Padded planes of one frame resident in device memory (owned; freed on close / garbage collection)."""
+
+    def __init__(self, planes, W, H, bd=8, qindex=0):
+        padded, aw, ah = pad_planes(planes, W, H)
+        self.W, self.H, self.aw, self.ah, self.bd, self.qindex = W, H, aw, ah, bd, qindex
+        self.ptrs, self.pitch = [], []
+        for a in padded:
+            p = C.c_void_p()
+            if lib().av1mi_dev_alloc(C.byref(p), a.nbytes) != 0:
+                self.close()
+                raise MemoryError("device allocation of %d bytes failed" % a.nbytes)
+            if lib().av1mi_copy_h2d(p, a.ctypes.data, a.nbytes) != 0:
+                self.close()
+                raise RuntimeError("host-to-device copy failed")
+            self.ptrs.append(p.value)
+            self.pitch.append(a.shape[1])
+
+    def frame_in(self):
+        return GpFrameIn(W=self.W, H=self.H, planes=tuple(self.ptrs), spitch=tuple(self.pitch), aw=self.aw, ah=self.ah,
+                         bitDepth=self.bd, qindex=self.qindex)
+
+    def close(self):
+        for p in self.ptrs:
+            lib().av1mi_dev_free(p)
+        self.ptrs = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpEncoder:
+    """GpEncoder (encoder/src/pipeline.h:191-211): init(config) once, encode(frames) per batch."""
+
+    def __init__(self):
+        self._h = None
+        self._cfg = None
+
+    def init(self, cfg: GpConfig):
+        if self._h:
+            lib().av1mi_destroy(self._h)
+            self._h = None
+        c = _Config(cfg.qindex, cfg.edgeFilter, cfg.P._c())
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        if lib().av1mi_create(C.byref(c), C.byref(h), err, 512) != 0:
+            raise RuntimeError("GpEncoder.init: " + err.value.decode())
+        self._h = h.value
+        self._cfg = cfg
+        return self
+
+    def config(self) -> GpConfig:
+        return self._cfg
+
+    def encode(self, frames, stream=None, want_recon=True):
+        """Encode a batch; frames: GpFrameIn with device pointers, or DeviceFrame.  Returns ([GpFrameOut], GpTimes)."""
+        if not self._h:
+            raise RuntimeError("GpEncoder.encode before init")
+        fins = [f.frame_in() if isinstance(f, DeviceFrame) else f for f in frames]
+        n = len(fins)
+        arr = (_FrameIn * max(n, 1))()
+        for i, f in enumerate(fins):
+            a = arr[i]
+            a.W, a.H, a.mono, a.yuv444, a.bitDepth = f.W, f.H, f.mono, f.yuv444, f.bitDepth
+            for p in range(3):
+                a.dsrc[p] = f.planes[p]
+                a.spitch[p] = f.spitch[p]
+            a.aw, a.ah, a.qindex = f.aw, f.ah, f.qindex
+        outs = (_FrameOut * max(n, 1))()
+        t = _Times()
+        err = C.create_string_buffer(512)
+        rc = lib().av1mi_encode(self._h, arr, n, outs, C.byref(t), stream or None, 1 if want_recon else 0, err, 512)
+        if rc != 0:
+            raise RuntimeError("GpEncoder.encode: " + err.value.decode())
+        res = []
+        try:
+            for i in range(n):
+                o = outs[i]
+                nb, nt, nc = o.n_blocks, o.n_txb, o.n_coeffs
+                blocks = np.frombuffer(C.string_at(o.blocks, nb * 16), BLOCK_DTYPE).copy() if nb else np.zeros(0, BLOCK_DTYPE)
+                bt = np.frombuffer(C.string_at(o.block_txb, (nb + 1) * 4), np.uint32).copy()
+                txb = np.frombuffer(C.string_at(o.txb, nt * 16), TXB_DTYPE).copy() if nt else np.zeros(0, TXB_DTYPE)
+                co = np.frombuffer(C.string_at(o.coeffs, nc * 2), np.int16).copy() if nc else np.zeros(0, np.int16)
+                rec = []
+                if want_recon:
+                    bps = 2 if o.bit_depth > 8 else 1
+                    for p in range(3):
+                        h = o.ah >> (1 if p else 0)
+                        w = o.recon_pitch[p]
+                        buf = C.string_at(o.recon[p], w * h * bps)
+                        rec.append(np.frombuffer(buf, np.uint16 if bps == 2 else np.uint8).reshape(h, w).copy())
+                res.append(GpFrameOut(o.width, o.height, o.base_q_idx, o.bit_depth, blocks, bt, txb, co, rec, o.tile_cols,
+                                      o.tile_rows))
+        finally:
+            for i in range(n):
+                lib().av1mi_frame_out_free(C.byref(outs[i]))
+        times = GpTimes(**{k: getattr(t, k) for k, _ in _Times._fields_})
+        return res, times
+
+    def close(self):
+        if self._h:
+            lib().av1mi_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

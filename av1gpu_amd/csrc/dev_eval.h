@@ -1,0 +1,403 @@
+// Device building blocks of the search, executed by a "group" of 32 lanes (half of a wave64; two groups per wave):
+// the float forward transform, dead-zone quantizer, distortion and exact coefficient rate of one transform block,
+// intra edge preparation and SATD.
+//
+// Arithmetic contract (shared with the CPU oracle, bit for bit): every forward-transform output is one fmaf chain in
+// ascending k; sums over coefficients run on 32 virtual lanes (lane l takes elements l, l + 32, ... in order) combined
+// by the xor butterfly 16, 8, 4, 2, 1; J = fmaf(lambda, bits, D).  Which physical lane computes a chain does not
+// change a value, so the passes spread their chains over the group freely.
+#pragma once
+#include "av1m_common.h"
+
+namespace av1m {
+
+constexpr int GW = 32;   // lanes per group
+
+__device__ __forceinline__ int glane() { return threadIdx.x & 31; }
+__device__ __forceinline__ int gid() { return threadIdx.x >> 5; }
+// orders the group's LDS accesses (one wave executes both groups in lockstep; this stops compiler reordering)
+__device__ __forceinline__ void gsync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ float gsum(float v) {
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+    return v;
+}
+__device__ __forceinline__ int gsumi(int v) {
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+    return v;
+}
+__device__ __forceinline__ int gmaxi(int v) {
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 32));
+    return v;
+}
+// index of the smallest (v, i) over the group, the lowest index among equal values; -1 if every v is INF
+__device__ __forceinline__ int gargmin(float v, int i) {
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) {
+        const float ov = __shfl_xor(v, o, 32);
+        const int oi = __shfl_xor(i, o, 32);
+        if (ov < v || (ov == v && oi < i)) { v = ov; i = oi; }
+    }
+    return v < INF ? i : -1;
+}
+
+struct EvalIn {
+    int txSz, txType, plane, dcQ, acQ, skipCtx, dcCtx, intraDir, bd;
+    float rnd, zbin, resEnergy;
+    const CostTab *C;
+};
+struct EvalOut { float D, R; int eob; };
+
+__device__ __forceinline__ int txsz_ctx(int t) { return (av1t::Tx_Size_Sqr[t] + av1t::Tx_Size_Sqr_Up[t] + 1) >> 1; }
+
+// txb_skip, luma tx type and eob position bits
+__device__ float base_bits(const EvalIn &e, int eob) {
+    const CostTab &C = *e.C;
+    const int t = e.txSz, ctx = txsz_ctx(t);
+    if (eob == 0) return C.txb_skip[ctx][e.skipCtx][1];
+    float b = C.txb_skip[ctx][e.skipCtx][0];
+    const int pt = e.plane > 0;
+    if (e.plane == 0) {
+        const int set = tx_set_intra(t), sqr = av1t::Tx_Size_Sqr[t];
+        if (set == TX_SET_INTRA_1) {
+            int idx = 0;
+            for (int i = 0; i < 7; i++) idx = av1t::Tx_Type_Intra_Inv_Set1[i] == e.txType ? i : idx;
+            b += C.tx_set1[sqr][e.intraDir][idx];
+        } else if (set == TX_SET_INTRA_2) {
+            int idx = 0;
+            for (int i = 0; i < 5; i++) idx = av1t::Tx_Type_Intra_Inv_Set2[i] == e.txType ? i : idx;
+            b += C.tx_set2[sqr][e.intraDir][idx];
+        }
+    }
+    const int ms = min((int)av1t::Tx_Width_Log2[t], 5) + min((int)av1t::Tx_Height_Log2[t], 5) - 4;
+    const int eobPt = eob <= 2 ? eob : 2 + (31 - __clz(eob - 1));
+    const int ec = tx_class(e.txType) == TX_CLASS_2D ? 0 : 1;
+    const float p = ms == 0 ? C.eob_pt16[pt][ec][eobPt - 1] : ms == 1 ? C.eob_pt32[pt][ec][eobPt - 1]
+                  : ms == 2 ? C.eob_pt64[pt][ec][eobPt - 1] : ms == 3 ? C.eob_pt128[pt][ec][eobPt - 1]
+                  : ms == 4 ? C.eob_pt256[pt][ec][eobPt - 1] : ms == 5 ? C.eob_pt512[pt][eobPt - 1] : C.eob_pt1024[pt][eobPt - 1];
+    b += p;
+    if (eobPt >= 3) {
+        const int rem = eob - ((1 << (eobPt - 2)) + 1);
+        b += C.eob_extra[ctx][pt][eobPt - 3][(rem >> (eobPt - 3)) & 1];
+        b += (float)(eobPt - 3);
+    }
+    return b;
+}
+
+// bits of the coefficient at scan position c (< eob) given the levels q (natural order, stride = adjusted width)
+__device__ float coeff_bits(int c, int eob, const int16_t *q, const uint16_t *scan, int t, int cls, int pt, int dcCtx, const CostTab &C) {
+    const int ctx = txsz_ctx(t), adj = av1t::Adjusted_Tx_Size[t];
+    const int bwl = av1t::Tx_Width_Log2[adj], width = 1 << bwl, height = av1t::Tx_Height[adj];
+    const int pos = scan[c], qv = q[pos], v = qv < 0 ? -qv : qv, level = v < 15 ? v : 15;
+    const int row = pos >> bwl, col = pos - (row << bwl);
+    float b;
+    if (c == eob - 1) {
+        const int bctx = c == 0 ? 0 : c <= (height << bwl) / 8 ? 1 : c <= (height << bwl) / 4 ? 2 : 3;
+        b = C.coeff_base_eob[ctx][pt][bctx][min(level, 3) - 1];
+    } else {
+        int mag = 0;
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            const int rr = row + av1t::Sig_Ref_Diff_Offset[cls][k][0], cc = col + av1t::Sig_Ref_Diff_Offset[cls][k][1];
+            if (rr < height && cc < width) { const int a = q[(rr << bwl) + cc]; mag += min(a < 0 ? -a : a, 3); }
+        }
+        int bctx = min((mag + 1) >> 1, 4);
+        if (cls == TX_CLASS_2D) bctx = (row == 0 && col == 0) ? 0 : bctx + av1t::Coeff_Base_Ctx_Offset[t][min(row, 4)][min(col, 4)];
+        else bctx += av1t::Coeff_Base_Pos_Ctx_Offset[min(cls == TX_CLASS_VERT ? row : col, 2)];
+        b = C.coeff_base[ctx][pt][bctx][min(level, 3)];
+    }
+    if (level > 2) {
+        int mag = 0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int rr = row + av1t::Mag_Ref_Offset_With_Tx_Class[cls][k][0], cc = col + av1t::Mag_Ref_Offset_With_Tx_Class[cls][k][1];
+            if (rr < height && cc < width) { const int a = q[rr * width + cc]; mag += min(a < 0 ? -a : a, 15); }
+        }
+        mag = min((mag + 1) >> 1, 6);
+        const int brctx = pos == 0 ? mag : cls == TX_CLASS_2D ? ((row < 2 && col < 2) ? mag + 7 : mag + 14)
+                        : cls == TX_CLASS_HORIZ ? (col == 0 ? mag + 7 : mag + 14) : (row == 0 ? mag + 7 : mag + 14);
+        const float *br = C.coeff_br[min(ctx, (int)TX_32X32)][pt][brctx];
+        int rem = level - 3;
+        for (int i = 0; i < 4; i++) {
+            const int k = min(rem, 3);
+            b += br[k];
+            rem -= k;
+            if (k < 3) break;
+        }
+    }
+    if (v) {
+        b += c == 0 ? C.dc_sign[pt][dcCtx][qv < 0] : 1.0f;
+        if (v > 14) b += (float)(2 * (31 - __clz(v - 14)) + 1);
+    }
+    return b;
+}
+
+// Evaluation scratch of one group in LDS, for tx blocks up to W x H
+template <int W, int H>
+struct EvalScr {
+    static constexpr int TS = W + 1, TH = H > 32 ? 32 : H;
+    int16_t res[W * H];
+    float tmp[TH * TS];
+    int16_t q[(W > 32 ? 32 : W) * TH];
+};
+
+// Forward transform + quantization + distortion + rate of the residual in S.res (W x H, row-major), by the 32 lanes of
+// the calling group.  Levels in S.q (natural order, tw x th).  All lanes return the same values.
+template <int W, int H, typename Scr>
+__device__ EvalOut grp_eval(Scr &S, const EvalIn &e, const DevTables &T) {
+    constexpr int TW = W > 32 ? 32 : W, TH = H > 32 ? 32 : H, TS = Scr::TS;
+    constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
+    constexpr int LTW = LW > 5 ? 5 : LW;
+    constexpr int N = TW * TH;
+    const int lane = glane();
+    const int rk = row_kind(e.txType), ck = col_kind(e.txType), fud = flip_ud(e.txType), flr = flip_lr(e.txType);
+    // ---- column pass: item = (column j, OC consecutive outputs) ----
+    {
+        constexpr int OC = TH < 8 ? TH : 8, NI = W * (TH / OC);
+        const float *FcT = T.fwdT + T.fwdOff[ck][LH];
+        const float *Fc = T.fwd + T.fwdOff[ck][LH];
+        for (int it = lane; it < NI; it += GW) {
+            const int j = it % W, i0 = (it / W) * OC, jj = flr ? W - 1 - j : j;
+            float acc[OC];
+            if (ck == K_IDT) {
+#pragma unroll
+                for (int o = 0; o < OC; o++) acc[o] = Fc[(i0 + o) * H + i0 + o] * (float)S.res[(fud ? H - 1 - (i0 + o) : i0 + o) * W + jj];
+            } else {
+#pragma unroll
+                for (int o = 0; o < OC; o++) acc[o] = 0.f;
+#pragma unroll 4
+                for (int k = 0; k < H; k++) {
+                    const float x = (float)S.res[(fud ? H - 1 - k : k) * W + jj];
+                    const float *f = FcT + k * H + i0;
+#pragma unroll
+                    for (int o = 0; o < OC; o++) acc[o] = fmaf(f[o], x, acc[o]);
+                }
+            }
+#pragma unroll
+            for (int o = 0; o < OC; o++) S.tmp[(i0 + o) * TS + j] = acc[o];
+        }
+    }
+    gsync();
+    // ---- row pass, in place: lane i owns row i (reads it whole, then writes its tw outputs) ----
+    {
+        const float *FrT = T.fwdT + T.fwdOff[rk][LW];
+        const float *Fr = T.fwd + T.fwdOff[rk][LW];
+        const float scale = T.txScale[e.txSz];
+        if (lane < TH) {
+            float acc[TW];
+            float *row = S.tmp + lane * TS;
+            if (rk == K_IDT) {
+#pragma unroll
+                for (int o = 0; o < TW; o++) acc[o] = Fr[o * W + o] * row[o];
+            } else {
+#pragma unroll
+                for (int o = 0; o < TW; o++) acc[o] = 0.f;
+#pragma unroll 2
+                for (int k = 0; k < W; k++) {
+                    const float x = row[k];
+                    const float *f = FrT + k * W;
+#pragma unroll
+                    for (int o = 0; o < TW; o++) acc[o] = fmaf(f[o], x, acc[o]);
+                }
+            }
+#pragma unroll
+            for (int o = 0; o < TW; o++) row[o] = acc[o] * scale;
+        }
+    }
+    gsync();
+    // ---- quantization + distortion (virtual lane = physical lane) ----
+    const int den = dq_denom(e.txSz), dsh = den == 4 ? 2 : den == 2 ? 1 : 0;
+    float dist = 0.f, kept = 0.f;
+    for (int idx = lane; idx < N; idx += GW) {
+        const float v = S.tmp[(idx >> LTW) * TS + (idx & (TW - 1))];
+        const int step = idx == 0 ? e.dcQ : e.acQ;
+        const float a = fabsf(v) * (float)den / (float)step;
+        int l = a < e.zbin ? 0 : (int)(a + e.rnd);
+        l = min(l, 16383);
+        const int qv = v < 0 ? -l : l;
+        S.q[idx] = (int16_t)qv;
+        const float dq = l ? (float)dequant(qv, step, dsh, e.bd) : 0.f;
+        const float d = v - dq;
+        const float sq = d * d;
+        dist = dist + sq;
+        kept = fmaf(v, v, kept);
+    }
+    const float distS = gsum(dist);
+    gsync();
+    const uint16_t *scan = scan_of(e.txSz, e.txType);
+    int last = 0;
+    for (int c = lane; c < N; c += GW)
+        if (S.q[scan[c]]) last = c + 1;
+    const int eob = gmaxi(last);
+    float bits = lane == 0 ? base_bits(e, eob) : 0.f;
+    const int cls = tx_class(e.txType);
+    for (int c = lane; c < eob; c += GW) bits += coeff_bits(c, eob, S.q, scan, e.txSz, cls, e.plane > 0, e.dcCtx, *e.C);
+    EvalOut o;
+    o.R = gsum(bits);
+    const float g = T.txg[e.txSz][e.txType];
+    o.D = g * distS;
+    if (W == 64 || H == 64) {
+        const float keptS = gsum(kept);
+        o.D = o.D + fmaxf(0.f, fmaf(-g, keptS, e.resEnergy));
+    }
+    o.eob = eob;
+    return o;
+}
+
+// sum of r^2 over a W x H residual in raster order (32 virtual lanes)
+template <int W, int H>
+__device__ float grp_res_energy(const int16_t *res) {
+    float e = 0.f;
+    for (int p = glane(); p < W * H; p += GW) e += (float)(res[p] * res[p]);
+    return gsum(e);
+}
+
+// ---- intra edge preparation by a group: av1g_edge_prepare's integer arithmetic (spec 7.11.2.4 step 4, 7.11.2.9-12)
+// buf: AboveRow or LeftCol, index 0 = first sample (indices -2 .. 2 (w + h) used)
+template <typename P>
+__device__ void grp_edge_filter(P *buf, int sz, int strength) {
+    if (!strength) return;
+    const int lane = glane();
+    const uint8_t *K = av1t::Intra_Edge_Kernel[strength - 1];
+    int v[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        const int i = lane + 1 + GW * k;
+        int s = 0;
+        if (i < sz) {
+#pragma unroll
+            for (int j = 0; j < 5; j++) s += K[j] * buf[clip3(0, sz - 1, i - 2 + j) - 1];
+        }
+        v[k] = (s + 8) >> 4;
+    }
+    gsync();
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+        if (lane + 1 + GW * k < sz) buf[lane + GW * k] = (P)v[k];
+    gsync();
+}
+template <typename P>
+__device__ void grp_edge_upsample(P *buf, int numPx, int bd) {   // numPx <= 16
+    const int lane = glane();
+    auto dup = [&](int k) -> int { return k == 0 ? buf[-1] : k == numPx + 2 ? buf[numPx - 1] : buf[k - 2]; };
+    int s = 0, e = 0;
+    if (lane < numPx) {
+        s = clip3(0, (1 << bd) - 1, round2(-dup(lane) + 9 * dup(lane + 1) + 9 * dup(lane + 2) - dup(lane + 3), 4));
+        e = dup(lane + 2);
+    }
+    const P c = buf[-1];
+    gsync();
+    if (lane < numPx) { buf[2 * lane - 1] = (P)s; buf[2 * lane] = (P)e; }
+    if (lane == 0) buf[-2] = c;
+    gsync();
+}
+// prepare directional-mode edges in place (A, L are the group's copies); returns the upsampling flags
+template <typename P>
+__device__ void grp_edge_prepare(int pa, int lw, int lh, int ft, int haveL, int haveA, int x, int y, int maxX, int maxY, P *A, P *L,
+                                 int bd, int &upA, int &upL) {
+    const int w = 1 << lw, h = 1 << lh;
+    if (pa != 90 && pa != 180) {
+        if (pa > 90 && pa < 180 && w + h >= 24) {
+            if (glane() == 0) {
+                const P c = (P)round2(L[0] * 5 + A[-1] * 6 + A[0] * 5, 4);
+                L[-1] = c;
+                A[-1] = c;
+            }
+            gsync();
+        }
+        if (haveA) grp_edge_filter(A, min(w, maxX - x + 1) + (pa < 90 ? h : 0) + 1, edge_strength(w, h, ft, pa - 90));
+        if (haveL) grp_edge_filter(L, min(h, maxY - y + 1) + (pa > 180 ? w : 0) + 1, edge_strength(w, h, ft, pa - 180));
+    }
+    upA = edge_upsample(w, h, ft, pa - 90);
+    if (upA) grp_edge_upsample(A, w + (pa < 90 ? h : 0), bd);
+    upL = edge_upsample(w, h, ft, pa - 180);
+    if (upL) grp_edge_upsample(L, h + (pa > 180 ? w : 0), bd);
+}
+
+// build AboveRow / LeftCol [-1 .. w + h - 1] of the tx block at (x, y) of plane (pitch) by the group's lanes
+template <typename SP, typename P>
+__device__ void grp_build_edges(const SP *plane, int pitch, int x, int y, int lw, int lh, int haveL, int haveA, int haveAR, int haveBL,
+                                int maxX, int maxY, P *A, P *L, int bd) {
+    const int w = 1 << lw, h = 1 << lh, n = w + h;
+    const int alim = min(x + (haveAR ? 2 * w : w) - 1, maxX), llim = min(y + (haveBL ? 2 * h : h) - 1, maxY);
+    for (int i = glane(); i < n; i += GW) {
+        P a, l;
+        if (!haveA && haveL) a = plane[(size_t)y * pitch + x - 1];
+        else if (!haveA) a = (P)((1 << (bd - 1)) - 1);
+        else a = plane[(size_t)(y - 1) * pitch + min(x + i, alim)];
+        if (!haveL && haveA) l = plane[(size_t)(y - 1) * pitch + x];
+        else if (!haveL) l = (P)((1 << (bd - 1)) + 1);
+        else l = plane[(size_t)min(y + i, llim) * pitch + x - 1];
+        A[i] = a;
+        L[i] = l;
+    }
+    if (glane() == 0) {
+        P c;
+        if (haveA && haveL) c = plane[(size_t)(y - 1) * pitch + x - 1];
+        else if (haveA) c = plane[(size_t)(y - 1) * pitch + x];
+        else if (haveL) c = plane[(size_t)y * pitch + x - 1];
+        else c = (P)(1 << (bd - 1));
+        A[-1] = c;
+        L[-1] = c;
+    }
+    gsync();
+}
+// DC value (spec 7.11.2.5) with the edge sums over the group
+template <typename P>
+__device__ int grp_dc_value(const P *A, const P *L, int haveL, int haveA, int lw, int lh, int bd) {
+    const int w = 1 << lw, h = 1 << lh;
+    int sa = 0, sl = 0;
+    for (int k = glane(); k < w; k += GW) sa += A[k];
+    for (int k = glane(); k < h; k += GW) sl += L[k];
+    sa = gsumi(sa);
+    sl = gsumi(sl);
+    if (haveL && haveA) return (sa + sl + ((w + h) >> 1)) / (w + h);
+    if (haveL) return clip3(0, (1 << bd) - 1, (sl + (h >> 1)) >> lh);
+    if (haveA) return clip3(0, (1 << bd) - 1, (sa + (w >> 1)) >> lw);
+    return 1 << (bd - 1);
+}
+
+// sum |4x4 Hadamard coefficients| of d
+__device__ __forceinline__ int hadamard4(int *d) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        int *r = d + 4 * i;
+        const int s0 = r[0] + r[1], s1 = r[0] - r[1], s2 = r[2] + r[3], s3 = r[2] - r[3];
+        r[0] = s0 + s2; r[1] = s1 + s3; r[2] = s0 - s2; r[3] = s1 - s3;
+    }
+    int sum = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int s0 = d[j] + d[4 + j], s1 = d[j] - d[4 + j], s2 = d[8 + j] + d[12 + j], s3 = d[8 + j] - d[12 + j];
+        sum += abs(s0 + s2) + abs(s1 + s3) + abs(s0 - s2) + abs(s1 - s3);
+    }
+    return sum;
+}
+
+constexpr int tx_of(int W, int H) {
+    return W == H ? (W == 4 ? TX_4X4 : W == 8 ? TX_8X8 : W == 16 ? TX_16X16 : W == 32 ? TX_32X32 : TX_64X64)
+         : (W == 4 && H == 8) ? TX_4X8 : (W == 8 && H == 4) ? TX_8X4 : (W == 8 && H == 16) ? TX_8X16 : (W == 16 && H == 8) ? TX_16X8
+         : (W == 16 && H == 32) ? TX_16X32 : (W == 32 && H == 16) ? TX_32X16 : (W == 32 && H == 64) ? TX_32X64
+         : (W == 64 && H == 32) ? TX_64X32 : (W == 4 && H == 16) ? TX_4X16 : (W == 16 && H == 4) ? TX_16X4
+         : (W == 8 && H == 32) ? TX_8X32 : (W == 32 && H == 8) ? TX_32X8 : (W == 16 && H == 64) ? TX_16X64 : TX_64X16;
+}
+
+// runtime tx-size dispatch to a template taking <W, H>
+#define AV1M_TX_SWITCH(txSz, M)                                                                                   \
+    switch (txSz) {                                                                                               \
+    case TX_4X4: M(4, 4); break; case TX_8X8: M(8, 8); break; case TX_16X16: M(16, 16); break;                    \
+    case TX_32X32: M(32, 32); break; case TX_64X64: M(64, 64); break; case TX_4X8: M(4, 8); break;                \
+    case TX_8X4: M(8, 4); break; case TX_8X16: M(8, 16); break; case TX_16X8: M(16, 8); break;                    \
+    case TX_16X32: M(16, 32); break; case TX_32X16: M(32, 16); break; case TX_32X64: M(32, 64); break;            \
+    case TX_64X32: M(64, 32); break; case TX_4X16: M(4, 16); break; case TX_16X4: M(16, 4); break;                \
+    case TX_8X32: M(8, 32); break; case TX_32X8: M(32, 8); break; case TX_16X64: M(16, 64); break;                \
+    default: M(64, 16); break;                                                                                    \
+    }
+
+}  // namespace av1m

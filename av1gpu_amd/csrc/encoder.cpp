@@ -1,0 +1,452 @@
+// Host orchestration of one batch of frames on one MI355X (the reference's GpEncoder::init / encode,
+// encoder/src/pipeline.cpp:221-1181, for the plain intra RD path) and the C ABI of include/av1mi.h.
+//
+//   encode: per-image descriptors and shape grids -> one device arena (stream-ordered) -> analysis (19 shape
+//   launches) -> partition DP -> closed loop in SB wavefront steps (step = col + 2 row inside each tile) -> per-SB
+//   counts to the host, prefix sums in decode order -> pack -> records and reconstruction to the host.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <algorithm>
+#include <string>
+#include <vector>
+#include "../../include/av1mi.h"
+#include "av1m_common.h"
+
+namespace av1m {
+void build_tables(DevTables &T);
+void launch_analysis(const Batch &B, const SearchParams &P, const uint32_t *shTotal, int bd, hipStream_t st);
+void launch_dp(const Batch &B, const SearchParams &P, hipStream_t st);
+void launch_recon(const Batch &B, const SearchParams &P, const uint32_t *dSbList, uint32_t n, int bd, hipStream_t st);
+int recon_prepare(int bd);
+size_t recon_smem_bytes();
+void launch_pack(const Batch &B, const PackOff *dOff, RecBlock *oB, uint32_t *oBT, RecTxb *oT, int16_t *oC, hipStream_t st);
+}  // namespace av1m
+
+using namespace av1m;
+
+static_assert(sizeof(av1mi_block) == 16 && sizeof(av1mi_txb) == 16, "DecisionRecord entries are 16 bytes");
+static_assert(sizeof(RecBlock) == 16 && sizeof(RecTxb) == 16, "DecisionRecord entries are 16 bytes");
+static_assert(sizeof(av1mi_params) == sizeof(SearchParams), "av1mi_params mirrors SearchParams");
+
+namespace {
+
+#define HCK(x)                                                                                                 \
+    do {                                                                                                       \
+        const hipError_t e_ = (x);                                                                             \
+        if (e_ != hipSuccess) {                                                                                \
+            snprintf(err, errlen, "HIP: %s at %s (%s:%d)", hipGetErrorString(e_), #x, __FILE__, __LINE__);       \
+            return -1;                                                                                         \
+        }                                                                                                      \
+    } while (0)
+
+int tile_log2(int blk, int target) { int k = 0; while ((blk << k) < target) k++; return k; }
+// minimum legal uniform tiling (spec tile_info; reference av1core/src/encoder.cpp:56-91)
+void tile_layout(int W, int H, int &cols, int &rows, int *cs, int *rs) {
+    const int MiCols = 2 * ((W + 7) >> 3), MiRows = 2 * ((H + 7) >> 3);
+    const int sbCols = (MiCols + 15) >> 4, sbRows = (MiRows + 15) >> 4;
+    const int maxW = 4096 >> 6, maxA = (4096 * 2304) >> 12;
+    const int minC = tile_log2(maxW, sbCols);
+    int minT = std::max(tile_log2(maxA, sbRows * sbCols), minC);
+    const int tws = (sbCols + (1 << minC) - 1) >> minC;
+    cols = 0;
+    for (int s = 0; s < sbCols; s += tws) cs[cols++] = s << 4;
+    cs[cols] = MiCols;
+    for (int rl = std::max(minT - minC, 0);; rl++) {
+        const int ths = (sbRows + (1 << rl) - 1) >> rl;
+        if (tws * ths <= maxA || ths == 1) {
+            rows = 0;
+            for (int s = 0; s < sbRows; s += ths) rs[rows++] = s << 4;
+            rs[rows] = MiRows;
+            return;
+        }
+    }
+}
+
+// stream-ordered device arena of one encode call
+struct Arena {
+    std::vector<std::pair<size_t, void **>> req;
+    size_t total = 0;
+    void *base = nullptr;
+    template <class T> void add(T *&p, size_t n) {
+        req.push_back({total, (void **)&p});
+        total += (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~(size_t)255;
+    }
+    hipError_t alloc(hipStream_t st) {
+        hipError_t e = hipMallocAsync(&base, total, st);
+        if (e != hipSuccess) return e;
+        for (auto &r : req) *r.second = (char *)base + r.first;
+        return hipSuccess;
+    }
+    void release(hipStream_t st) {
+        if (base) hipFreeAsync(base, st);
+        base = nullptr;
+    }
+};
+
+}  // namespace
+
+struct av1mi_encoder {
+    av1mi_config cfg;
+    int device = 0;
+    DevTables *dT = nullptr;
+    hipEvent_t ev[8] = {};
+};
+
+extern "C" {
+
+void av1mi_default_params(av1mi_params *p) {
+    memset(p, 0, sizeof *p);
+    p->lambdaMul = 1.f; p->satdLambda = 1.f;
+    p->rnd = 0.375f; p->zbin = 0.656f; p->rndC = 0.375f; p->zbinC = 0.656f;
+    p->K = 6; p->Krecheck = 6; p->txSearch = 1; p->txSearchCL = 1; p->txAllCL = 1; p->deltas = 1; p->uvDeltas = 1;
+    p->cfl = 1; p->uvK = 6; p->shapeMask = 0x3F1FFF; p->partMask8 = 0xF; p->partMaskN = 0x3FF; p->maxPart = 64;
+    p->uvOverheadBits = 1.5f; p->splitBias = 1.f; p->lambdaCMul = 1.f;
+}
+
+int av1mi_check_partitions(const av1mi_params *p, char *err, int errlen) {
+    // every superblock at every distance from the frame edges keeps a partition of finite J (pipeline.cpp:136-205)
+    for (int FH = 8; FH <= 64; FH += 8)
+        for (int FW = 8; FW <= 64; FW += 8) {
+            auto blk = [&](int bs, int x, int y) {
+                return bs < BLOCK_INVALID && !(bs >= 13 && bs <= 15) && ((p->shapeMask >> bs) & 1) && x < FW && y < FH;
+            };
+            bool fin[85];
+            const int base[4] = {0, 1, 5, 21};
+            for (int L = 3; L >= 0; L--) {
+                const int n = 64 >> L, half = n >> 1, q4 = n >> 2, per = 1 << L;
+                const int sq = n == 8 ? BLOCK_8X8 : n == 16 ? BLOCK_16X16 : n == 32 ? BLOCK_32X32 : BLOCK_64X64;
+                auto sub = [&](int pt) { return (int)av1t::Partition_Subsize[pt][sq]; };
+                const int pm = n == 8 ? p->partMask8 : p->partMaskN;
+                auto on = [&](int pt) { return ((pm >> pt) & 1) != 0; };
+                for (int ny = 0; ny < per; ny++)
+                    for (int nx = 0; nx < per; nx++) {
+                        bool &f = fin[base[L] + ny * per + nx];
+                        const int x = nx * n, y = ny * n;
+                        if (x >= FW || y >= FH) { f = true; continue; }
+                        const bool hasRows = y + half < FH, hasCols = x + half < FW;
+                        bool split = true;
+                        if (n == 8) {
+                            for (int k = 0; k < 4; k++) {
+                                const int xx = x + (k & 1) * 4, yy = y + (k >> 1) * 4;
+                                if (xx < FW && yy < FH) split = split && blk(BLOCK_4X4, xx, yy);
+                            }
+                        } else {
+                            for (int k = 0; k < 4; k++) split = split && fin[base[L + 1] + (ny * 2 + (k >> 1)) * per * 2 + nx * 2 + (k & 1)];
+                        }
+                        f = split;
+                        if (n > p->maxPart) continue;
+                        const int hz = sub(PARTITION_HORZ), vt = sub(PARTITION_VERT);
+                        if (hasRows && hasCols) {
+                            const int sp = sub(PARTITION_SPLIT);
+                            f = f || (on(PARTITION_NONE) && blk(sq, x, y));
+                            f = f || (on(PARTITION_HORZ) && blk(hz, x, y) && blk(hz, x, y + half));
+                            f = f || (on(PARTITION_VERT) && blk(vt, x, y) && blk(vt, x + half, y));
+                            if (n > 8) {
+                                const bool s00 = blk(sp, x, y), s01 = blk(sp, x + half, y), s10 = blk(sp, x, y + half), s11 = blk(sp, x + half, y + half);
+                                f = f || (on(PARTITION_HORZ_A) && s00 && s01 && blk(hz, x, y + half));
+                                f = f || (on(PARTITION_HORZ_B) && blk(hz, x, y) && s10 && s11);
+                                f = f || (on(PARTITION_VERT_A) && s00 && s10 && blk(vt, x + half, y));
+                                f = f || (on(PARTITION_VERT_B) && blk(vt, x, y) && s01 && s11);
+                                bool fh = on(PARTITION_HORZ_4), fv = on(PARTITION_VERT_4);
+                                const int h4 = sub(PARTITION_HORZ_4), v4 = sub(PARTITION_VERT_4);
+                                for (int i = 0; i < 4; i++) {
+                                    if (i < 3 || y + 3 * q4 < FH) fh = fh && blk(h4, x, y + i * q4);
+                                    if (i < 3 || x + 3 * q4 < FW) fv = fv && blk(v4, x + i * q4, y);
+                                }
+                                f = f || fh || fv;
+                            }
+                        } else if (hasCols) {
+                            f = f || blk(hz, x, y);
+                        } else if (hasRows) {
+                            f = f || blk(vt, x, y);
+                        }
+                    }
+            }
+            if (!fin[0]) {
+                snprintf(err, errlen, "the shape / partition options leave some superblock no partition the analysis evaluates");
+                return -1;
+            }
+        }
+    return 0;
+}
+
+int av1mi_create(const av1mi_config *cfg, av1mi_encoder **out, char *err, int errlen) {
+    *out = nullptr;
+    av1mi_config c = *cfg;
+    c.P.K = std::max(1, std::min(c.P.K, MAXK));
+    c.P.Krecheck = std::max(1, std::min(c.P.Krecheck, c.P.K));
+    if (c.qindex < 1 || c.qindex > 255) { snprintf(err, errlen, "qindex %d outside 1..255", c.qindex); return -1; }
+    if (av1mi_check_partitions(&c.P, err, errlen)) return -1;
+    av1mi_encoder *e = new av1mi_encoder();
+    e->cfg = c;
+    hipError_t he = hipGetDevice(&e->device);
+    if (he != hipSuccess) { snprintf(err, errlen, "HIP: %s (no usable GPU)", hipGetErrorString(he)); delete e; return -1; }
+    DevTables *h = (DevTables *)malloc(sizeof(DevTables));
+    build_tables(*h);
+    he = hipMalloc(&e->dT, sizeof(DevTables));
+    if (he == hipSuccess) he = hipMemcpy(e->dT, h, sizeof(DevTables), hipMemcpyHostToDevice);
+    free(h);
+    if (he != hipSuccess) { snprintf(err, errlen, "HIP: %s (table upload)", hipGetErrorString(he)); delete e; return -1; }
+    if (recon_prepare(8) || recon_prepare(10)) { snprintf(err, errlen, "closed-loop kernel needs %zu bytes of LDS", recon_smem_bytes()); delete e; return -1; }
+    for (auto &v : e->ev) hipEventCreate(&v);
+    // keep freed stream-ordered allocations cached between batches
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, e->device) == hipSuccess) {
+        uint64_t thr = UINT64_MAX;
+        hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+    *out = e;
+    return 0;
+}
+
+void av1mi_destroy(av1mi_encoder *e) {
+    if (!e) return;
+    if (e->dT) hipFree(e->dT);
+    for (auto &v : e->ev) if (v) hipEventDestroy(v);
+    delete e;
+}
+
+void av1mi_get_config(const av1mi_encoder *e, av1mi_config *cfg) { *cfg = e->cfg; }
+
+int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_frame_out *out, av1mi_times *t, void *stream, int wantRecon,
+                 char *err, int errlen) {
+    hipStream_t st = (hipStream_t)stream;
+    av1mi_times tm;
+    memset(&tm, 0, sizeof tm);
+    for (int i = 0; i < n; i++) memset(&out[i], 0, sizeof out[i]);
+    if (n <= 0) { if (t) *t = tm; return 0; }
+    const SearchParams P = *(const SearchParams *)&enc->cfg.P;
+    const int bd = in[0].bitDepth == 10 ? 10 : 8, bps = bd == 8 ? 1 : 2;
+    hipEventRecord(enc->ev[0], st);
+    std::vector<Img> imgs(n);
+    uint32_t shTotal[NSHAPES] = {0};
+    std::vector<uint32_t> shStart((size_t)NSHAPES * (n + 1), 0), sbImg;
+    uint32_t nSb = 0;
+    for (int i = 0; i < n; i++) {
+        const av1mi_frame_in &f = in[i];
+        if ((f.bitDepth == 10 ? 10 : 8) != bd) { snprintf(err, errlen, "frames of different bit depths in one batch"); return -1; }
+        if (f.yuv444 && !f.mono) { snprintf(err, errlen, "4:4:4 frames are not supported by this build"); return -1; }
+        if (f.W <= 0 || f.H <= 0 || f.W > 65536 || f.H > 65536) { snprintf(err, errlen, "frame %d: bad size %dx%d", i, f.W, f.H); return -1; }
+        Img &I = imgs[i];
+        memset(&I, 0, sizeof I);
+        I.W = f.W; I.H = f.H;
+        I.MiCols = 2 * ((f.W + 7) >> 3); I.MiRows = 2 * ((f.H + 7) >> 3);
+        I.sbCols = (I.MiCols + 15) >> 4; I.sbRows = (I.MiRows + 15) >> 4;
+        if (f.aw < I.sbCols * 64 || f.ah < I.sbRows * 64) { snprintf(err, errlen, "frame %d: input planes not padded to 64", i); return -1; }
+        I.aw = f.aw; I.ah = f.ah;
+        I.ss = 1; I.mono = f.mono; I.bd = bd;
+        I.qindex = f.qindex > 0 ? f.qindex : enc->cfg.qindex;
+        if (I.qindex < 1 || I.qindex > 255) { snprintf(err, errlen, "frame %d: qindex %d outside 1..255", i, I.qindex); return -1; }
+        I.qctx = I.qindex <= 20 ? 0 : I.qindex <= 60 ? 1 : I.qindex <= 120 ? 2 : 3;
+        I.edgeFilter = enc->cfg.edgeFilter ? 1 : 0;
+        tile_layout(f.W, f.H, I.tileCols, I.tileRows, I.tileColStart, I.tileRowStart);
+        for (int p = 0; p < 3; p++) { I.src[p] = f.dsrc[p]; I.spitch[p] = f.spitch[p]; }
+        I.sbBase = nSb;
+        nSb += (uint32_t)(I.sbCols * I.sbRows);
+        for (int k = 0; k < I.sbCols * I.sbRows; k++) sbImg.push_back((uint32_t)i);
+    }
+    uint64_t blkTotal = 0;
+    for (int s = 0; s < NSHAPES; s++) {
+        const uint64_t shapeBase = blkTotal;
+        const bool en = s < BLOCK_INVALID && !(s >= 13 && s <= 15) && ((P.shapeMask >> s) & 1);
+        for (int i = 0; i < n; i++) {
+            Img &I = imgs[i];
+            shStart[(size_t)s * (n + 1) + i] = shTotal[s];
+            if (en) {
+                const int w = 4 * bw4(s), h = 4 * bh4(s);
+                I.shNbx[s] = (I.MiCols * 4 + w - 1) / w;
+                I.shNby[s] = (I.MiRows * 4 + h - 1) / h;
+            }
+            I.shOff[s] = (uint32_t)(shapeBase + shTotal[s]);
+            shTotal[s] += (uint32_t)(I.shNbx[s] * I.shNby[s]);
+        }
+        shStart[(size_t)s * (n + 1) + n] = shTotal[s];
+        blkTotal += shTotal[s];
+    }
+    // ---- device arena ----
+    Arena A;
+    std::vector<uint8_t *> recMem(n), mapMem(n);
+    size_t mapTotal = 0;
+    for (int i = 0; i < n; i++) {
+        const Img &I = imgs[i];
+        A.add(recMem[i], ((size_t)I.aw * I.ah + 2 * (size_t)(I.aw >> 1) * (I.ah >> 1)) * bps);
+        const size_t mi = (size_t)I.MiCols * I.MiRows, c1 = (size_t)(I.MiCols >> 1) * (I.MiRows >> 1);
+        A.add(mapMem[i], 5 * mi + 2 * (mi + 2 * c1));
+        mapTotal += 5 * mi + 2 * (mi + 2 * c1);
+    }
+    Img *dImg; BlockRes *dBlk; uint32_t *dShStart, *dSbImg, *dStats; uint16_t *dLeafCount; Leaf *dLeaves; float *dSbJ; SbOut *dSbOut;
+    RecBlock *dOB; uint16_t *dOBT; RecTxb *dOT; int16_t *dOC;
+    A.add(dImg, n); A.add(dBlk, blkTotal); A.add(dShStart, shStart.size()); A.add(dSbImg, nSb); A.add(dLeafCount, nSb);
+    A.add(dLeaves, (size_t)nSb * MAX_LEAVES); A.add(dSbJ, nSb); A.add(dSbOut, nSb); A.add(dOB, (size_t)nSb * MAX_LEAVES);
+    A.add(dOBT, (size_t)nSb * MAX_LEAVES); A.add(dOT, (size_t)nSb * SB_TXB_CAP); A.add(dOC, (size_t)nSb * SB_COEF_CAP); A.add(dStats, 8);
+    // wavefront step lists: per tile, step = (c - c0) + 2 (r - r0)
+    std::vector<std::vector<uint32_t>> steps;
+    for (int i = 0; i < n; i++) {
+        const Img &I = imgs[i];
+        for (int tr = 0; tr < I.tileRows; tr++)
+            for (int tc = 0; tc < I.tileCols; tc++) {
+                const int r0 = I.tileRowStart[tr] >> 4, r1 = (I.tileRowStart[tr + 1] + 15) >> 4;
+                const int c0 = I.tileColStart[tc] >> 4, c1 = (I.tileColStart[tc + 1] + 15) >> 4;
+                for (int r = r0; r < r1; r++)
+                    for (int c = c0; c < c1; c++) {
+                        const size_t s = (size_t)(c - c0) + 2 * (size_t)(r - r0);
+                        if (steps.size() <= s) steps.resize(s + 1);
+                        steps[s].push_back(I.sbBase + (uint32_t)(r * I.sbCols + c));
+                    }
+            }
+    }
+    std::vector<uint32_t> stepFlat, stepOff;
+    for (auto &v : steps) { stepOff.push_back((uint32_t)stepFlat.size()); stepFlat.insert(stepFlat.end(), v.begin(), v.end()); }
+    stepOff.push_back((uint32_t)stepFlat.size());
+    uint32_t *dSteps;
+    PackOff *dPO;
+    A.add(dSteps, stepFlat.size()); A.add(dPO, nSb);
+    HCK(A.alloc(st));
+    tm.deviceBytes = A.total;
+    for (int i = 0; i < n; i++) {
+        Img &I = imgs[i];
+        const size_t ysz = (size_t)I.aw * I.ah * bps, csz = (size_t)(I.aw >> 1) * (I.ah >> 1) * bps;
+        I.rec[0] = recMem[i]; I.rec[1] = recMem[i] + ysz; I.rec[2] = recMem[i] + ysz + csz;
+        I.pitch[0] = I.aw; I.pitch[1] = I.pitch[2] = I.aw >> 1;
+        const size_t mi = (size_t)I.MiCols * I.MiRows, c1 = (size_t)(I.MiCols >> 1) * (I.MiRows >> 1);
+        uint8_t *m = mapMem[i];
+        I.mY = m; I.mUV = m + mi; I.mSkip = m + 2 * mi; I.mMiSize = m + 3 * mi; I.mTx = m + 4 * mi;
+        uint8_t *c = m + 5 * mi;
+        I.cLvl[0] = c; I.cLvl[1] = c + mi; I.cLvl[2] = c + mi + c1;
+        I.cDc[0] = c + mi + 2 * c1; I.cDc[1] = c + 2 * mi + 2 * c1; I.cDc[2] = c + 2 * mi + 3 * c1;
+        HCK(hipMemsetAsync(m, 0, 5 * mi + 2 * (mi + 2 * c1), st));
+    }
+    HCK(hipMemsetAsync(dStats, 0, 8 * 4, st));
+    HCK(hipMemcpyAsync(dImg, imgs.data(), n * sizeof(Img), hipMemcpyHostToDevice, st));
+    HCK(hipMemcpyAsync(dShStart, shStart.data(), shStart.size() * 4, hipMemcpyHostToDevice, st));
+    HCK(hipMemcpyAsync(dSbImg, sbImg.data(), (size_t)nSb * 4, hipMemcpyHostToDevice, st));
+    HCK(hipMemcpyAsync(dSteps, stepFlat.data(), stepFlat.size() * 4, hipMemcpyHostToDevice, st));
+    Batch B;
+    memset(&B, 0, sizeof B);
+    B.nimg = n; B.img = dImg; B.T = enc->dT; B.blk = dBlk; B.shStart = dShStart; B.nSb = nSb; B.sbImg = dSbImg;
+    B.leafCount = dLeafCount; B.leaves = dLeaves; B.sbJ = dSbJ; B.sbOut = dSbOut; B.oBlocks = dOB; B.oBlockTxb = dOBT;
+    B.oTxb = dOT; B.oCoeffs = dOC; B.stats = dStats;
+    hipEventRecord(enc->ev[1], st);
+    launch_analysis(B, P, shTotal, bd, st);
+    HCK(hipGetLastError());
+    hipEventRecord(enc->ev[2], st);
+    launch_dp(B, P, st);
+    HCK(hipGetLastError());
+    hipEventRecord(enc->ev[3], st);
+    for (size_t s = 0; s + 1 < stepOff.size(); s++) launch_recon(B, P, dSteps + stepOff[s], stepOff[s + 1] - stepOff[s], bd, st);
+    HCK(hipGetLastError());
+    hipEventRecord(enc->ev[4], st);
+    tm.steps = (uint32_t)steps.size();
+    tm.sbs = nSb;
+    // ---- pack: prefix sums in decode order (tiles raster, SBs raster inside each tile) ----
+    std::vector<SbOut> so(nSb);
+    uint32_t stats[8];
+    HCK(hipMemcpyAsync(so.data(), dSbOut, (size_t)nSb * sizeof(SbOut), hipMemcpyDeviceToHost, st));
+    HCK(hipMemcpyAsync(stats, dStats, sizeof stats, hipMemcpyDeviceToHost, st));
+    HCK(hipStreamSynchronize(st));
+    std::vector<PackOff> po(nSb);
+    std::vector<uint64_t> imgBlk(n + 1), imgTxb(n + 1), imgCoef(n + 1);
+    uint64_t gBlk = 0, gTxb = 0, gCoef = 0;
+    for (int i = 0; i < n; i++) {
+        const Img &I = imgs[i];
+        imgBlk[i] = gBlk; imgTxb[i] = gTxb; imgCoef[i] = gCoef;
+        uint32_t rTxb = 0, rCoef = 0;
+        for (int tr = 0; tr < I.tileRows; tr++)
+            for (int tc = 0; tc < I.tileCols; tc++)
+                for (int r = I.tileRowStart[tr]; r < I.tileRowStart[tr + 1]; r += 16)
+                    for (int c = I.tileColStart[tc]; c < I.tileColStart[tc + 1]; c += 16) {
+                        const uint32_t sb = I.sbBase + (uint32_t)((r >> 4) * I.sbCols + (c >> 4));
+                        PackOff &o = po[sb];
+                        o.gBlk = (uint32_t)gBlk; o.gTxb = (uint32_t)gTxb; o.gCoef = gCoef; o.rTxb = rTxb; o.rCoef = rCoef;
+                        gBlk += so[sb].nBlocks; gTxb += so[sb].nTxb; gCoef += so[sb].nCoeffs;
+                        rTxb += so[sb].nTxb; rCoef += so[sb].nCoeffs;
+                    }
+    }
+    imgBlk[n] = gBlk; imgTxb[n] = gTxb; imgCoef[n] = gCoef;
+    Arena R;
+    RecBlock *rB; uint32_t *rBT; RecTxb *rT; int16_t *rC;
+    R.add(rB, gBlk); R.add(rBT, gBlk); R.add(rT, gTxb); R.add(rC, gCoef);
+    HCK(R.alloc(st));
+    HCK(hipMemcpyAsync(dPO, po.data(), (size_t)nSb * sizeof(PackOff), hipMemcpyHostToDevice, st));
+    hipEventRecord(enc->ev[5], st);
+    launch_pack(B, dPO, rB, rBT, rT, rC, st);
+    HCK(hipGetLastError());
+    hipEventRecord(enc->ev[6], st);
+    for (int i = 0; i < n; i++) {
+        const Img &I = imgs[i];
+        av1mi_frame_out &o = out[i];
+        o.width = I.W; o.height = I.H; o.base_q_idx = I.qindex; o.bit_depth = bd; o.chroma_format = 0; o.monochrome = I.mono;
+        o.n_blocks = (uint32_t)(imgBlk[i + 1] - imgBlk[i]);
+        o.n_txb = (uint32_t)(imgTxb[i + 1] - imgTxb[i]);
+        o.n_coeffs = imgCoef[i + 1] - imgCoef[i];
+        o.blocks = (av1mi_block *)malloc(std::max<size_t>(o.n_blocks, 1) * 16);
+        o.block_txb = (uint32_t *)malloc(((size_t)o.n_blocks + 1) * 4);
+        o.txb = (av1mi_txb *)malloc(std::max<size_t>(o.n_txb, 1) * 16);
+        o.coeffs = (int16_t *)malloc(std::max<uint64_t>(o.n_coeffs, 1) * 2);
+        o.aw = I.aw; o.ah = I.ah; o.tile_cols = I.tileCols; o.tile_rows = I.tileRows;
+        HCK(hipMemcpyAsync(o.blocks, rB + imgBlk[i], (size_t)o.n_blocks * 16, hipMemcpyDeviceToHost, st));
+        HCK(hipMemcpyAsync(o.block_txb, rBT + imgBlk[i], (size_t)o.n_blocks * 4, hipMemcpyDeviceToHost, st));
+        HCK(hipMemcpyAsync(o.txb, rT + imgTxb[i], (size_t)o.n_txb * 16, hipMemcpyDeviceToHost, st));
+        HCK(hipMemcpyAsync(o.coeffs, rC + imgCoef[i], (size_t)o.n_coeffs * 2, hipMemcpyDeviceToHost, st));
+        if (wantRecon) {
+            for (int p = 0; p < 3; p++) {
+                const int pw = p ? I.aw >> 1 : I.aw, ph = p ? I.ah >> 1 : I.ah;
+                o.recon[p] = malloc((size_t)pw * ph * bps);
+                o.recon_pitch[p] = pw;
+                HCK(hipMemcpyAsync(o.recon[p], I.rec[p], (size_t)pw * ph * bps, hipMemcpyDeviceToHost, st));
+            }
+        }
+    }
+    hipEventRecord(enc->ev[7], st);
+    HCK(hipStreamSynchronize(st));
+    for (int i = 0; i < n; i++) out[i].block_txb[out[i].n_blocks] = out[i].n_txb;
+    R.release(st);
+    A.release(st);
+    float ms[7];
+    for (int k = 0; k < 7; k++) hipEventElapsedTime(&ms[k], enc->ev[k], enc->ev[k + 1]);
+    tm.setup = ms[0]; tm.analysis = ms[1]; tm.dp = ms[2]; tm.recon = ms[3]; tm.pack = ms[5]; tm.download = ms[6] + ms[4];
+    hipEventElapsedTime(&tm.total, enc->ev[0], enc->ev[7]);
+    tm.blocks = stats[0]; tm.skipBlocks = stats[1]; tm.cflBlocks = stats[2]; tm.rangeFixes = stats[3];
+    if (t) *t = tm;
+    return 0;
+}
+
+void av1mi_frame_out_free(av1mi_frame_out *o) {
+    if (!o) return;
+    free(o->blocks); free(o->block_txb); free(o->txb); free(o->coeffs);
+    for (int p = 0; p < 3; p++) free(o->recon[p]);
+    memset(o, 0, sizeof *o);
+}
+
+int av1mi_dev_alloc(void **ptr, size_t bytes) { return hipMalloc(ptr, bytes) == hipSuccess ? 0 : -1; }
+int av1mi_dev_free(void *ptr) { return hipFree(ptr) == hipSuccess ? 0 : -1; }
+int av1mi_copy_h2d(void *dst, const void *src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : -1; }
+int av1mi_device_sync(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : -1; }
+
+int av1mi_fwd_matrix(int kind, int ln, float *out) {
+    static DevTables *T = nullptr;
+    if (!T) { T = (DevTables *)malloc(sizeof(DevTables)); build_tables(*T); }
+    if (kind < 0 || kind > 2 || ln < 2 || ln > 6 || (kind == K_ADST && ln > 4) || (kind == K_IDT && ln > 5)) return 0;
+    const int nn = 1 << (2 * ln);
+    memcpy(out, T->fwd + T->fwdOff[kind][ln], nn * 4);
+    return nn;
+}
+void av1mi_misc_tables(float *txg, float *txScale, float *avgY, float *avgPart, float *avgSkip0, float *lambda, float *lamS) {
+    DevTables *T = (DevTables *)malloc(sizeof(DevTables));
+    build_tables(*T);
+    memcpy(txg, T->txg, sizeof T->txg); memcpy(txScale, T->txScale, sizeof T->txScale); memcpy(avgY, T->avgYMode, sizeof T->avgYMode);
+    memcpy(avgPart, T->avgPart, sizeof T->avgPart); *avgSkip0 = T->avgSkip0;
+    memcpy(lambda, T->lambda, sizeof T->lambda); memcpy(lamS, T->lamS, sizeof T->lamS);
+    free(T);
+}
+int av1mi_cost_table(int qctx, float *out) {
+    DevTables *T = (DevTables *)malloc(sizeof(DevTables));
+    build_tables(*T);
+    memcpy(out, &T->cost[qctx & 3], sizeof(CostTab));
+    free(T);
+    return (int)(sizeof(CostTab) / 4);
+}
+
+}  // extern "C"

@@ -1,0 +1,367 @@
+// Open-loop luma analysis and per-superblock partition DP.
+//
+// k_analysis<W, H>: one 32-lane group per (image, block position) of one shape, NG groups per workgroup.  Per block:
+// edges from the SOURCE with Z-order availability, SATD screening of the 13 modes (integer totals through LDS atomics:
+// order free), angle deltas +-1..3 around the two best directional modes, top-K by SATD + lambda_s * mode bits, exact
+// RD (forward transform, quantizer, coefficient rate) of the top-K with DCT_DCT, then the other tx types of the set for
+// the winner.  Output: BlockRes {J, candidates best first, tx type}.  Reference: encoder/src/k_analysis.cuh:496-689.
+//
+// k_dp: one thread per 64x64 superblock: bottom-up DP over the 85 square nodes and 10 partition types with the
+// averaged partition bits, then the leaves in decode order.  Reference: encoder/src/k_dp.cuh:72-297.
+#include "dev_eval.h"
+
+namespace av1m {
+
+template <int W, int H>
+struct AnScr {
+    EvalScr<W, H> ev;
+    uint16_t ea[EDGE_LEN], el[EDGE_LEN];   // source edges
+    uint16_t pa[EDGE_LEN], pl[EDGE_LEN];   // edge-filtered copies of the candidate under evaluation
+    int satdI[32];
+    int cand[32];
+};
+
+constexpr int an_groups(int W, int H) { return W * H <= 256 ? 8 : W * H <= 1024 ? 4 : 2; }
+
+struct TileBox { int x0, y0, x1, y1; };
+__device__ __forceinline__ TileBox tile_box(const Img &I, int x, int y) {
+    int c = 0, r = 0;
+    while (c + 1 < I.tileCols && x >= I.tileColStart[c + 1] * 4) c++;
+    while (r + 1 < I.tileRows && y >= I.tileRowStart[r + 1] * 4) r++;
+    return TileBox{I.tileColStart[c] * 4, I.tileRowStart[r] * 4, I.tileColStart[c + 1] * 4, I.tileRowStart[r + 1] * 4};
+}
+__device__ __forceinline__ int zorder16(int x, int y) {
+    int z = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) z |= (((x >> b) & 1) << (2 * b)) | (((y >> b) & 1) << (2 * b + 1));
+    return z;
+}
+// open-loop guess that luma sample (px, py) is decoded before the block at (x, y): the SB row above and the SB to the
+// left as the spec's BlockDecoded initialisation, Z-order inside the SB (dev_common.cuh:151-163)
+__device__ __forceinline__ int ol_decoded(int px, int py, int x, int y, const TileBox &t) {
+    if (px < t.x0 || py < t.y0 || px >= t.x1 || py >= t.y1) return 0;
+    const int sbx = x & ~63, sby = y & ~63;
+    if (py < sby) return 1;
+    if (px < sbx) return py < sby + 64;
+    if (px >= sbx + 64 || py >= sby + 64) return 0;
+    return zorder16(((px - sbx) >> 2) & 15, ((py - sby) >> 2) & 15) < zorder16(((x - sbx) >> 2) & 15, ((y - sby) >> 2) & 15);
+}
+
+template <int W, int H, int BS, typename Px>
+__global__ void __launch_bounds__(an_groups(W, H) * 32) k_analysis(Batch B, SearchParams P, uint32_t nTotal) {
+    constexpr int NG = an_groups(W, H), TXS = tx_of(W, H);
+    constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
+    constexpr int NSUB = (W / 4) * (H / 4), NSX = W / 4;
+    __shared__ AnScr<W, H> sm[NG];
+    AnScr<W, H> &S = sm[gid()];
+    const DevTables &T = *B.T;
+    const int lane = glane();
+    const uint32_t *starts = B.shStart + (size_t)BS * (B.nimg + 1);
+    const int set = tx_set_intra(TXS);
+    for (uint32_t gb = blockIdx.x * NG + gid(); gb < nTotal; gb += gridDim.x * NG) {
+        int lo = 0, hi = B.nimg - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (starts[mid] <= gb) lo = mid; else hi = mid - 1;
+        }
+        const Img &I = B.img[lo];
+        const uint32_t li = gb - starts[lo];
+        const int nbx = I.shNbx[BS];
+        const int x = (int)(li % nbx) * W, y = (int)(li / nbx) * H;
+        BlockRes &out = B.blk[I.shOff[BS] + li];
+        if (x >= I.MiCols * 4 || y >= I.MiRows * 4) {
+            if (lane == 0) { out.J = INF; out.ncand = 0; }
+            continue;
+        }
+        const int bd = I.bd;
+        const TileBox tb = tile_box(I, x, y);
+        const int haveL = x > tb.x0, haveA = y > tb.y0;
+        const int haveAR = ol_decoded(x + W, y - 1, x, y, tb), haveBL = ol_decoded(x - 1, y + H, x, y, tb);
+        const int maxX = I.MiCols * 4 - 1, maxY = I.MiRows * 4 - 1;
+        const int sp = I.spitch[0];
+        const Px *plane = (const Px *)I.src[0];
+        const Px *src = plane + (size_t)y * sp + x;
+        const float lambda = T.lambda[bd == 10][I.qindex] * P.lambdaMul, lamS = P.satdLambda * T.lamS[bd == 10][I.qindex];
+        const CostTab &C = T.cost[I.qctx];
+        uint16_t *A = S.ea + EDGE_OFF, *L = S.el + EDGE_OFF;
+        grp_build_edges(plane, sp, x, y, LW, LH, haveL, haveA, haveAR, haveBL, maxX, maxY, A, L, bd);
+        const int dcv = grp_dc_value(A, L, haveL, haveA, LW, LH, bd);
+        auto mbits = [&](int c) -> float {
+            const int m = cand_mode(c);
+            float b = T.avgYMode[m];
+            if (BS >= BLOCK_8X8 && is_dir(m)) b += C.angle_delta[m - V_PRED][cand_delta(c) + 3];
+            return b;
+        };
+        // ---- SATD screening (unfiltered edges) ----
+        S.satdI[lane] = 0;
+        S.cand[lane] = lane < 13 ? cand_pack(lane, 0) : 0;
+        gsync();
+        auto screen = [&](int c0, int c1) {
+            const int items = (c1 - c0) * NSUB;
+            for (int it = lane; it < items; it += GW) {
+                const int ci = c0 + it / NSUB, sb = it % NSUB, sx = (sb % NSX) * 4, sy = (sb / NSX) * 4;
+                const int c = S.cand[ci], m = cand_mode(c), pa = pangle(m, cand_delta(c));
+                int d[16];
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int p = m == DC_PRED ? dcv : pred_px(m, pa, sy + i, sx + j, LW, LH, A, L, 0, 0);
+                        d[i * 4 + j] = (int)src[(sy + i) * sp + sx + j] - p;
+                    }
+                atomicAdd(&S.satdI[ci], hadamard4(d));
+            }
+            gsync();
+        };
+        screen(0, 13);
+        float js = lane < 13 ? fmaf(lamS, mbits(S.cand[lane]), (float)S.satdI[lane] * 0.25f) : INF;
+        if (P.deltas && BS >= BLOCK_8X8) {
+            const float jd = (lane >= V_PRED && lane <= D67_PRED) ? js : INF;
+            const int b0 = gargmin(jd, lane);
+            const int b1 = gargmin(lane == b0 ? INF : jd, lane);
+            if (lane < 12) {
+                const int k = lane % 6;
+                S.cand[13 + lane] = cand_pack(lane < 6 ? b0 : b1, k < 3 ? -(k + 1) : k - 2);
+            }
+            gsync();
+            screen(13, 25);
+            if (lane >= 13 && lane < 25) js = fmaf(lamS, mbits(S.cand[lane]), (float)S.satdI[lane] * 0.25f);
+        }
+        const int K = min(P.K, MAXK);
+        int top[MAXK];
+        int nK = 0;
+        for (int k = 0; k < K; k++) {
+            top[k] = gargmin(js, lane);
+            if (top[k] < 0) break;
+            if (lane == top[k]) js = INF;
+            nK++;
+        }
+        // ---- RD of the top-K (DCT_DCT), then the tx types of the set for the winner ----
+        const int hasC = has_chroma(BS, y >> 2, x >> 2, I.mono, I.ss);
+        const float overhead = (hasC ? P.uvOverheadBits : 0.f) + T.avgSkip0;
+        EvalIn e;
+        e.txSz = TXS; e.plane = 0; e.dcQ = dc_q(I.qindex, bd); e.acQ = ac_q(I.qindex, bd); e.skipCtx = 0; e.dcCtx = 0;
+        e.bd = bd; e.rnd = P.rnd; e.zbin = P.zbin; e.resEnergy = 0.f; e.C = &C; e.intraDir = 0; e.txType = 0;
+        float bestJ = INF;
+        int bestK = 0, bestTx = DCT_DCT, resK = -1;
+        const int nSteps = nK + ((P.txSearch && set != TX_SET_DCTONLY) ? 15 : 0);
+        for (int st = 0; st < nSteps; st++) {
+            const int k = st < nK ? st : bestK, t = st < nK ? DCT_DCT : st - nK + 1;
+            if (t != DCT_DCT && !tx_in_set(set, t)) continue;
+            const int c = S.cand[top[k]], m = cand_mode(c), pa = pangle(m, cand_delta(c));
+            if (k != resK) {
+                const uint16_t *EA = A, *EL = L;
+                int upA = 0, upL = 0;
+                if (I.edgeFilter && is_dir(m)) {
+                    for (int i = lane; i < EDGE_LEN; i += GW) { S.pa[i] = S.ea[i]; S.pl[i] = S.el[i]; }
+                    gsync();
+                    grp_edge_prepare(pa, LW, LH, 0, haveL, haveA, x, y, maxX, maxY, S.pa + EDGE_OFF, S.pl + EDGE_OFF, bd, upA, upL);
+                    EA = S.pa + EDGE_OFF;
+                    EL = S.pl + EDGE_OFF;
+                }
+                for (int p = lane; p < W * H; p += GW) {
+                    const int i = p >> LW, j = p & (W - 1);
+                    const int pr = m == DC_PRED ? dcv : pred_px(m, pa, i, j, LW, LH, EA, EL, upA, upL);
+                    S.ev.res[p] = (int16_t)((int)src[i * sp + j] - pr);
+                }
+                gsync();
+                e.resEnergy = (W == 64 || H == 64) ? grp_res_energy<W, H>(S.ev.res) : 0.f;
+                resK = k;
+            }
+            e.txType = t;
+            e.intraDir = m;
+            const EvalOut o = grp_eval<W, H>(S.ev, e, T);
+            const float J = fmaf(lambda, (o.R + mbits(c)) + overhead, o.D);
+            if (J < bestJ) {
+                bestJ = J;
+                if (st < nK) bestK = k; else bestTx = t;
+            }
+            gsync();
+        }
+        if (lane == 0) {
+            out.J = nK ? bestJ : INF;
+            out.tx = (uint8_t)bestTx;
+            int n = 0;
+            if (nK) out.cand[n++] = (uint8_t)S.cand[top[bestK]];
+            for (int k = 0; k < nK; k++)
+                if (k != bestK) out.cand[n++] = (uint8_t)S.cand[top[k]];
+            out.ncand = (uint8_t)n;
+        }
+        gsync();
+    }
+}
+
+template <int W, int H, int BS>
+static void launch_an(const Batch &B, const SearchParams &P, uint32_t n, int bd, hipStream_t st) {
+    if (!n) return;
+    constexpr int NG = an_groups(W, H);
+    uint32_t grid = (n + NG - 1) / NG;
+    if (grid > (1u << 30)) grid = 1u << 30;
+    if (bd == 10) hipLaunchKernelGGL((k_analysis<W, H, BS, uint16_t>), dim3(grid), dim3(NG * 32), 0, st, B, P, n);
+    else hipLaunchKernelGGL((k_analysis<W, H, BS, uint8_t>), dim3(grid), dim3(NG * 32), 0, st, B, P, n);
+}
+
+// all shapes of the batch (shTotal[s]: blocks of shape s over the batch; 0 = not analysed), largest first
+void launch_analysis(const Batch &B, const SearchParams &P, const uint32_t *shTotal, int bd, hipStream_t st) {
+    launch_an<64, 64, BLOCK_64X64>(B, P, shTotal[BLOCK_64X64], bd, st);
+    launch_an<64, 32, BLOCK_64X32>(B, P, shTotal[BLOCK_64X32], bd, st);
+    launch_an<32, 64, BLOCK_32X64>(B, P, shTotal[BLOCK_32X64], bd, st);
+    launch_an<64, 16, BLOCK_64X16>(B, P, shTotal[BLOCK_64X16], bd, st);
+    launch_an<16, 64, BLOCK_16X64>(B, P, shTotal[BLOCK_16X64], bd, st);
+    launch_an<32, 32, BLOCK_32X32>(B, P, shTotal[BLOCK_32X32], bd, st);
+    launch_an<32, 16, BLOCK_32X16>(B, P, shTotal[BLOCK_32X16], bd, st);
+    launch_an<16, 32, BLOCK_16X32>(B, P, shTotal[BLOCK_16X32], bd, st);
+    launch_an<32, 8, BLOCK_32X8>(B, P, shTotal[BLOCK_32X8], bd, st);
+    launch_an<8, 32, BLOCK_8X32>(B, P, shTotal[BLOCK_8X32], bd, st);
+    launch_an<16, 16, BLOCK_16X16>(B, P, shTotal[BLOCK_16X16], bd, st);
+    launch_an<16, 8, BLOCK_16X8>(B, P, shTotal[BLOCK_16X8], bd, st);
+    launch_an<8, 16, BLOCK_8X16>(B, P, shTotal[BLOCK_8X16], bd, st);
+    launch_an<16, 4, BLOCK_16X4>(B, P, shTotal[BLOCK_16X4], bd, st);
+    launch_an<4, 16, BLOCK_4X16>(B, P, shTotal[BLOCK_4X16], bd, st);
+    launch_an<8, 8, BLOCK_8X8>(B, P, shTotal[BLOCK_8X8], bd, st);
+    launch_an<8, 4, BLOCK_8X4>(B, P, shTotal[BLOCK_8X4], bd, st);
+    launch_an<4, 8, BLOCK_4X8>(B, P, shTotal[BLOCK_4X8], bd, st);
+    launch_an<4, 4, BLOCK_4X4>(B, P, shTotal[BLOCK_4X4], bd, st);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// partition DP (one thread per SB)
+__device__ __forceinline__ int sq_bsize(int n) {
+    return n == 4 ? BLOCK_4X4 : n == 8 ? BLOCK_8X8 : n == 16 ? BLOCK_16X16 : n == 32 ? BLOCK_32X32 : BLOCK_64X64;
+}
+__device__ __forceinline__ int sub_bsize(int p, int n) { return av1t::Partition_Subsize[p][sq_bsize(n)]; }
+
+__global__ void __launch_bounds__(64) k_dp(Batch B, SearchParams P) {
+    const uint32_t sb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sb >= B.nSb) return;
+    const Img &I = B.img[B.sbImg[sb]];
+    const DevTables &T = *B.T;
+    const uint32_t local = sb - I.sbBase;
+    const int sbx = (int)(local % I.sbCols) * 64, sby = (int)(local / I.sbCols) * 64;
+    const int FW = I.MiCols * 4, FH = I.MiRows * 4;
+    const float lambda = T.lambda[I.bd == 10][I.qindex] * P.lambdaMul;
+    auto blockJ = [&](int bs, int x, int y) -> float {
+        if (bs >= BLOCK_INVALID || I.shNbx[bs] == 0) return INF;
+        const int w = 4 * bw4(bs), h = 4 * bh4(bs);
+        return B.blk[I.shOff[bs] + (uint32_t)(y / h) * I.shNbx[bs] + x / w].J;
+    };
+    float nJ[85];
+    int8_t nP[85];
+    const int lvBase[4] = {0, 1, 5, 21};
+    for (int Lv = 3; Lv >= 0; Lv--) {
+        const int n = 64 >> Lv, half = n >> 1, q4 = n >> 2, per = 1 << Lv;
+        const int bsl = Lv == 3 ? 1 : Lv == 2 ? 2 : Lv == 1 ? 3 : 4;
+        const int pmask = n == 8 ? P.partMask8 : P.partMaskN;
+        for (int ny = 0; ny < per; ny++)
+            for (int nx = 0; nx < per; nx++) {
+                const int ni = lvBase[Lv] + ny * per + nx;
+                const int x = sbx + nx * n, y = sby + ny * n;
+                if (x >= FW || y >= FH) { nJ[ni] = 0.f; nP[ni] = -1; continue; }
+                const bool hasRows = y + half < FH, hasCols = x + half < FW;
+                float splitJ = 0.f;
+                if (n == 8) {
+                    for (int k = 0; k < 4; k++) {
+                        const int xx = x + (k & 1) * 4, yy = y + (k >> 1) * 4;
+                        if (xx < FW && yy < FH) splitJ = splitJ + blockJ(BLOCK_4X4, xx, yy);
+                    }
+                } else {
+                    const int cb = lvBase[Lv + 1], cper = per * 2;
+                    for (int k = 0; k < 4; k++) splitJ = splitJ + nJ[cb + (ny * 2 + (k >> 1)) * cper + nx * 2 + (k & 1)];
+                }
+                float best = INF;
+                int bp = PARTITION_SPLIT;
+                auto consider = [&](int p, float J, float bits) {
+                    const float t = fmaf(lambda, bits, J);
+                    if (t < best) { best = t; bp = p; }
+                };
+                const bool big = n > P.maxPart;
+                if (hasRows && hasCols) {
+                    const float *pb = T.avgPart[bsl];
+                    consider(PARTITION_SPLIT, splitJ, pb[PARTITION_SPLIT]);
+                    if (!big) {
+                        const int hz = sub_bsize(PARTITION_HORZ, n), vt = sub_bsize(PARTITION_VERT, n), sp = sub_bsize(PARTITION_SPLIT, n);
+                        if (pmask & (1 << PARTITION_NONE)) {
+                            const float jn = blockJ(sq_bsize(n), x, y);
+                            if (jn < INF) consider(PARTITION_NONE, jn * P.splitBias, pb[PARTITION_NONE]);
+                        }
+                        if (pmask & (1 << PARTITION_HORZ)) consider(PARTITION_HORZ, blockJ(hz, x, y) + blockJ(hz, x, y + half), pb[PARTITION_HORZ]);
+                        if (pmask & (1 << PARTITION_VERT)) consider(PARTITION_VERT, blockJ(vt, x, y) + blockJ(vt, x + half, y), pb[PARTITION_VERT]);
+                        if (n > 8) {
+                            const float s00 = blockJ(sp, x, y), s01 = blockJ(sp, x + half, y), s10 = blockJ(sp, x, y + half),
+                                        s11 = blockJ(sp, x + half, y + half);
+                            if (pmask & (1 << PARTITION_HORZ_A)) consider(PARTITION_HORZ_A, s00 + s01 + blockJ(hz, x, y + half), pb[PARTITION_HORZ_A]);
+                            if (pmask & (1 << PARTITION_HORZ_B)) consider(PARTITION_HORZ_B, blockJ(hz, x, y) + s10 + s11, pb[PARTITION_HORZ_B]);
+                            if (pmask & (1 << PARTITION_VERT_A)) consider(PARTITION_VERT_A, s00 + s10 + blockJ(vt, x + half, y), pb[PARTITION_VERT_A]);
+                            if (pmask & (1 << PARTITION_VERT_B)) consider(PARTITION_VERT_B, blockJ(vt, x, y) + s01 + s11, pb[PARTITION_VERT_B]);
+                            const int h4 = sub_bsize(PARTITION_HORZ_4, n), v4 = sub_bsize(PARTITION_VERT_4, n);
+                            if (pmask & (1 << PARTITION_HORZ_4)) {
+                                float J = 0.f;
+                                for (int i = 0; i < 4; i++) if (i < 3 || y + 3 * q4 < FH) J = J + blockJ(h4, x, y + i * q4);
+                                consider(PARTITION_HORZ_4, J, pb[PARTITION_HORZ_4]);
+                            }
+                            if (pmask & (1 << PARTITION_VERT_4)) {
+                                float J = 0.f;
+                                for (int i = 0; i < 4; i++) if (i < 3 || x + 3 * q4 < FW) J = J + blockJ(v4, x + i * q4, y);
+                                consider(PARTITION_VERT_4, J, pb[PARTITION_VERT_4]);
+                            }
+                        }
+                    }
+                } else if (hasCols) {
+                    consider(PARTITION_SPLIT, splitJ, 1.f);
+                    if (!big) consider(PARTITION_HORZ, blockJ(sub_bsize(PARTITION_HORZ, n), x, y), 1.f);
+                } else if (hasRows) {
+                    consider(PARTITION_SPLIT, splitJ, 1.f);
+                    if (!big) consider(PARTITION_VERT, blockJ(sub_bsize(PARTITION_VERT, n), x, y), 1.f);
+                } else {
+                    consider(PARTITION_SPLIT, splitJ, 0.f);
+                }
+                nJ[ni] = best;
+                nP[ni] = (int8_t)bp;
+            }
+    }
+    B.sbJ[sb] = nJ[0];
+    Leaf *out = B.leaves + (size_t)sb * MAX_LEAVES;
+    int nl = 0;
+    auto leaf = [&](int bs, int x, int y) {
+        if (x >= FW || y >= FH) return;
+        Leaf lf;
+        lf.bsize = (uint8_t)bs; lf.x4 = (uint8_t)((x - sbx) >> 2); lf.y4 = (uint8_t)((y - sby) >> 2); lf.pad = 0;
+        out[nl++] = lf;
+    };
+    int stk[64][3];
+    int sp = 0;
+    stk[0][0] = 0; stk[0][1] = 0; stk[0][2] = 0; sp = 1;
+    while (sp > 0) {
+        sp--;
+        const int Lv = stk[sp][0], nx = stk[sp][1], ny = stk[sp][2];
+        const int n = 64 >> Lv, half = n >> 1, q4 = n >> 2, per = 1 << Lv;
+        const int p = nP[lvBase[Lv] + ny * per + nx];
+        const int x = sbx + nx * n, y = sby + ny * n;
+        if (p < 0) continue;
+        const int sub = sub_bsize(p, n), sp2 = sub_bsize(PARTITION_SPLIT, n);
+        const bool hasRows = y + half < FH, hasCols = x + half < FW;
+        switch (p) {
+        case PARTITION_NONE: leaf(sub, x, y); break;
+        case PARTITION_HORZ: leaf(sub, x, y); if (hasRows) leaf(sub, x, y + half); break;
+        case PARTITION_VERT: leaf(sub, x, y); if (hasCols) leaf(sub, x + half, y); break;
+        case PARTITION_SPLIT:
+            if (n == 8) { leaf(BLOCK_4X4, x, y); leaf(BLOCK_4X4, x + 4, y); leaf(BLOCK_4X4, x, y + 4); leaf(BLOCK_4X4, x + 4, y + 4); }
+            else for (int k = 3; k >= 0; k--) { stk[sp][0] = Lv + 1; stk[sp][1] = nx * 2 + (k & 1); stk[sp][2] = ny * 2 + (k >> 1); sp++; }
+            break;
+        case PARTITION_HORZ_A: leaf(sp2, x, y); leaf(sp2, x + half, y); leaf(sub, x, y + half); break;
+        case PARTITION_HORZ_B: leaf(sub, x, y); leaf(sp2, x, y + half); leaf(sp2, x + half, y + half); break;
+        case PARTITION_VERT_A: leaf(sp2, x, y); leaf(sp2, x, y + half); leaf(sub, x + half, y); break;
+        case PARTITION_VERT_B: leaf(sub, x, y); leaf(sp2, x + half, y); leaf(sp2, x + half, y + half); break;
+        case PARTITION_HORZ_4: for (int i = 0; i < 4; i++) if (i < 3 || y + 3 * q4 < FH) leaf(sub, x, y + i * q4); break;
+        default: for (int i = 0; i < 4; i++) if (i < 3 || x + 3 * q4 < FW) leaf(sub, x + i * q4, y); break;
+        }
+    }
+    B.leafCount[sb] = (uint16_t)nl;
+}
+
+void launch_dp(const Batch &B, const SearchParams &P, hipStream_t st) {
+    if (!B.nSb) return;
+    hipLaunchKernelGGL(k_dp, dim3((B.nSb + 63) / 64), dim3(64), 0, st, B, P);
+}
+
+}  // namespace av1m

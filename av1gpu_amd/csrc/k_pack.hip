@@ -1,0 +1,30 @@
+// Pack: per-SB output slots of the closed loop -> DecisionRecord arrays in decode order (tiles in raster order, SBs in
+// raster order inside each tile), block_txb and coeff_off made image-relative.  One workgroup per SB; the offsets come
+// from a host prefix sum over the SB counts.  Reference: encoder/src/k_misc.cu:46-106, pipeline.cpp:878-904.
+#include "av1m_common.h"
+
+namespace av1m {
+
+__global__ void __launch_bounds__(256) k_pack(Batch B, const PackOff *off, RecBlock *oB, uint32_t *oBT, RecTxb *oT, int16_t *oC) {
+    const uint32_t sb = blockIdx.x;
+    const SbOut so = B.sbOut[sb];
+    const PackOff o = off[sb];
+    for (uint32_t k = threadIdx.x; k < so.nBlocks; k += blockDim.x) {
+        oB[o.gBlk + k] = B.oBlocks[(size_t)sb * MAX_LEAVES + k];
+        oBT[o.gBlk + k] = o.rTxb + B.oBlockTxb[(size_t)sb * MAX_LEAVES + k];
+    }
+    for (uint32_t k = threadIdx.x; k < so.nTxb; k += blockDim.x) {
+        RecTxb t = B.oTxb[(size_t)sb * SB_TXB_CAP + k];
+        t.coeff_off += o.rCoef;
+        oT[o.gTxb + k] = t;
+    }
+    const int16_t *src = B.oCoeffs + (size_t)sb * SB_COEF_CAP;
+    for (uint32_t k = threadIdx.x; k < so.nCoeffs; k += blockDim.x) oC[o.gCoef + k] = src[k];
+}
+
+void launch_pack(const Batch &B, const PackOff *dOff, RecBlock *oB, uint32_t *oBT, RecTxb *oT, int16_t *oC, hipStream_t st) {
+    if (!B.nSb) return;
+    hipLaunchKernelGGL(k_pack, dim3(B.nSb), dim3(256), 0, st, B, dOff, oB, oBT, oT, oC);
+}
+
+}  // namespace av1m

@@ -1,0 +1,741 @@
+// Closed-loop decisions and reconstruction: one 256-thread workgroup (8 groups of 32 lanes) per 64x64 superblock,
+// launched per wavefront step (SBs whose left, above and above-right neighbours are complete).  Inside the SB the
+// leaves run in decode order exactly as the decoder sees them: BlockDecoded state, real reconstructed edges, exact
+// coefficient contexts from the 4x4 maps.  Reference: encoder/src/k_recon.cuh:1692-1808 (kernel), 822-1213 (luma),
+// 1220-1686 (chroma), 139-197 (inverse + add), 203-238 (contexts), 685-730 (records).
+//
+// Per block:
+//   luma    every re-checked analysis candidate x every intra tx type of the set (tx blocks up to 16x16; DCT_DCT only
+//           above), RD tasks spread over the groups, the first minimum of J = fmaf(lambda, R + mode bits, D) wins;
+//   chroma  CfL alpha (least squares from the reconstructed luma, then +-1 by exact SSE), SATD screening of the 13 UV
+//           modes and the deltas around the best directional one, top uvK (+ CfL) RD tasks for U and V, pick;
+//   final   the winner re-quantized (the same levels), prediction, exact dequant + inverse transform + add with the
+//           conformance repair, tx records, context maps, BlockDecoded, block record, 4x4 maps.
+#include "dev_eval.h"
+
+namespace av1m {
+
+constexpr int RC_NT = 256, RC_NG = 8, RC_NBIG = 2;
+
+struct RcGroup {
+    EvalScr<16, 16> ev;
+    uint16_t ea[EDGE_LEN], el[EDGE_LEN];
+};
+struct RcBig {
+    EvalScr<64, 64> ev;
+    uint16_t ea[EDGE_LEN], el[EDGE_LEN];
+};
+struct RcShared {
+    RcGroup g[RC_NG];
+    RcBig big[RC_NBIG];
+    int32_t inv[64 * 64];                    // inverse transform residual
+    uint16_t ea[3][EDGE_LEN], el[3][EDGE_LEN];   // unfiltered edges of the current tx block per plane
+    int dcv[3], ft[3], hvL[3], hvA[3], px[3], py[3], mx[3], my[3];
+    int skipCtx[3], dcCtx[3];
+    int gup[RC_NG][2];                       // upsampling flags of each group's prepared edges
+    uint8_t bdm[3][18][18];                  // BlockDecoded[plane][y + 1][x + 1]
+    int taskC[64], taskT[64], taskP[64], taskE[64];
+    float taskJ[64];
+    int nTask;
+    float gBestJ[RC_NG];
+    int gBestT[RC_NG];
+    int win, winEob[3];
+    int cflL[32 * 32];
+    int cflAvg, cflAlpha[2];
+    unsigned long long acc64[8];
+    int satdI[32], uvc[32];
+    int nTxb, nCoef, errFlag, nzFlag, eobOut, culOut;
+    int am, lm, mode, delta, uvMode, uvDelta, nuv, bsize;
+};
+
+template <typename Px> __device__ __forceinline__ Px *rec_plane(const Img &I, int p) { return (Px *)I.rec[p]; }
+template <typename Px> __device__ __forceinline__ const Px *src_plane(const Img &I, int p) { return (const Px *)I.src[p]; }
+
+// get_filter_type (spec 7.11.2.8): 1 if the above or left block (chroma: the chroma-carrying one) uses a SMOOTH mode
+__device__ int filter_type(const Img &I, int plane, int mi_row, int mi_col, int availU, int availL) {
+    const uint8_t *m = plane ? I.mUV : I.mY;
+    auto smooth = [](int v) { return v == SMOOTH_PRED || v == SMOOTH_V_PRED || v == SMOOTH_H_PRED; };
+    int a = 0, l = 0;
+    if (availU) {
+        int r = mi_row - 1, c = mi_col;
+        if (plane > 0 && I.ss) { if (!(mi_col & 1)) c++; if (mi_row & 1) r--; }
+        a = smooth(m[(size_t)r * I.MiCols + c]);
+    }
+    if (availL) {
+        int r = mi_row, c = mi_col - 1;
+        if (plane > 0 && I.ss) { if (mi_col & 1) c--; if (!(mi_row & 1)) r++; }
+        l = smooth(m[(size_t)r * I.MiCols + c]);
+    }
+    return a || l;
+}
+
+// coefficient contexts of a tx block from the context maps (one thread)
+__device__ void txb_ctx(const Img &I, int plane, int txSz, int x4, int y4, int bsize, int tx0, int ty0, int *skipCtx, int *dcCtx) {
+    const int ss = plane ? I.ss : 0;
+    const int stride = I.MiCols >> ss, maxX4 = I.MiCols >> ss, maxY4 = I.MiRows >> ss;
+    const uint8_t *L = I.cLvl[plane], *D = I.cDc[plane];
+    const bool hasA = y4 > ((ty0 >> 2) >> ss), hasL = x4 > ((tx0 >> 2) >> ss);
+    const int w = tx_w(txSz), h = tx_h(txSz), w4 = w >> 2, h4 = h >> 2;
+    int top = 0, left = 0, topAny = 0, leftAny = 0, sgn = 0;
+    if (hasA)
+        for (int k = 0; k < w4 && x4 + k < maxX4; k++) {
+            const int l = L[(size_t)(y4 - 1) * stride + x4 + k], d = D[(size_t)(y4 - 1) * stride + x4 + k];
+            top = max(top, l); topAny |= l | d; sgn += d == 2 ? 1 : d == 1 ? -1 : 0;
+        }
+    if (hasL)
+        for (int k = 0; k < h4 && y4 + k < maxY4; k++) {
+            const int l = L[(size_t)(y4 + k) * stride + x4 - 1], d = D[(size_t)(y4 + k) * stride + x4 - 1];
+            left = max(left, l); leftAny |= l | d; sgn += d == 2 ? 1 : d == 1 ? -1 : 0;
+        }
+    if (plane == 0) {
+        const int bw = 4 * bw4(bsize), bh = 4 * bh4(bsize);
+        *skipCtx = (bw == w && bh == h) ? 0 : (top == 0 && left == 0) ? 1 : (top == 0 || left == 0) ? 2 + (max(top, left) > 3)
+                 : max(top, left) <= 3 ? 4 : min(top, left) <= 3 ? 5 : 6;
+    } else {
+        const int ub = uv_bsize(bsize, I.ss);
+        *skipCtx = (topAny != 0) + (leftAny != 0) + 7 + ((16 * bw4(ub) * bh4(ub) > w * h) ? 3 : 0);
+    }
+    *dcCtx = sgn < 0 ? 1 : sgn > 0 ? 2 : 0;
+}
+
+// base edges of the tx block of plane p at (px, py) from the reconstruction (group g), DC value and filter type
+template <typename Px>
+__device__ void rc_edges(RcShared &S, const Img &I, int p, int px, int py, int lw, int lh, int haveL, int haveA, int haveAR, int haveBL,
+                         int mi_row, int mi_col) {
+    const int ss = p ? I.ss : 0;
+    const int maxX = ((I.MiCols * 4) >> ss) - 1, maxY = ((I.MiRows * 4) >> ss) - 1;
+    uint16_t *A = S.ea[p] + EDGE_OFF, *L = S.el[p] + EDGE_OFF;
+    grp_build_edges(rec_plane<Px>(I, p), I.pitch[p], px, py, lw, lh, haveL, haveA, haveAR, haveBL, maxX, maxY, A, L, I.bd);
+    const int dcv = grp_dc_value(A, L, haveL, haveA, lw, lh, I.bd);
+    if (glane() == 0) {
+        S.dcv[p] = dcv; S.hvL[p] = haveL; S.hvA[p] = haveA; S.px[p] = px; S.py[p] = py; S.mx[p] = maxX; S.my[p] = maxY;
+        S.ft[p] = I.edgeFilter ? filter_type(I, p, mi_row, mi_col, haveA, haveL) : 0;
+    }
+}
+
+// prediction of candidate c (255 = CfL) of plane p for a W x H tx block, by the group, into res = src - pred; the group's
+// edge buffers receive the prepared (filtered) edges of directional modes.  Returns nothing; S.gup[gid] = flags.
+template <typename Px>
+__device__ void grp_residual(RcShared &S, uint16_t *gea, uint16_t *gel, int p, int c, int lw, int lh, const Px *src, int sp,
+                             int16_t *res, int bd) {
+    const int W = 1 << lw, H = 1 << lh, lane = glane();
+    const int m = c == 255 ? DC_PRED : cand_mode(c), pa = pangle(m, c == 255 ? 0 : cand_delta(c));
+    const uint16_t *EA = S.ea[p] + EDGE_OFF, *EL = S.el[p] + EDGE_OFF;
+    int upA = 0, upL = 0;
+    const bool filt = c != 255 && is_dir(m) && gea != nullptr;
+    if (filt) {
+        for (int i = lane; i < EDGE_LEN; i += GW) { gea[i] = S.ea[p][i]; gel[i] = S.el[p][i]; }
+        gsync();
+        grp_edge_prepare(pa, lw, lh, S.ft[p], S.hvL[p], S.hvA[p], S.px[p], S.py[p], S.mx[p], S.my[p], gea + EDGE_OFF, gel + EDGE_OFF, bd, upA,
+                         upL);
+        EA = gea + EDGE_OFF;
+        EL = gel + EDGE_OFF;
+    }
+    const int dcv = S.dcv[p];
+    for (int q = lane; q < W * H; q += GW) {
+        const int i = q >> lw, j = q & (W - 1);
+        int pr;
+        if (c == 255) pr = clip3(0, (1 << bd) - 1, dcv + round2signed(S.cflAlpha[p - 1] * (S.cflL[i * W + j] - S.cflAvg), 6));
+        else if (m == DC_PRED) pr = dcv;
+        else pr = pred_px(m, pa, i, j, lw, lh, EA, EL, upA, upL);
+        res[q] = (int16_t)((int)src[i * sp + j] - pr);
+    }
+    if (lane == 0) { S.gup[gid()][0] = upA; S.gup[gid()][1] = upL; }
+    gsync();
+}
+
+// one RD task (candidate c of plane p with tx type t) by the calling group, on scratch Scr (the group's or a big slot)
+template <int W, int H, typename Px, typename Scr>
+__device__ EvalOut grp_task(RcShared &S, Scr &X, uint16_t *gea, uint16_t *gel, const Img &I, const DevTables &T, int p, int c, int t,
+                            const Px *src, int sp, const SearchParams &P, int edgeFilter) {
+    constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
+    grp_residual<Px>(S, edgeFilter ? gea : nullptr, gel, p, c, LW, LH, src, sp, X.ev.res, I.bd);
+    EvalIn e;
+    e.txSz = tx_of(W, H); e.txType = t; e.plane = p;
+    e.dcQ = dc_q(I.qindex, I.bd); e.acQ = ac_q(I.qindex, I.bd);
+    e.skipCtx = S.skipCtx[p]; e.dcCtx = S.dcCtx[p]; e.intraDir = p == 0 ? cand_mode(c) : 0; e.bd = I.bd;
+    e.rnd = p ? P.rndC : P.rnd; e.zbin = p ? P.zbinC : P.zbin;
+    e.resEnergy = (W == 64 || H == 64) ? grp_res_energy<W, H>(X.ev.res) : 0.f;
+    e.C = &T.cost[I.qctx];
+    return grp_eval<W, H>(X.ev, e, T);
+}
+
+// ---- exact inverse transform + reconstruction by the workgroup (spec 7.13.3, 7.12.3 step 3) ----
+template <int LW>
+__device__ uint32_t inv_row(int i, int W, int th, int tw, const int16_t *q, int dcQ, int acQ, int dsh, int bd, bool rect, int rk,
+                            int rowShift, int32_t *res) {
+    constexpr int N = 1 << LW;
+    int32_t Tv[N];
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        const int v = (i < th && j < tw) ? q[i * tw + j] : 0;
+        Tv[j] = v ? dequant(v, (i == 0 && j == 0) ? dcQ : acQ, dsh, bd) : 0;
+        if (rect) Tv[j] = (int32_t)round2l((int64_t)Tv[j] * 2896, 12);
+    }
+    uint32_t err = 0;
+    Inv::run<LW>(Tv, rk, bd + 8, err);
+    const int colR = bd + 6 > 16 ? bd + 6 : 16;
+#pragma unroll
+    for (int j = 0; j < N; j++) res[i * W + j] = clip3(-(1 << (colR - 1)), (1 << (colR - 1)) - 1, (int32_t)round2l(Tv[j], rowShift));
+    return err;
+}
+template <int LH>
+__device__ uint32_t inv_col(int j, int W, int ck, int bd, int32_t *res) {
+    constexpr int N = 1 << LH;
+    int32_t Tv[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) Tv[i] = res[i * W + j];
+    uint32_t err = 0;
+    Inv::run<LH>(Tv, ck, bd + 6 > 16 ? bd + 6 : 16, err);
+#pragma unroll
+    for (int i = 0; i < N; i++) res[i * W + j] = (int32_t)round2l(Tv[i], 4);
+    return err;
+}
+
+// dequantize + inverse + add the levels q (LDS, modified by the repair) onto dst; returns the number of repairs
+template <typename Px>
+__device__ int wg_inverse_add(RcShared &S, int16_t *q, int txSz, int txType, int bd, int dcQ, int acQ, Px *dst, int pitch) {
+    const int tid = threadIdx.x, lw = av1t::Tx_Width_Log2[txSz], lh = av1t::Tx_Height_Log2[txSz], W = 1 << lw, H = 1 << lh;
+    const int tw = W > 32 ? 32 : W, th = H > 32 ? 32 : H, n = tw * th, den = dq_denom(txSz), dsh = den == 4 ? 2 : den == 2 ? 1 : 0;
+    const bool rect = lw - lh == 1 || lh - lw == 1;
+    const int rk = row_kind(txType), ck = col_kind(txType), rs = av1t::Transform_Row_Shift[txSz];
+    int fixes = 0;
+    for (;;) {
+        if (tid == 0) { S.errFlag = 0; S.nzFlag = 0; }
+        __syncthreads();
+        uint32_t err = 0;
+        if (tid < H) {
+            switch (lw) {
+            case 2: err = inv_row<2>(tid, W, th, tw, q, dcQ, acQ, dsh, bd, rect, rk, rs, S.inv); break;
+            case 3: err = inv_row<3>(tid, W, th, tw, q, dcQ, acQ, dsh, bd, rect, rk, rs, S.inv); break;
+            case 4: err = inv_row<4>(tid, W, th, tw, q, dcQ, acQ, dsh, bd, rect, rk, rs, S.inv); break;
+            case 5: err = inv_row<5>(tid, W, th, tw, q, dcQ, acQ, dsh, bd, rect, rk, rs, S.inv); break;
+            default: err = inv_row<6>(tid, W, th, tw, q, dcQ, acQ, dsh, bd, rect, rk, rs, S.inv); break;
+            }
+        }
+        if (err) atomicOr(&S.errFlag, (int)err);
+        __syncthreads();
+        err = 0;
+        if (tid < W) {
+            switch (lh) {
+            case 2: err = inv_col<2>(tid, W, ck, bd, S.inv); break;
+            case 3: err = inv_col<3>(tid, W, ck, bd, S.inv); break;
+            case 4: err = inv_col<4>(tid, W, ck, bd, S.inv); break;
+            case 5: err = inv_col<5>(tid, W, ck, bd, S.inv); break;
+            default: err = inv_col<6>(tid, W, ck, bd, S.inv); break;
+            }
+        }
+        if (err) atomicOr(&S.errFlag, (int)err);
+        __syncthreads();
+        if (!S.errFlag) break;
+        fixes++;
+        for (int i = tid; i < n; i += RC_NT) {
+            q[i] = (int16_t)(q[i] / 2);
+            if (q[i]) S.nzFlag = 1;
+        }
+        __syncthreads();
+        if (!S.nzFlag) return fixes;
+    }
+    const int fud = flip_ud(txType), flr = flip_lr(txType), maxv = (1 << bd) - 1;
+    for (int p = tid; p < W * H; p += RC_NT) {
+        const int i = p >> lw, j = p & (W - 1);
+        Px *d = dst + (size_t)(fud ? H - 1 - i : i) * pitch + (flr ? W - 1 - j : j);
+        *d = (Px)clip3(0, maxv, (int)*d + S.inv[i * W + j]);
+    }
+    __syncthreads();
+    return fixes;
+}
+
+// write the prediction of candidate c of plane p (edges prepared by group g, or the base edges) into dst (workgroup)
+template <typename Px>
+__device__ void wg_write_pred(const RcShared &S, const uint16_t *gea, const uint16_t *gel, int g, int p, int c, int lw, int lh, Px *dst,
+                              int pitch, int bd) {
+    const int W = 1 << lw, H = 1 << lh;
+    const int m = c == 255 ? DC_PRED : cand_mode(c), pa = pangle(m, c == 255 ? 0 : cand_delta(c));
+    const bool filt = c != 255 && is_dir(m) && gea != nullptr;
+    const uint16_t *EA = filt ? gea + EDGE_OFF : S.ea[p] + EDGE_OFF, *EL = filt ? gel + EDGE_OFF : S.el[p] + EDGE_OFF;
+    const int upA = filt ? S.gup[g][0] : 0, upL = filt ? S.gup[g][1] : 0, dcv = S.dcv[p];
+    for (int q = threadIdx.x; q < W * H; q += RC_NT) {
+        const int i = q >> lw, j = q & (W - 1);
+        int pr;
+        if (c == 255) pr = clip3(0, (1 << bd) - 1, dcv + round2signed(S.cflAlpha[p - 1] * (S.cflL[i * W + j] - S.cflAvg), 6));
+        else if (m == DC_PRED) pr = dcv;
+        else pr = pred_px(m, pa, i, j, lw, lh, EA, EL, upA, upL);
+        dst[(size_t)i * pitch + j] = (Px)pr;
+    }
+    __syncthreads();
+}
+
+// record the finished tx block with levels q: coefficients, txb entry, context maps, BlockDecoded (workgroup)
+__device__ int wg_record_txb(RcShared &S, const Batch &B, uint32_t sb, const Img &I, int p, int txSz, int txType, int x4, int y4,
+                             const int16_t *q) {
+    const int tid = threadIdx.x, n = tx_ncoef(txSz);
+    const uint16_t *scan = scan_of(txSz, txType);
+    const uint32_t off = (uint32_t)S.nCoef;
+    if (tid < 32) {
+        int last = 0;
+        for (int c = tid; c < n; c += 32) if (q[scan[c]]) last = c + 1;
+        const int eob = gmaxi(last);
+        int cul = 0;
+        for (int c = tid; c < eob; c += 32) cul += abs((int)q[scan[c]]);
+        cul = gsumi(cul);
+        if (tid == 0) { S.eobOut = eob; S.culOut = min(cul, 63); }
+    }
+    int16_t *oc = B.oCoeffs + (size_t)sb * SB_COEF_CAP + off;
+    for (int i = tid; i < n; i += RC_NT) oc[i] = q[i];
+    __syncthreads();
+    const int eob = S.eobOut, cul = S.culOut, dcCat = q[0] < 0 ? 1 : q[0] > 0 ? 2 : 0;
+    const int ss = p ? I.ss : 0, stride = I.MiCols >> ss, rows = I.MiRows >> ss;
+    const int w4 = tx_w(txSz) >> 2, h4 = tx_h(txSz) >> 2, lw4 = av1t::Tx_Width_Log2[txSz] - 2;
+    const int r4 = y4 & ((16 >> ss) - 1), c4 = x4 & ((16 >> ss) - 1);
+    for (int k = tid; k < w4 * h4; k += RC_NT) {
+        const int yy = k >> lw4, xx = k & (w4 - 1);
+        if (y4 + yy < rows && x4 + xx < stride) {
+            I.cLvl[p][(size_t)(y4 + yy) * stride + x4 + xx] = (uint8_t)(eob ? cul : 0);
+            I.cDc[p][(size_t)(y4 + yy) * stride + x4 + xx] = (uint8_t)(eob ? dcCat : 0);
+        }
+        S.bdm[p][r4 + yy + 1][c4 + xx + 1] = 1;
+    }
+    if (tid == 0) {
+        RecTxb t;
+        t.coeff_off = off; t.x4 = (uint16_t)x4; t.y4 = (uint16_t)y4; t.plane = (uint8_t)p; t.tx_size = (uint8_t)txSz;
+        t.tx_type = (uint8_t)(p == 0 && eob == 0 ? DCT_DCT : txType); t.flags = 0; t.eob = (uint16_t)eob; t.reserved = 0;
+        B.oTxb[(size_t)sb * SB_TXB_CAP + S.nTxb] = t;
+    }
+    __syncthreads();
+    if (tid == 0) { S.nTxb++; S.nCoef += n; }
+    __syncthreads();
+    return eob;
+}
+
+// run S.taskC/T/P[0 .. nTask) on the groups (tx > 16x16: the RC_NBIG big slots); per task J = fmaf(lambda, R + extra, D)
+// with extra = luma mode bits (plane 0) or 0; S.taskJ / taskE.  Then S.win = the first minimum over the tasks.
+template <int W, int H, typename Px>
+__device__ void wg_run_tasks(RcShared &S, const Img &I, const DevTables &T, const SearchParams &P, const Px *const *srcs, int sp, float lambda,
+                             bool lumaBits, bool pickWinner) {
+    constexpr bool big = W > 16 || H > 16;
+    const int g = gid(), lane = glane();
+    const CostTab &C = T.cost[I.qctx];
+    const int ng = big ? RC_NBIG : RC_NG;
+    if (g < ng) {
+        for (int t = g; t < S.nTask; t += ng) {
+            const int c = S.taskC[t], tt = S.taskT[t], p = S.taskP[t];
+            EvalOut o;
+            if constexpr (big) o = grp_task<W, H, Px>(S, S.big[g], S.big[g].ea, S.big[g].el, I, T, p, c, tt, srcs[p], sp, P, I.edgeFilter);
+            else o = grp_task<W, H, Px>(S, S.g[g], S.g[g].ea, S.g[g].el, I, T, p, c, tt, srcs[p], sp, P, I.edgeFilter);
+            float J;
+            if (lumaBits) {
+                const int m = cand_mode(c);
+                float mb = C.y_mode[S.am][S.lm][m];
+                if (S.bsize >= BLOCK_8X8 && is_dir(m)) mb += C.angle_delta[m - V_PRED][cand_delta(c) + 3];
+                J = fmaf(lambda, o.R + mb, o.D);
+            } else {
+                J = fmaf(lambda, o.R, o.D);
+            }
+            if (lane == 0) { S.taskJ[t] = J; S.taskE[t] = o.eob; }
+            gsync();
+        }
+    }
+    __syncthreads();
+    if (pickWinner && threadIdx.x == 0) {
+        int w = 0;
+        for (int t = 1; t < S.nTask; t++) if (S.taskJ[t] < S.taskJ[w]) w = t;
+        S.win = w;
+    }
+    __syncthreads();
+}
+
+// re-evaluate task t on group g (big slot for large tx) so its levels and prepared edges are in that group's scratch;
+// returns a pointer to the levels
+template <int W, int H, typename Px>
+__device__ int16_t *wg_final_task(RcShared &S, const Img &I, const DevTables &T, const SearchParams &P, const Px *src, int sp, int t, int g) {
+    constexpr bool big = W > 16 || H > 16;
+    if (gid() == g) {
+        if constexpr (big) grp_task<W, H, Px>(S, S.big[g], S.big[g].ea, S.big[g].el, I, T, S.taskP[t], S.taskC[t], S.taskT[t], src, sp, P, I.edgeFilter);
+        else grp_task<W, H, Px>(S, S.g[g], S.g[g].ea, S.g[g].el, I, T, S.taskP[t], S.taskC[t], S.taskT[t], src, sp, P, I.edgeFilter);
+    }
+    __syncthreads();
+    if constexpr (big) return S.big[g].ev.q;
+    else return S.g[g].ev.q;
+}
+
+template <int W, int H, typename Px>
+__device__ void rc_luma(RcShared &S, const Batch &B, const SearchParams &P, uint32_t sb, const Img &I, int bsize, int x, int y, int availU,
+                        int availL, int tx0, int ty0, float lambda) {
+    constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
+    constexpr bool big = W > 16 || H > 16;
+    const DevTables &T = *B.T;
+    const int tid = threadIdx.x, txSz = tx_of(W, H);
+    const int sbr4 = (y >> 2) & 15, sbc4 = (x >> 2) & 15, mi_row = y >> 2, mi_col = x >> 2;
+    if (gid() == 0)
+        rc_edges<Px>(S, I, 0, x, y, LW, LH, availL, availU, S.bdm[0][sbr4][sbc4 + (W >> 2) + 1], S.bdm[0][sbr4 + (H >> 2) + 1][sbc4],
+                     mi_row, mi_col);
+    if (tid == 64) {
+        txb_ctx(I, 0, txSz, mi_col, mi_row, bsize, tx0, ty0, &S.skipCtx[0], &S.dcCtx[0]);
+        S.am = av1t::Intra_Mode_Context[availU ? I.mY[(size_t)(mi_row - 1) * I.MiCols + mi_col] : DC_PRED];
+        S.lm = av1t::Intra_Mode_Context[availL ? I.mY[(size_t)mi_row * I.MiCols + mi_col - 1] : DC_PRED];
+        const BlockRes &br = B.blk[I.shOff[bsize] + (uint32_t)(y / (4 * bh4(bsize))) * I.shNbx[bsize] + x / (4 * bw4(bsize))];
+        const int nc = max(1, min((int)br.ncand, P.Krecheck));
+        const int set = tx_set_intra(txSz);
+        int n = 0;
+        if (P.txAllCL && set != TX_SET_DCTONLY) {
+            for (int k = 0; k < nc; k++)
+                for (int t = 0; t < 16; t++)
+                    if (tx_in_set(set, t)) { S.taskC[n] = br.cand[k]; S.taskT[n] = t; S.taskP[n] = 0; n++; }
+        } else {
+            for (int k = 0; k < nc; k++) { S.taskC[n] = br.cand[k]; S.taskT[n] = DCT_DCT; S.taskP[n] = 0; n++; }
+            if (br.tx != DCT_DCT) { S.taskC[n] = br.cand[0]; S.taskT[n] = br.tx; S.taskP[n] = 0; n++; }
+        }
+        S.nTask = n;
+        S.bsize = bsize;
+    }
+    __syncthreads();
+    const Px *src = src_plane<Px>(I, 0) + (size_t)y * I.spitch[0] + x;
+    const Px *srcs[3] = {src, src, src};
+    wg_run_tasks<W, H, Px>(S, I, T, P, srcs, I.spitch[0], lambda, true, true);
+    if (!(P.txAllCL && tx_set_intra(txSz) != TX_SET_DCTONLY) && P.txSearchCL && tx_set_intra(txSz) != TX_SET_DCTONLY) {
+        // tx-type pass for the pass-1 winner (txAllCL off): the other types of the set
+        if (tid == 0) {
+            const int c0 = S.taskC[S.win], n0 = S.nTask;
+            const BlockRes &br = B.blk[I.shOff[bsize] + (uint32_t)(y / (4 * bh4(bsize))) * I.shNbx[bsize] + x / (4 * bw4(bsize))];
+            int n = n0;
+            for (int t = 1; t < 16; t++)
+                if (tx_in_set(tx_set_intra(txSz), t) && !(c0 == br.cand[0] && t == br.tx)) { S.taskC[n] = c0; S.taskT[n] = t; S.taskP[n] = 0; n++; }
+            S.nTask = n;
+        }
+        __syncthreads();
+        wg_run_tasks<W, H, Px>(S, I, T, P, srcs, I.spitch[0], lambda, true, true);
+    }
+    const int win = S.win;
+    int16_t *q = wg_final_task<W, H, Px>(S, I, T, P, src, I.spitch[0], win, 0);
+    Px *dst = rec_plane<Px>(I, 0) + (size_t)y * I.pitch[0] + x;
+    const uint16_t *gea = I.edgeFilter ? (big ? S.big[0].ea : S.g[0].ea) : nullptr;
+    const uint16_t *gel = big ? S.big[0].el : S.g[0].el;
+    wg_write_pred<Px>(S, gea, gel, 0, 0, S.taskC[win], LW, LH, dst, I.pitch[0], I.bd);
+    if (S.taskE[win] > 0) {
+        const int fixes = wg_inverse_add<Px>(S, q, txSz, S.taskT[win], I.bd, dc_q(I.qindex, I.bd), ac_q(I.qindex, I.bd), dst, I.pitch[0]);
+        if (fixes && tid == 0) atomicAdd(&B.stats[3], 1u);
+    }
+    const int eob = wg_record_txb(S, B, sb, I, 0, txSz, S.taskT[win], x >> 2, y >> 2, q);
+    if (tid == 0) { S.mode = cand_mode(S.taskC[win]); S.delta = cand_delta(S.taskC[win]); S.winEob[0] = eob; }
+    __syncthreads();
+}
+
+template <int W, int H, typename Px>
+__device__ void rc_chroma(RcShared &S, const Batch &B, const SearchParams &P, uint32_t sb, const Img &I, int bsize, int mi_row, int mi_col,
+                          int availUC, int availLC, int tx0, int ty0, float lambda) {
+    constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
+    constexpr bool big = W > 16 || H > 16;
+    const DevTables &T = *B.T;
+    const CostTab &C = T.cost[I.qctx];
+    const int tid = threadIdx.x, ss = I.ss, bd = I.bd, txSz = tx_of(W, H);
+    const int cx = (mi_col >> ss) * 4, cy = (mi_row >> ss) * 4, x4 = cx >> 2, y4 = cy >> 2;
+    const int sbr4 = (mi_row & 15) >> ss, sbc4 = (mi_col & 15) >> ss;
+    if (gid() < 2) {
+        const int p = 1 + gid();
+        rc_edges<Px>(S, I, p, cx, cy, LW, LH, availLC, availUC, S.bdm[p][sbr4][sbc4 + (W >> 2) + 1], S.bdm[p][sbr4 + (H >> 2) + 1][sbc4],
+                     mi_row, mi_col);
+    }
+    if (tid == 64) {
+        txb_ctx(I, 1, txSz, x4, y4, bsize, tx0, ty0, &S.skipCtx[1], &S.dcCtx[1]);
+        txb_ctx(I, 2, txSz, x4, y4, bsize, tx0, ty0, &S.skipCtx[2], &S.dcCtx[2]);
+    }
+    if (tid < 32) S.satdI[tid] = 0;
+    if (tid < 8) S.acc64[tid] = 0;
+    __syncthreads();
+    const int sp = I.spitch[1];
+    const Px *srcU = src_plane<Px>(I, 1) + (size_t)cy * sp + cx, *srcV = src_plane<Px>(I, 2) + (size_t)cy * sp + cx;
+    const bool cflOk = P.cfl && cfl_allowed(bsize);
+    const float lamS = P.satdLambda * T.lamS[bd == 10][I.qindex];   // the luma lambda's screening weight
+    const int yMode = S.mode;
+    auto uvbits = [&](int m) -> float { return cfl_allowed(bsize) ? C.uv_cfl_yes[yMode][m] : C.uv_cfl_no[yMode][m < 13 ? m : 0]; };
+    auto uvcbits = [&](int c) -> float {
+        const int m = cand_mode(c);
+        float b = uvbits(m);
+        if (bsize >= BLOCK_8X8 && is_dir(m)) b += C.angle_delta[m - V_PRED][cand_delta(c) + 3];
+        return b;
+    };
+    // ---- CfL: luma values, average, least-squares alpha, +-1 refinement by exact SSE ----
+    if (cflOk) {
+        const int lt = av1t::Max_Tx_Size_Rect[bsize], tw = tx_w(lt), th = tx_h(lt);
+        int maxLumaW = 0, maxLumaH = 0;
+        for (int ly = mi_row * 4; ly < mi_row * 4 + 4 * bh4(bsize); ly += th)
+            for (int lx = mi_col * 4; lx < mi_col * 4 + 4 * bw4(bsize); lx += tw)
+                if (lx < I.MiCols * 4 && ly < I.MiRows * 4) { maxLumaW = lx + tw; maxLumaH = ly + th; }
+        const Px *luma = rec_plane<Px>(I, 0);
+        const int lp = I.pitch[0];
+        int sum = 0;
+        for (int q = tid; q < W * H; q += RC_NT) {
+            const int i = q >> LW, j = q & (W - 1);
+            int v;
+            if (ss) {
+                const int ly = min((cy + i) << 1, maxLumaH - 2), lx = min((cx + j) << 1, maxLumaW - 2);
+                v = ((int)luma[(size_t)ly * lp + lx] + luma[(size_t)ly * lp + lx + 1] + luma[(size_t)(ly + 1) * lp + lx] + luma[(size_t)(ly + 1) * lp + lx + 1]) << 1;
+            } else {
+                v = (int)luma[(size_t)min(cy + i, maxLumaH - 1) * lp + min(cx + j, maxLumaW - 1)] << 3;
+            }
+            S.cflL[q] = v;
+            sum += v;
+        }
+        sum = gsumi(sum);
+        if (glane() == 0) atomicAdd(&S.satdI[31], sum);
+        __syncthreads();
+        if (tid == 0) S.cflAvg = round2(S.satdI[31], LW + LH);
+        __syncthreads();
+        long long nU = 0, nV = 0, dd = 0;
+        for (int q = tid; q < W * H; q += RC_NT) {
+            const int i = q >> LW, j = q & (W - 1);
+            const long long l = S.cflL[q] - S.cflAvg;
+            nU += l * ((int)srcU[i * sp + j] - S.dcv[1]);
+            nV += l * ((int)srcV[i * sp + j] - S.dcv[2]);
+            dd += l * l;
+        }
+        atomicAdd(&S.acc64[0], (unsigned long long)nU);
+        atomicAdd(&S.acc64[1], (unsigned long long)nV);
+        atomicAdd(&S.acc64[2], (unsigned long long)dd);
+        __syncthreads();
+        if (tid == 0) {
+            const long long d = (long long)S.acc64[2];
+            for (int k = 0; k < 2; k++) {
+                const long long n = (long long)S.acc64[k];
+                const int a = d > 0 ? (int)rintf(64.f * (float)n / (float)d) : 0;
+                S.cflAlpha[k] = max(-16, min(16, a));
+            }
+            for (int k = 0; k < 6; k++) S.acc64[2 + k] = 0;
+        }
+        __syncthreads();
+        long long sse[6] = {0, 0, 0, 0, 0, 0};
+        for (int q = tid; q < W * H; q += RC_NT) {
+            const int i = q >> LW, j = q & (W - 1), l = S.cflL[q] - S.cflAvg;
+            for (int pl = 0; pl < 2; pl++) {
+                const int s = pl == 0 ? srcU[i * sp + j] : srcV[i * sp + j];
+                for (int k = 0; k < 3; k++) {
+                    const int a = max(-16, min(16, S.cflAlpha[pl] + k - 1));
+                    const int d = s - clip3(0, (1 << bd) - 1, S.dcv[pl + 1] + round2signed(a * l, 6));
+                    sse[pl * 3 + k] += (long long)d * d;
+                }
+            }
+        }
+        for (int k = 0; k < 6; k++) atomicAdd(&S.acc64[2 + k], (unsigned long long)sse[k]);
+        __syncthreads();
+        if (tid == 0)
+            for (int pl = 0; pl < 2; pl++) {
+                int bk = 1;
+                for (int k = 0; k < 3; k++) if ((long long)S.acc64[2 + pl * 3 + k] < (long long)S.acc64[2 + pl * 3 + bk]) bk = k;
+                S.cflAlpha[pl] = max(-16, min(16, S.cflAlpha[pl] + bk - 1));
+            }
+        if (tid < 32) S.satdI[tid] = 0;
+        __syncthreads();
+    } else if (tid == 0) {
+        S.cflAlpha[0] = S.cflAlpha[1] = 0;
+    }
+    // ---- SATD screening of the UV candidates over U and V (unfiltered edges) ----
+    if (tid < 13) S.uvc[tid] = cand_pack(tid, 0);
+    __syncthreads();
+    constexpr int NSUB = (W / 4) * (H / 4), NSX = W / 4;
+    auto screen = [&](int c0, int c1) {
+        const int items = (c1 - c0) * NSUB * 2;
+        for (int it = tid; it < items; it += RC_NT) {
+            const int ci = c0 + it / (2 * NSUB), rr = it % (2 * NSUB), p = 1 + rr / NSUB, sbk = rr % NSUB;
+            const int sx = (sbk % NSX) * 4, sy = (sbk / NSX) * 4, c = S.uvc[ci], m = cand_mode(c), pa = pangle(m, cand_delta(c));
+            const Px *s = p == 1 ? srcU : srcV;
+            const uint16_t *A = S.ea[p] + EDGE_OFF, *L = S.el[p] + EDGE_OFF;
+            int d[16];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int pr = m == DC_PRED ? S.dcv[p] : pred_px(m, pa, sy + i, sx + j, LW, LH, A, L, 0, 0);
+                    d[i * 4 + j] = (int)s[(sy + i) * sp + sx + j] - pr;
+                }
+            atomicAdd(&S.satdI[ci], hadamard4(d));
+        }
+        __syncthreads();
+    };
+    screen(0, 13);
+    if (tid == 0) {
+        S.nuv = 13;
+        if (P.uvDeltas && bsize >= BLOCK_8X8) {
+            int bm = -1;
+            float bj = INF;
+            for (int m = V_PRED; m <= D67_PRED; m++) {
+                const float j = fmaf(lamS, uvcbits(cand_pack(m, 0)), (float)S.satdI[m] * 0.25f);
+                if (bm < 0 || j < bj) { bj = j; bm = m; }
+            }
+            for (int k = 0; k < 6; k++) S.uvc[13 + k] = cand_pack(bm, k < 3 ? -(k + 1) : k - 2);
+            S.nuv = 19;
+        }
+    }
+    __syncthreads();
+    if (S.nuv > 13) screen(13, 19);
+    if (tid == 0) {
+        float js[19];
+        bool used[19];
+        for (int i = 0; i < S.nuv; i++) { js[i] = fmaf(lamS, uvcbits(S.uvc[i]), (float)S.satdI[i] * 0.25f); used[i] = false; }
+        const int K = max(1, min(P.uvK, 6));
+        int n = 0;
+        for (int k = 0; k < K; k++) {
+            int b = -1;
+            for (int i = 0; i < S.nuv; i++) if (!used[i] && (b < 0 || js[i] < js[b])) b = i;
+            if (b < 0) break;
+            used[b] = true;
+            for (int p = 1; p <= 2; p++) { S.taskC[n] = S.uvc[b]; S.taskT[n] = chroma_tx_type(txSz, cand_mode(S.uvc[b])); S.taskP[n] = p; n++; }
+        }
+        if (cflOk && (S.cflAlpha[0] || S.cflAlpha[1]))
+            for (int p = 1; p <= 2; p++) { S.taskC[n] = 255; S.taskT[n] = chroma_tx_type(txSz, UV_CFL_PRED); S.taskP[n] = p; n++; }
+        S.nTask = n;
+    }
+    __syncthreads();
+    const Px *srcs[3] = {srcU, srcU, srcV};
+    wg_run_tasks<W, H, Px>(S, I, T, P, srcs, sp, lambda, false, false);
+    if (tid == 0) {
+        float bJ = INF;
+        int bt = 0;
+        for (int t = 0; t + 1 < S.nTask; t += 2) {
+            const int c = S.taskC[t];
+            float J = S.taskJ[t] + S.taskJ[t + 1];
+            if (c == 255) {
+                const int aU = S.cflAlpha[0], aV = S.cflAlpha[1];
+                const int sU = aU == 0 ? 0 : aU < 0 ? 1 : 2, sV = aV == 0 ? 0 : aV < 0 ? 1 : 2;
+                float b = uvbits(UV_CFL_PRED) + C.cfl_sign[sU * 3 + sV - 1];
+                if (sU) b += C.cfl_alpha[(sU - 1) * 3 + sV][abs(aU) - 1];
+                if (sV) b += C.cfl_alpha[(sV - 1) * 3 + sU][abs(aV) - 1];
+                J = fmaf(lambda, b, J);
+            } else {
+                J = fmaf(lambda, uvcbits(c), J);
+            }
+            if (J < bJ) { bJ = J; bt = t; }
+        }
+        S.win = bt;
+        const int c = S.taskC[bt];
+        S.uvMode = c == 255 ? UV_CFL_PRED : cand_mode(c);
+        S.uvDelta = c == 255 ? 0 : cand_delta(c);
+    }
+    __syncthreads();
+    const int win = S.win, uvTxType = S.taskT[win];
+    for (int p = 1; p <= 2; p++) {
+        const int t = win + p - 1;
+        int16_t *q = wg_final_task<W, H, Px>(S, I, T, P, p == 1 ? srcU : srcV, sp, t, 0);
+        Px *dst = rec_plane<Px>(I, p) + (size_t)cy * I.pitch[p] + cx;
+        const uint16_t *gea = I.edgeFilter ? (big ? S.big[0].ea : S.g[0].ea) : nullptr;
+        const uint16_t *gel = big ? S.big[0].el : S.g[0].el;
+        wg_write_pred<Px>(S, gea, gel, 0, p, S.taskC[t], LW, LH, dst, I.pitch[p], bd);
+        if (S.taskE[t] > 0) {
+            const int fixes = wg_inverse_add<Px>(S, q, txSz, uvTxType, bd, dc_q(I.qindex, bd), ac_q(I.qindex, bd), dst, I.pitch[p]);
+            if (fixes && tid == 0) atomicAdd(&B.stats[3], 1u);
+        }
+        const int eob = wg_record_txb(S, B, sb, I, p, txSz, uvTxType, x4, y4, q);
+        if (tid == 0) S.winEob[p] = eob;
+        __syncthreads();
+    }
+}
+
+template <typename Px>
+__global__ void __launch_bounds__(RC_NT) k_recon(Batch B, SearchParams P, const uint32_t *sbList) {
+    extern __shared__ __align__(16) char smraw[];
+    RcShared &S = *reinterpret_cast<RcShared *>(smraw);
+    const uint32_t sb = sbList[blockIdx.x];
+    const Img &I = B.img[B.sbImg[sb]];
+    const DevTables &T = *B.T;
+    const uint32_t local = sb - I.sbBase;
+    const int sbc = (int)(local % I.sbCols), sbr = (int)(local / I.sbCols), sbx = sbc * 64, sby = sbr * 64;
+    const int tid = threadIdx.x;
+    int tc = 0, tr = 0;
+    while (tc + 1 < I.tileCols && sbx >= I.tileColStart[tc + 1] * 4) tc++;
+    while (tr + 1 < I.tileRows && sby >= I.tileRowStart[tr + 1] * 4) tr++;
+    const int tx0 = I.tileColStart[tc] * 4, ty0 = I.tileRowStart[tr] * 4;
+    const int miColEnd = I.tileColStart[tc + 1], miRowEnd = I.tileRowStart[tr + 1];
+    // clear_block_decoded_flags (spec 7.3.3)
+    for (int k = tid; k < 3 * 18 * 18; k += RC_NT) {
+        const int p = k / 324, yy = (k / 18) % 18 - 1, xx = k % 18 - 1;
+        const int ss = p ? I.ss : 0;
+        const int sbW4 = (miColEnd - sbc * 16) >> ss, sbH4 = (miRowEnd - sbr * 16) >> ss;
+        uint8_t v = (yy < 0 && xx < sbW4) ? 1 : (xx < 0 && yy < sbH4) ? 1 : 0;
+        if (yy == (16 >> ss) && xx == -1) v = 0;
+        S.bdm[p][yy + 1][xx + 1] = v;
+    }
+    if (tid == 0) { S.nTxb = 0; S.nCoef = 0; }
+    __syncthreads();
+    const float lambda = T.lambda[I.bd == 10][I.qindex] * P.lambdaMul;
+    const int nLeaves = B.leafCount[sb];
+    for (int li = 0; li < nLeaves; li++) {
+        const Leaf lf = B.leaves[(size_t)sb * MAX_LEAVES + li];
+        const int bsize = lf.bsize, x = sbx + lf.x4 * 4, y = sby + lf.y4 * 4, mi_row = y >> 2, mi_col = x >> 2;
+        auto inside = [&](int r, int c) { return c >= (tx0 >> 2) && c < miColEnd && r >= (ty0 >> 2) && r < miRowEnd; };
+        const int availU = inside(mi_row - 1, mi_col), availL = inside(mi_row, mi_col - 1);
+        const int hasC = has_chroma(bsize, mi_row, mi_col, I.mono, I.ss);
+        const int availUC = hasC ? ((I.ss && bh4(bsize) == 1) ? inside(mi_row - 2, mi_col) : availU) : 0;
+        const int availLC = hasC ? ((I.ss && bw4(bsize) == 1) ? inside(mi_row, mi_col - 2) : availL) : 0;
+        const uint32_t firstTxb = (uint32_t)S.nTxb;
+        const int txSz = av1t::Max_Tx_Size_Rect[bsize];
+#define RC_LUMA(w, h) rc_luma<w, h, Px>(S, B, P, sb, I, bsize, x, y, availU, availL, tx0, ty0, lambda)
+        AV1M_TX_SWITCH(txSz, RC_LUMA)
+#undef RC_LUMA
+        if (hasC) {
+            const int uvTx = uv_tx_size(bsize, I.ss);
+#define RC_CHROMA(w, h) rc_chroma<w, h, Px>(S, B, P, sb, I, bsize, mi_row, mi_col, availUC, availLC, tx0, ty0, lambda * P.lambdaCMul)
+            switch (uvTx) {
+            case TX_4X4: RC_CHROMA(4, 4); break; case TX_8X8: RC_CHROMA(8, 8); break; case TX_16X16: RC_CHROMA(16, 16); break;
+            case TX_32X32: RC_CHROMA(32, 32); break; case TX_4X8: RC_CHROMA(4, 8); break; case TX_8X4: RC_CHROMA(8, 4); break;
+            case TX_8X16: RC_CHROMA(8, 16); break; case TX_16X8: RC_CHROMA(16, 8); break; case TX_16X32: RC_CHROMA(16, 32); break;
+            case TX_32X16: RC_CHROMA(32, 16); break; case TX_4X16: RC_CHROMA(4, 16); break; case TX_16X4: RC_CHROMA(16, 4); break;
+            case TX_8X32: RC_CHROMA(8, 32); break; default: RC_CHROMA(32, 8); break;
+            }
+#undef RC_CHROMA
+        } else if (tid == 0) {
+            S.winEob[1] = S.winEob[2] = 0; S.uvMode = 0; S.uvDelta = 0; S.cflAlpha[0] = S.cflAlpha[1] = 0;
+        }
+        __syncthreads();
+        const int skip = S.winEob[0] == 0 && S.winEob[1] == 0 && S.winEob[2] == 0;
+        const int yMode = S.mode, uv = S.uvMode, bw = bw4(bsize), bh = bh4(bsize);
+        for (int k = tid; k < bw * bh; k += RC_NT) {
+            const int r = mi_row + k / bw, c = mi_col + k % bw;
+            if (r < I.MiRows && c < I.MiCols) {
+                const size_t o = (size_t)r * I.MiCols + c;
+                I.mY[o] = (uint8_t)yMode;
+                if (hasC) I.mUV[o] = (uint8_t)uv;
+                I.mSkip[o] = (uint8_t)skip;
+                I.mMiSize[o] = (uint8_t)bsize;
+                I.mTx[o] = (uint8_t)txSz;
+            }
+        }
+        if (tid == 0) {
+            RecBlock b;
+            b.mi_row = (uint16_t)mi_row; b.mi_col = (uint16_t)mi_col; b.bsize = (uint8_t)bsize; b.skip = (uint8_t)skip; b.tx_depth = 0;
+            b.y_mode = (uint8_t)yMode;
+            b.angle_delta_y = (int8_t)(bsize >= BLOCK_8X8 && is_dir(yMode) ? S.delta : 0);
+            b.uv_mode = (uint8_t)(hasC ? uv : 0);
+            b.angle_delta_uv = (int8_t)(hasC && bsize >= BLOCK_8X8 && is_dir(uv) ? S.uvDelta : 0);
+            const bool cfl = hasC && uv == UV_CFL_PRED;
+            b.cfl_alpha_u = (int8_t)(cfl ? S.cflAlpha[0] : 0); b.cfl_alpha_v = (int8_t)(cfl ? S.cflAlpha[1] : 0);
+            b.filter_intra = 0; b.cdef_idx = 0; b.sb_qindex = 0;
+            B.oBlocks[(size_t)sb * MAX_LEAVES + li] = b;
+            B.oBlockTxb[(size_t)sb * MAX_LEAVES + li] = (uint16_t)firstTxb;
+            atomicAdd(&B.stats[0], 1u);
+            if (skip) atomicAdd(&B.stats[1], 1u);
+            if (cfl) atomicAdd(&B.stats[2], 1u);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        SbOut o;
+        o.nBlocks = (uint32_t)nLeaves; o.nTxb = (uint32_t)S.nTxb; o.nCoeffs = (uint32_t)S.nCoef; o.pad = 0;
+        B.sbOut[sb] = o;
+    }
+}
+
+size_t recon_smem_bytes() { return sizeof(RcShared); }
+
+int recon_prepare(int bd) {
+    const hipError_t e = bd == 10 ? hipFuncSetAttribute((const void *)k_recon<uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(RcShared))
+                                  : hipFuncSetAttribute((const void *)k_recon<uint8_t>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(RcShared));
+    return e == hipSuccess ? 0 : -1;
+}
+
+void launch_recon(const Batch &B, const SearchParams &P, const uint32_t *dSbList, uint32_t n, int bd, hipStream_t st) {
+    if (!n) return;
+    if (bd == 10) hipLaunchKernelGGL(k_recon<uint16_t>, dim3(n), dim3(RC_NT), sizeof(RcShared), st, B, P, dSbList);
+    else hipLaunchKernelGGL(k_recon<uint8_t>, dim3(n), dim3(RC_NT), sizeof(RcShared), st, B, P, dSbList);
+}
+
+}  // namespace av1m

@@ -1,0 +1,128 @@
+/* av1mi.h - C ABI of the MI355X-native AV1 intra rate-distortion core (libav1mi.so).
+ *
+ * The drop-in boundary for the reference's GPU intra RD path.  The entry points mirror the reference's C++ encoder API
+ * (maddog107/av1gpu encoder/src/pipeline.h:191-216: GpEncoder::init / encode / config, gp_check_partitions) and its
+ * launch layer (encoder/src/kernels.h:11-82), flattened to plain C: an opaque encoder handle, POD structs, plain
+ * pointers and sizes, a hipStream_t passed as void *.  Host C++ (include/av1mi_encoder.hpp) and Python
+ * (av1gpu_amd/__init__.py) sit on top of these functions only.
+ *
+ * Output: for every frame, the DecisionRecord of the reference's av1core (av1core/include/av1g/decisions.h:225-322:
+ * 16-byte av1g_block / av1g_txb entries in decode order, row-major int16 levels) and the exact reconstruction.
+ * Errors: 0 = success, -1 = failure with a message in err (the reference's int 0 / -1 + std::string err convention).
+ */
+#ifndef AV1MI_H
+#define AV1MI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- the DecisionRecord entries (decisions.h:225-252; identical 16-byte layouts) ---- */
+typedef struct av1mi_block {
+    uint16_t mi_row, mi_col;
+    uint8_t bsize, skip, tx_depth, y_mode;
+    int8_t angle_delta_y;
+    uint8_t uv_mode;
+    int8_t angle_delta_uv, cfl_alpha_u, cfl_alpha_v;
+    uint8_t filter_intra;
+    int8_t cdef_idx;
+    uint8_t sb_qindex;
+} av1mi_block;
+typedef struct av1mi_txb {
+    uint32_t coeff_off;
+    uint16_t x4, y4;
+    uint8_t plane, tx_size, tx_type, flags;
+    uint16_t eob, reserved;
+} av1mi_txb;
+
+/* Search knobs: the reference's GpParams (encoder/src/gp.h:29-68) fields this core implements, same names and meaning.
+ * av1mi_default_params() gives the plain (maximum-effort) search: K = Krecheck = 6, every tx type in the closed loop,
+ * angle deltas, chroma deltas, CfL, uvK = 6, all 19 shapes, all 10 partitions. */
+typedef struct av1mi_params {
+    float lambdaMul, satdLambda, rnd, zbin, rndC, zbinC;
+    int K, Krecheck, txSearch, txSearchCL, txAllCL, deltas, uvDeltas, cfl, uvK, shapeMask, partMask8, partMaskN, maxPart;
+    float uvOverheadBits, splitBias, lambdaCMul;
+} av1mi_params;
+
+/* GpConfig (pipeline.h:44-105) subset: qindex, the search knobs, enable_intra_edge_filter. */
+typedef struct av1mi_config {
+    int qindex;        /* base_q_idx 1..255 of frames that do not set their own */
+    int edgeFilter;    /* enable_intra_edge_filter */
+    av1mi_params P;
+} av1mi_config;
+
+/* GpFrameIn (pipeline.h:149-163): device planes padded (edge replicated) to aw x ah, multiples of 64; chroma
+ * (aw >> 1) x (ah >> 1) for 4:2:0.  8-bit: uint8 samples; 10-bit: uint16 samples; pitches in samples.  qindex: this
+ * frame's base_q_idx, or 0 for the config's (per-image qindex inside one batch is allowed). */
+typedef struct av1mi_frame_in {
+    int W, H, mono, yuv444, bitDepth;
+    const void *dsrc[3];
+    int spitch[3];
+    int aw, ah;
+    int qindex;
+} av1mi_frame_in;
+
+/* GpFrameOut (pipeline.h:165-178): host arrays owned by the library until av1mi_frame_out_free().  recon: the padded
+ * reconstruction planes (aw x ah, chroma >> 1; 2 bytes per sample for 10-bit), NULL when not requested. */
+typedef struct av1mi_frame_out {
+    uint32_t width, height, base_q_idx, bit_depth, chroma_format, monochrome;
+    uint32_t n_blocks, n_txb;
+    uint64_t n_coeffs;
+    av1mi_block *blocks;
+    uint32_t *block_txb; /* n_blocks + 1 */
+    av1mi_txb *txb;
+    int16_t *coeffs;
+    void *recon[3];
+    int recon_pitch[3];
+    int aw, ah;
+    int tile_cols, tile_rows;
+} av1mi_frame_out;
+
+/* GpTimes (pipeline.h:180-189): milliseconds from HIP events, and counts */
+typedef struct av1mi_times {
+    float setup, analysis, dp, recon, pack, download, total;
+    uint32_t steps, sbs, blocks, skipBlocks, cflBlocks, rangeFixes;
+    size_t deviceBytes;
+} av1mi_times;
+
+typedef struct av1mi_encoder av1mi_encoder;
+
+void av1mi_default_params(av1mi_params *p);
+/* gp_check_partitions (pipeline.h:216): 0, or -1 when the shape / partition options leave some superblock no
+ * partition of finite cost. */
+int av1mi_check_partitions(const av1mi_params *p, char *err, int errlen);
+/* GpEncoder::init (pipeline.h:195): selects the current HIP device, builds and uploads the tables. */
+int av1mi_create(const av1mi_config *cfg, av1mi_encoder **out, char *err, int errlen);
+void av1mi_destroy(av1mi_encoder *enc);
+/* GpEncoder::config (pipeline.h:209) */
+void av1mi_get_config(const av1mi_encoder *enc, av1mi_config *cfg);
+/* GpEncoder::encode (pipeline.h:196-204): a batch of n frames (all of one bit depth) in flight at once on stream
+ * (hipStream_t; NULL = default).  The input planes must stay valid until the call returns (it synchronises).
+ * want_recon: also download the reconstruction. */
+int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_frame_out *out, av1mi_times *t, void *stream,
+                 int want_recon, char *err, int errlen);
+void av1mi_frame_out_free(av1mi_frame_out *out);
+
+/* Device-buffer helpers for callers without their own allocator (hipMalloc / hipFree / hipMemcpy). */
+int av1mi_dev_alloc(void **ptr, size_t bytes);
+int av1mi_dev_free(void *ptr);
+int av1mi_copy_h2d(void *dst, const void *src, size_t bytes);
+int av1mi_device_sync(void);
+
+/* Seeded synthetic test image (the benchmark generator): W x H planes, chroma ((W + ssx) >> ssx) x ((H + ssx) >> ssx);
+ * bd 8: uint8 samples, bd 10: uint16; strides in samples.  Host memory. */
+int av1mi_synth_image(uint64_t seed, int W, int H, int ssx, int bd, void *y, void *u, void *v, const int *strides);
+
+/* Host copies of the device tables, for the oracle comparison tests: forward matrix (kind 0 DCT / 1 ADST / 2 identity,
+ * log2 size) -> n * n floats or 0; the other tables as in the oracle's orc_misc_tables / orc_cost_table. */
+int av1mi_fwd_matrix(int kind, int ln, float *out);
+void av1mi_misc_tables(float *txg, float *txScale, float *avgY, float *avgPart, float *avgSkip0, float *lambda, float *lamS);
+int av1mi_cost_table(int qctx, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

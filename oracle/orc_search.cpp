@@ -511,7 +511,7 @@ static void analyse_block(Frame &F, const Params &P, int bs, int bx, int by) {
     const int maxX = F.MiCols * 4 - 1, maxY = F.MiRows * 4 - 1;
     const int sp = F.stride[0];
     const uint16_t *src = F.src[0].data() + (size_t)y * sp + x;
-    const float lambda = T.lambda[bd == 10][F.qindex] * 1.0f, lamS = T.lamS[bd == 10][F.qindex];
+    const float lambda = T.lambda[bd == 10][F.qindex] * P.lambdaMul, lamS = P.satdLambda * T.lamS[bd == 10][F.qindex];
     const CostTab &C = T.cost[F.qctx];
     uint16_t ea[EDGE_LEN], el[EDGE_LEN];
     uint16_t *A = ea + EDGE_OFF, *L = el + EDGE_OFF;
@@ -605,7 +605,7 @@ static void dp_sb(Frame &F, const Params &P, int sb) {
     const Tables &T = tables();
     const int sbx = (sb % F.sbCols) * 64, sby = (sb / F.sbCols) * 64;
     const int FW = F.MiCols * 4, FH = F.MiRows * 4;
-    const float lambda = T.lambda[F.bd == 10][F.qindex];
+    const float lambda = T.lambda[F.bd == 10][F.qindex] * P.lambdaMul;
     auto blockJ = [&](int bs, int x, int y) -> float {
         if (bs >= BLOCK_INVALID || F.nbx[bs] == 0 || !((P.shapeMask >> bs) & 1)) return INF;
         const int w = 4 * bw4(bs), h = 4 * bh4(bs);
@@ -945,7 +945,7 @@ static ChromaOut rc_chroma(Frame &F, const Params &P, SbState &S, int sb, int bs
     const uint16_t *src[3] = {nullptr, F.src[1].data() + (size_t)cy * F.stride[1] + cx, F.src[2].data() + (size_t)cy * F.stride[2] + cx};
     const int sp = F.stride[1];
     const bool cflOk = P.cfl && cfl_allowed(bsize);
-    const float lamS = T.lamS[bd == 10][F.qindex];   // satdLambda * sqrt(lambda): the chroma screening uses the luma lambda's
+    const float lamS = P.satdLambda * T.lamS[bd == 10][F.qindex];   // the luma lambda's screening weight
     auto uvbits = [&](int m) -> float { return cfl_allowed(bsize) ? C.uv_cfl_yes[yMode][m] : C.uv_cfl_no[yMode][m]; };
     auto uvcbits = [&](int c) -> float {
         const int m = cand_mode(c);
@@ -1132,7 +1132,7 @@ void closed_loop_sb(Frame &F, const Params &P, int sb) {
     }
     S.nTxb = 0; S.nCoef = 0;
     F.sbBlocks[sb].clear(); F.sbBlockTxb[sb].clear(); F.sbTxb[sb].clear(); F.sbCoef[sb].clear();
-    const float lambda = T.lambda[F.bd == 10][F.qindex] * 1.0f;
+    const float lambda = T.lambda[F.bd == 10][F.qindex] * P.lambdaMul;
     for (const Leaf &lf : F.leaves[sb]) {
         const int bsize = lf.bsize, x = sbx + lf.x4 * 4, y = sby + lf.y4 * 4, mi_row = y >> 2, mi_col = x >> 2;
         auto inside = [&](int r, int c) { return c >= (tb.x0 >> 2) && c < miColEnd && r >= (tb.y0 >> 2) && r < miRowEnd; };
