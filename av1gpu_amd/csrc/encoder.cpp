@@ -317,6 +317,8 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
         I.cLvl[0] = c; I.cLvl[1] = c + mi; I.cLvl[2] = c + mi + c1;
         I.cDc[0] = c + mi + 2 * c1; I.cDc[1] = c + 2 * mi + 2 * c1; I.cDc[2] = c + 2 * mi + 3 * c1;
         HCK(hipMemsetAsync(m, 0, 5 * mi + 2 * (mi + 2 * c1), st));
+        // the padding beyond the coded area is never reconstructed: zero, so the returned planes are fully defined
+        HCK(hipMemsetAsync(recMem[i], 0, ysz + 2 * csz, st));
     }
     HCK(hipMemsetAsync(dStats, 0, 8 * 4, st));
     HCK(hipMemcpyAsync(dImg, imgs.data(), n * sizeof(Img), hipMemcpyHostToDevice, st));

@@ -1,0 +1,119 @@
+"""Benchmark of the intra RD core on one MI355X.
+
+A step is one encode (analysis -> partition DP -> closed-loop reconstruction -> pack -> records and reconstruction on
+the host) of the workload with its input planes already resident in device memory.  Default workload: the plain
+exhaustive search on one 4000 x 3000 8-bit 4:2:0 synthetic image (seed 1000) at qindex 120.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+roofline: the dominant kernel's algorithmic HBM bytes (source planes read once + reconstruction written once +
+2 bytes per coefficient + 16 bytes per block and tx-block record) per launch over its average launch duration, timed
+with HIP events on the encode stream.  cpu_baseline: the CPU restatement (oracle/liborc.so) on a bounded crop of the
+same image with all host threads.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MP/s at qindex 120, batched+learned vs plain, at <=+1% BD-rate; p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--width", type=int, default=4000)
+    ap.add_argument("--height", type=int, default=3000)
+    ap.add_argument("--qindex", type=int, default=120)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the path shards by image / tile with no collective); "
+              "--gpus %d is not supported" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    import av1gpu_amd as A
+
+    W, H = a.width, a.height
+    planes = A.synth_image(1000, W, H)
+    frame = A.DeviceFrame(planes, W, H, qindex=a.qindex)
+    enc = A.GpEncoder().init(A.GpConfig(qindex=a.qindex))
+    for _ in range(a.warmup):
+        enc.encode([frame], want_recon=True)
+    A.lib().av1mi_device_sync()
+    times, stage = [], []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        s0 = time.perf_counter()
+        out, t = enc.encode([frame], want_recon=True)
+        times.append(time.perf_counter() - s0)
+        stage.append(t)
+    A.lib().av1mi_device_sync()
+    elapsed = time.perf_counter() - t0
+    mp = W * H / 1e6
+    value = mp * a.steps / elapsed
+    rec = out[0]
+    n_coeffs, n_blocks, n_txb = len(rec.coeffs), len(rec.blocks), len(rec.txb)
+    alg_bytes = W * H * 1 * 1.5 * 2 + 2 * n_coeffs + 16 * (n_blocks + n_txb)
+    recon_ms = sum(s.recon for s in stage) / len(stage)
+    analysis_ms = sum(s.analysis for s in stage) / len(stage)
+    steps = stage[-1].steps
+    # dominant kernel: the closed loop (one launch per wavefront step); per launch: the image's bytes / steps
+    per_launch_ms = recon_ms / max(steps, 1)
+    achieved = (alg_bytes / max(steps, 1)) / (per_launch_ms * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "MP/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 * elapsed / a.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": "plain exhaustive intra search, one %dx%d 8-bit 4:2:0 synthetic image (seed 1000), "
+                               "qindex %d" % (W, H, a.qindex), "width": W, "height": H, "qindex": a.qindex,
+                   "parallelism": "single GPU"},
+        "stages_ms": {"analysis": round(analysis_ms, 3), "dp": round(sum(s.dp for s in stage) / len(stage), 3),
+                      "recon": round(recon_ms, 3), "pack": round(sum(s.pack for s in stage) / len(stage), 3),
+                      "download": round(sum(s.download for s in stage) / len(stage), 3),
+                      "total_events": round(sum(s.total for s in stage) / len(stage), 3)},
+        "record": {"blocks": n_blocks, "txb": n_txb, "coeffs": n_coeffs, "wavefront_steps": steps, "sbs": stage[-1].sbs},
+        "roofline": {"bound": "hbm", "kernel": "k_recon", "achieved": round(achieved, 4), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "per_launch_ms": round(per_launch_ms, 4), "alg_bytes_per_launch": round(alg_bytes / max(steps, 1))},
+    }
+    if not a.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(planes, W, H, a.qindex)
+    print(json.dumps(line), flush=True)
+    frame.close()
+    enc.close()
+
+
+def cpu_baseline(planes, W, H, qindex):
+    """The oracle (the CPU restatement of the same search) on a 1024 x 768 crop of the same image, all host threads."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import refcheck
+    cw, ch = min(1024, W), min(768, H)
+    crop = [planes[0][:ch, :cw], planes[1][:(ch + 1) // 2, :(cw + 1) // 2], planes[2][:(ch + 1) // 2, :(cw + 1) // 2]]
+    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1))
+    o = refcheck.Oracle()
+    t0 = time.perf_counter()
+    o.encode(crop, cw, ch, qindex=qindex, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": round(cw * ch / 1e6 / dt, 5), "unit": "MP/s", "cores": threads, "kind": "port",
+            "sample": "oracle/liborc.so, the same search on the top-left %dx%d crop of the benchmark image, %d threads, "
+                      "%.2f s" % (cw, ch, threads, dt)}
+
+
+if __name__ == "__main__":
+    main()

@@ -156,3 +156,12 @@ int orc_tile_layout(int W, int H, int *cols, int *rows, int *colStart, int *rowS
 }
 
 }  // extern "C"
+
+extern "C" {
+// one 1-D inverse transform (kind 0 DCT / 1 ADST / 2 identity, log2 size n, clamp range r); returns the flags
+uint32_t orc_inv_1d(int32_t *T, int kind, int n, int r) {
+    uint32_t err = 0;
+    inv_1d(T, kind, n, r, &err);
+    return err;
+}
+}
