@@ -13,6 +13,21 @@ namespace av1m {
 
 constexpr int GW = 32;   // lanes per group
 
+// LDS view of a type: accesses through it compile to ds_* instructions, also inside functions the compiler keeps out
+// of line (where a plain pointer would lose its address space and go through flat instructions)
+template <typename T> using lds = __attribute__((address_space(3))) T;
+// read-only global data seen through the constant address space: wave-uniform indices load through the scalar unit
+template <typename T> using cst = __attribute__((address_space(4))) const T;
+// the generic pointer to an LDS object (for library calls such as atomicAdd)
+template <typename T> __device__ __forceinline__ T *gen(lds<T> *p) { return (T *)p; }
+// p (the same value in every lane) as a wave-uniform constant-space pointer; pointers reach out-of-line functions in
+// vector registers, so uniformity has to be restated for scalar loads
+template <typename T> __device__ __forceinline__ cst<T> *uni(const T *p) {
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (cst<T> *)(((uint64_t)hi << 32) | lo);
+}
+
 __device__ __forceinline__ int glane() { return threadIdx.x & 31; }
 __device__ __forceinline__ int gid() { return threadIdx.x >> 5; }
 // orders the group's LDS accesses (one wave executes both groups in lockstep; this stops compiler reordering)
@@ -47,18 +62,22 @@ __device__ __forceinline__ int gargmin(float v, int i) {
     return v < INF ? i : -1;
 }
 
-struct EvalIn {
+// inputs of one tx block evaluation; CP: pointer to the cost tables (global, or an LDS copy)
+template <typename CP>
+struct EvalInT {
     int txSz, txType, plane, dcQ, acQ, skipCtx, dcCtx, intraDir, bd;
     float rnd, zbin, resEnergy;
-    const CostTab *C;
+    CP C;
 };
+using EvalIn = EvalInT<const CostTab *>;
 struct EvalOut { float D, R; int eob; };
 
 __device__ __forceinline__ int txsz_ctx(int t) { return (av1t::Tx_Size_Sqr[t] + av1t::Tx_Size_Sqr_Up[t] + 1) >> 1; }
 
 // txb_skip, luma tx type and eob position bits
-__device__ float base_bits(const EvalIn &e, int eob) {
-    const CostTab &C = *e.C;
+template <typename E>
+__device__ float base_bits(const E &e, int eob) {
+    const auto &C = *e.C;
     const int t = e.txSz, ctx = txsz_ctx(t);
     if (eob == 0) return C.txb_skip[ctx][e.skipCtx][1];
     float b = C.txb_skip[ctx][e.skipCtx][0];
@@ -91,7 +110,8 @@ __device__ float base_bits(const EvalIn &e, int eob) {
 }
 
 // bits of the coefficient at scan position c (< eob) given the levels q (natural order, stride = adjusted width)
-__device__ float coeff_bits(int c, int eob, const int16_t *q, const uint16_t *scan, int t, int cls, int pt, int dcCtx, const CostTab &C) {
+template <typename QP, typename CT>
+__device__ float coeff_bits(int c, int eob, QP q, const uint16_t *scan, int t, int cls, int pt, int dcCtx, const CT &C) {
     const int ctx = txsz_ctx(t), adj = av1t::Adjusted_Tx_Size[t];
     const int bwl = av1t::Tx_Width_Log2[adj], width = 1 << bwl, height = av1t::Tx_Height[adj];
     const int pos = scan[c], qv = q[pos], v = qv < 0 ? -qv : qv, level = v < 15 ? v : 15;
@@ -122,7 +142,7 @@ __device__ float coeff_bits(int c, int eob, const int16_t *q, const uint16_t *sc
         mag = min((mag + 1) >> 1, 6);
         const int brctx = pos == 0 ? mag : cls == TX_CLASS_2D ? ((row < 2 && col < 2) ? mag + 7 : mag + 14)
                         : cls == TX_CLASS_HORIZ ? (col == 0 ? mag + 7 : mag + 14) : (row == 0 ? mag + 7 : mag + 14);
-        const float *br = C.coeff_br[min(ctx, (int)TX_32X32)][pt][brctx];
+        const auto *br = C.coeff_br[min(ctx, (int)TX_32X32)][pt][brctx];
         int rem = level - 3;
         for (int i = 0; i < 4; i++) {
             const int k = min(rem, 3);
@@ -147,21 +167,22 @@ struct EvalScr {
     int16_t q[(W > 32 ? 32 : W) * TH];
 };
 
+template <int W, int H, typename FP, typename QP, typename E, typename TT>
+__device__ EvalOut grp_qrate(FP tmp, int ts, QP q, const E &e, const TT &T);
+
 // Forward transform + quantization + distortion + rate of the residual in S.res (W x H, row-major), by the 32 lanes of
 // the calling group.  Levels in S.q (natural order, tw x th).  All lanes return the same values.
-template <int W, int H, typename Scr>
-__device__ EvalOut grp_eval(Scr &S, const EvalIn &e, const DevTables &T) {
+template <int W, int H, typename Scr, typename E, typename TT>
+__device__ EvalOut grp_eval(Scr &S, const E &e, const TT &T) {
     constexpr int TW = W > 32 ? 32 : W, TH = H > 32 ? 32 : H, TS = Scr::TS;
     constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
-    constexpr int LTW = LW > 5 ? 5 : LW;
-    constexpr int N = TW * TH;
     const int lane = glane();
     const int rk = row_kind(e.txType), ck = col_kind(e.txType), fud = flip_ud(e.txType), flr = flip_lr(e.txType);
     // ---- column pass: item = (column j, OC consecutive outputs) ----
     {
         constexpr int OC = TH < 8 ? TH : 8, NI = W * (TH / OC);
-        const float *FcT = T.fwdT + T.fwdOff[ck][LH];
-        const float *Fc = T.fwd + T.fwdOff[ck][LH];
+        const auto *FcT = T.fwdT + T.fwdOff[ck][LH];
+        const auto *Fc = T.fwd + T.fwdOff[ck][LH];
         for (int it = lane; it < NI; it += GW) {
             const int j = it % W, i0 = (it / W) * OC, jj = flr ? W - 1 - j : j;
             float acc[OC];
@@ -174,7 +195,7 @@ __device__ EvalOut grp_eval(Scr &S, const EvalIn &e, const DevTables &T) {
 #pragma unroll 4
                 for (int k = 0; k < H; k++) {
                     const float x = (float)S.res[(fud ? H - 1 - k : k) * W + jj];
-                    const float *f = FcT + k * H + i0;
+                    const auto *f = FcT + k * H + i0;
 #pragma unroll
                     for (int o = 0; o < OC; o++) acc[o] = fmaf(f[o], x, acc[o]);
                 }
@@ -186,12 +207,12 @@ __device__ EvalOut grp_eval(Scr &S, const EvalIn &e, const DevTables &T) {
     gsync();
     // ---- row pass, in place: lane i owns row i (reads it whole, then writes its tw outputs) ----
     {
-        const float *FrT = T.fwdT + T.fwdOff[rk][LW];
-        const float *Fr = T.fwd + T.fwdOff[rk][LW];
+        const auto *FrT = T.fwdT + T.fwdOff[rk][LW];
+        const auto *Fr = T.fwd + T.fwdOff[rk][LW];
         const float scale = T.txScale[e.txSz];
         if (lane < TH) {
             float acc[TW];
-            float *row = S.tmp + lane * TS;
+            auto *row = S.tmp + lane * TS;
             if (rk == K_IDT) {
 #pragma unroll
                 for (int o = 0; o < TW; o++) acc[o] = Fr[o * W + o] * row[o];
@@ -201,7 +222,7 @@ __device__ EvalOut grp_eval(Scr &S, const EvalIn &e, const DevTables &T) {
 #pragma unroll 2
                 for (int k = 0; k < W; k++) {
                     const float x = row[k];
-                    const float *f = FrT + k * W;
+                    const auto *f = FrT + k * W;
 #pragma unroll
                     for (int o = 0; o < TW; o++) acc[o] = fmaf(f[o], x, acc[o]);
                 }
@@ -211,17 +232,28 @@ __device__ EvalOut grp_eval(Scr &S, const EvalIn &e, const DevTables &T) {
         }
     }
     gsync();
+    return grp_qrate<W, H>(S.tmp, TS, S.q, e, T);
+}
+
+// Quantization, distortion and rate of the coefficients of a W x H tx block held in tmp (row i of the kept tw x th
+// region at tmp + i * ts), levels into q (natural order, tw x th).  All lanes of the group return the same values.
+template <int W, int H, typename FP, typename QP, typename E, typename TT>
+__device__ EvalOut grp_qrate(FP tmp, int ts, QP q, const E &e, const TT &T) {
+    constexpr int TW = W > 32 ? 32 : W, TH = H > 32 ? 32 : H;
+    constexpr int LTW = TW == 4 ? 2 : TW == 8 ? 3 : TW == 16 ? 4 : 5;
+    constexpr int N = TW * TH;
+    const int lane = glane();
     // ---- quantization + distortion (virtual lane = physical lane) ----
     const int den = dq_denom(e.txSz), dsh = den == 4 ? 2 : den == 2 ? 1 : 0;
     float dist = 0.f, kept = 0.f;
     for (int idx = lane; idx < N; idx += GW) {
-        const float v = S.tmp[(idx >> LTW) * TS + (idx & (TW - 1))];
+        const float v = tmp[(idx >> LTW) * ts + (idx & (TW - 1))];
         const int step = idx == 0 ? e.dcQ : e.acQ;
         const float a = fabsf(v) * (float)den / (float)step;
         int l = a < e.zbin ? 0 : (int)(a + e.rnd);
         l = min(l, 16383);
         const int qv = v < 0 ? -l : l;
-        S.q[idx] = (int16_t)qv;
+        q[idx] = (int16_t)qv;
         const float dq = l ? (float)dequant(qv, step, dsh, e.bd) : 0.f;
         const float d = v - dq;
         const float sq = d * d;
@@ -233,11 +265,11 @@ __device__ EvalOut grp_eval(Scr &S, const EvalIn &e, const DevTables &T) {
     const uint16_t *scan = scan_of(e.txSz, e.txType);
     int last = 0;
     for (int c = lane; c < N; c += GW)
-        if (S.q[scan[c]]) last = c + 1;
+        if (q[scan[c]]) last = c + 1;
     const int eob = gmaxi(last);
     float bits = lane == 0 ? base_bits(e, eob) : 0.f;
     const int cls = tx_class(e.txType);
-    for (int c = lane; c < eob; c += GW) bits += coeff_bits(c, eob, S.q, scan, e.txSz, cls, e.plane > 0, e.dcCtx, *e.C);
+    for (int c = lane; c < eob; c += GW) bits += coeff_bits(c, eob, q, scan, e.txSz, cls, e.plane > 0, e.dcCtx, *e.C);
     EvalOut o;
     o.R = gsum(bits);
     const float g = T.txg[e.txSz][e.txType];
@@ -251,8 +283,8 @@ __device__ EvalOut grp_eval(Scr &S, const EvalIn &e, const DevTables &T) {
 }
 
 // sum of r^2 over a W x H residual in raster order (32 virtual lanes)
-template <int W, int H>
-__device__ float grp_res_energy(const int16_t *res) {
+template <int W, int H, typename RP>
+__device__ float grp_res_energy(RP res) {
     float e = 0.f;
     for (int p = glane(); p < W * H; p += GW) e += (float)(res[p] * res[p]);
     return gsum(e);
@@ -279,7 +311,7 @@ __device__ void grp_edge_filter(P *buf, int sz, int strength) {
     gsync();
 #pragma unroll
     for (int k = 0; k < 5; k++)
-        if (lane + 1 + GW * k < sz) buf[lane + GW * k] = (P)v[k];
+        if (lane + 1 + GW * k < sz) buf[lane + GW * k] = v[k];
     gsync();
 }
 template <typename P>
@@ -291,9 +323,9 @@ __device__ void grp_edge_upsample(P *buf, int numPx, int bd) {   // numPx <= 16
         s = clip3(0, (1 << bd) - 1, round2(-dup(lane) + 9 * dup(lane + 1) + 9 * dup(lane + 2) - dup(lane + 3), 4));
         e = dup(lane + 2);
     }
-    const P c = buf[-1];
+    const int c = buf[-1];
     gsync();
-    if (lane < numPx) { buf[2 * lane - 1] = (P)s; buf[2 * lane] = (P)e; }
+    if (lane < numPx) { buf[2 * lane - 1] = s; buf[2 * lane] = e; }
     if (lane == 0) buf[-2] = c;
     gsync();
 }
@@ -305,7 +337,7 @@ __device__ void grp_edge_prepare(int pa, int lw, int lh, int ft, int haveL, int 
     if (pa != 90 && pa != 180) {
         if (pa > 90 && pa < 180 && w + h >= 24) {
             if (glane() == 0) {
-                const P c = (P)round2(L[0] * 5 + A[-1] * 6 + A[0] * 5, 4);
+                const int c = round2(L[0] * 5 + A[-1] * 6 + A[0] * 5, 4);
                 L[-1] = c;
                 A[-1] = c;
             }
@@ -327,22 +359,22 @@ __device__ void grp_build_edges(const SP *plane, int pitch, int x, int y, int lw
     const int w = 1 << lw, h = 1 << lh, n = w + h;
     const int alim = min(x + (haveAR ? 2 * w : w) - 1, maxX), llim = min(y + (haveBL ? 2 * h : h) - 1, maxY);
     for (int i = glane(); i < n; i += GW) {
-        P a, l;
+        int a, l;
         if (!haveA && haveL) a = plane[(size_t)y * pitch + x - 1];
-        else if (!haveA) a = (P)((1 << (bd - 1)) - 1);
+        else if (!haveA) a = (1 << (bd - 1)) - 1;
         else a = plane[(size_t)(y - 1) * pitch + min(x + i, alim)];
         if (!haveL && haveA) l = plane[(size_t)(y - 1) * pitch + x];
-        else if (!haveL) l = (P)((1 << (bd - 1)) + 1);
+        else if (!haveL) l = (1 << (bd - 1)) + 1;
         else l = plane[(size_t)min(y + i, llim) * pitch + x - 1];
         A[i] = a;
         L[i] = l;
     }
     if (glane() == 0) {
-        P c;
+        int c;
         if (haveA && haveL) c = plane[(size_t)(y - 1) * pitch + x - 1];
         else if (haveA) c = plane[(size_t)(y - 1) * pitch + x];
         else if (haveL) c = plane[(size_t)y * pitch + x - 1];
-        else c = (P)(1 << (bd - 1));
+        else c = 1 << (bd - 1);
         A[-1] = c;
         L[-1] = c;
     }

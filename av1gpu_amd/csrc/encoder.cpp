@@ -80,7 +80,7 @@ struct Arena {
         return hipSuccess;
     }
     void release(hipStream_t st) {
-        if (base) hipFreeAsync(base, st);
+        if (base) (void)hipFreeAsync(base, st);
         base = nullptr;
     }
 };
@@ -190,12 +190,13 @@ int av1mi_create(const av1mi_config *cfg, av1mi_encoder **out, char *err, int er
     free(h);
     if (he != hipSuccess) { snprintf(err, errlen, "HIP: %s (table upload)", hipGetErrorString(he)); delete e; return -1; }
     if (recon_prepare(8) || recon_prepare(10)) { snprintf(err, errlen, "closed-loop kernel needs %zu bytes of LDS", recon_smem_bytes()); delete e; return -1; }
-    for (auto &v : e->ev) hipEventCreate(&v);
+    for (auto &v : e->ev)
+        if ((he = hipEventCreate(&v)) != hipSuccess) { snprintf(err, errlen, "HIP: %s (event)", hipGetErrorString(he)); av1mi_destroy(e); return -1; }
     // keep freed stream-ordered allocations cached between batches
     hipMemPool_t pool;
     if (hipDeviceGetDefaultMemPool(&pool, e->device) == hipSuccess) {
         uint64_t thr = UINT64_MAX;
-        hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
     }
     *out = e;
     return 0;
@@ -203,8 +204,8 @@ int av1mi_create(const av1mi_config *cfg, av1mi_encoder **out, char *err, int er
 
 void av1mi_destroy(av1mi_encoder *e) {
     if (!e) return;
-    if (e->dT) hipFree(e->dT);
-    for (auto &v : e->ev) if (v) hipEventDestroy(v);
+    if (e->dT) (void)hipFree(e->dT);
+    for (auto &v : e->ev) if (v) (void)hipEventDestroy(v);
     delete e;
 }
 
@@ -219,7 +220,7 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     if (n <= 0) { if (t) *t = tm; return 0; }
     const SearchParams P = *(const SearchParams *)&enc->cfg.P;
     const int bd = in[0].bitDepth == 10 ? 10 : 8, bps = bd == 8 ? 1 : 2;
-    hipEventRecord(enc->ev[0], st);
+    (void)hipEventRecord(enc->ev[0], st);
     std::vector<Img> imgs(n);
     uint32_t shTotal[NSHAPES] = {0};
     std::vector<uint32_t> shStart((size_t)NSHAPES * (n + 1), 0), sbImg;
@@ -330,16 +331,16 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     B.nimg = n; B.img = dImg; B.T = enc->dT; B.blk = dBlk; B.shStart = dShStart; B.nSb = nSb; B.sbImg = dSbImg;
     B.leafCount = dLeafCount; B.leaves = dLeaves; B.sbJ = dSbJ; B.sbOut = dSbOut; B.oBlocks = dOB; B.oBlockTxb = dOBT;
     B.oTxb = dOT; B.oCoeffs = dOC; B.stats = dStats;
-    hipEventRecord(enc->ev[1], st);
+    (void)hipEventRecord(enc->ev[1], st);
     launch_analysis(B, P, shTotal, bd, st);
     HCK(hipGetLastError());
-    hipEventRecord(enc->ev[2], st);
+    (void)hipEventRecord(enc->ev[2], st);
     launch_dp(B, P, st);
     HCK(hipGetLastError());
-    hipEventRecord(enc->ev[3], st);
+    (void)hipEventRecord(enc->ev[3], st);
     for (size_t s = 0; s + 1 < stepOff.size(); s++) launch_recon(B, P, dSteps + stepOff[s], stepOff[s + 1] - stepOff[s], bd, st);
     HCK(hipGetLastError());
-    hipEventRecord(enc->ev[4], st);
+    (void)hipEventRecord(enc->ev[4], st);
     tm.steps = (uint32_t)steps.size();
     tm.sbs = nSb;
     // ---- pack: prefix sums in decode order (tiles raster, SBs raster inside each tile) ----
@@ -372,10 +373,10 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     R.add(rB, gBlk); R.add(rBT, gBlk); R.add(rT, gTxb); R.add(rC, gCoef);
     HCK(R.alloc(st));
     HCK(hipMemcpyAsync(dPO, po.data(), (size_t)nSb * sizeof(PackOff), hipMemcpyHostToDevice, st));
-    hipEventRecord(enc->ev[5], st);
+    (void)hipEventRecord(enc->ev[5], st);
     launch_pack(B, dPO, rB, rBT, rT, rC, st);
     HCK(hipGetLastError());
-    hipEventRecord(enc->ev[6], st);
+    (void)hipEventRecord(enc->ev[6], st);
     for (int i = 0; i < n; i++) {
         const Img &I = imgs[i];
         av1mi_frame_out &o = out[i];
@@ -401,15 +402,15 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
             }
         }
     }
-    hipEventRecord(enc->ev[7], st);
+    (void)hipEventRecord(enc->ev[7], st);
     HCK(hipStreamSynchronize(st));
     for (int i = 0; i < n; i++) out[i].block_txb[out[i].n_blocks] = out[i].n_txb;
     R.release(st);
     A.release(st);
     float ms[7];
-    for (int k = 0; k < 7; k++) hipEventElapsedTime(&ms[k], enc->ev[k], enc->ev[k + 1]);
+    for (int k = 0; k < 7; k++) (void)hipEventElapsedTime(&ms[k], enc->ev[k], enc->ev[k + 1]);
     tm.setup = ms[0]; tm.analysis = ms[1]; tm.dp = ms[2]; tm.recon = ms[3]; tm.pack = ms[5]; tm.download = ms[6] + ms[4];
-    hipEventElapsedTime(&tm.total, enc->ev[0], enc->ev[7]);
+    (void)hipEventElapsedTime(&tm.total, enc->ev[0], enc->ev[7]);
     tm.blocks = stats[0]; tm.skipBlocks = stats[1]; tm.cflBlocks = stats[2]; tm.rangeFixes = stats[3];
     if (t) *t = tm;
     return 0;

@@ -15,19 +15,37 @@
 
 namespace av1m {
 
-constexpr int RC_NT = 256, RC_NG = 8, RC_NBIG = 2;
+constexpr int RC_NT = 256, RC_NG = 8;
 
 struct RcGroup {
     EvalScr<16, 16> ev;
     uint16_t ea[EDGE_LEN], el[EDGE_LEN];
 };
-struct RcBig {
-    EvalScr<64, 64> ev;
+// scratch of one group evaluating a transform block with a side of 32 or 64 (DCT_DCT only)
+struct RcTeam {
+    float tmp[32 * 65];      // th x (W + 1): column-pass outputs, then the coefficients
+    int16_t q[1024];
     uint16_t ea[EDGE_LEN], el[EDGE_LEN];
 };
-struct RcShared {
+// the tables the small-block evaluations and the decisions read, staged once per workgroup
+struct RcTabs {
+    float fwdT[3 * 336];     // DCT, ADST, identity for n = 4, 8, 16, transposed (offsets fwdOff)
+    float fwd[3 * 336];      // the same, row-major (the identity kind reads its diagonal)
+    int fwdOff[3][7];
+    float txScale[19];
+    float txg[19][16];
+};
+using RcEvalIn = EvalInT<const lds<CostTab> *>;
+
+union RcScr {
     RcGroup g[RC_NG];
-    RcBig big[RC_NBIG];
+    RcTeam t[RC_NG];
+};
+struct RcShared {
+    RcScr u;
+    CostTab cost;                            // the cost tables of the image's quantizer context
+    RcTabs tab;
+    uint16_t srcb[64 * 64];                  // source block of the current leaf: luma W x H, or U at 0 and V at 2048
     int32_t inv[64 * 64];                    // inverse transform residual
     uint16_t ea[3][EDGE_LEN], el[3][EDGE_LEN];   // unfiltered edges of the current tx block per plane
     int dcv[3], ft[3], hvL[3], hvA[3], px[3], py[3], mx[3], my[3];
@@ -69,42 +87,45 @@ __device__ int filter_type(const Img &I, int plane, int mi_row, int mi_col, int 
     return a || l;
 }
 
-// coefficient contexts of a tx block from the context maps (one thread)
-__device__ void txb_ctx(const Img &I, int plane, int txSz, int x4, int y4, int bsize, int tx0, int ty0, int *skipCtx, int *dcCtx) {
-    const int ss = plane ? I.ss : 0;
+// coefficient contexts of a tx block from the context maps (spec get_tx_block ctx), by the 32 lanes of the group:
+// lane k reads the k-th 4x4 column above and row to the left
+__device__ void grp_txb_ctx(const Img &I, int plane, int txSz, int x4, int y4, int bsize, int tx0, int ty0, lds<int> *skipCtx,
+                            lds<int> *dcCtx) {
+    const int lane = glane(), ss = plane ? I.ss : 0;
     const int stride = I.MiCols >> ss, maxX4 = I.MiCols >> ss, maxY4 = I.MiRows >> ss;
     const uint8_t *L = I.cLvl[plane], *D = I.cDc[plane];
     const bool hasA = y4 > ((ty0 >> 2) >> ss), hasL = x4 > ((tx0 >> 2) >> ss);
     const int w = tx_w(txSz), h = tx_h(txSz), w4 = w >> 2, h4 = h >> 2;
     int top = 0, left = 0, topAny = 0, leftAny = 0, sgn = 0;
-    if (hasA)
-        for (int k = 0; k < w4 && x4 + k < maxX4; k++) {
-            const int l = L[(size_t)(y4 - 1) * stride + x4 + k], d = D[(size_t)(y4 - 1) * stride + x4 + k];
-            top = max(top, l); topAny |= l | d; sgn += d == 2 ? 1 : d == 1 ? -1 : 0;
-        }
-    if (hasL)
-        for (int k = 0; k < h4 && y4 + k < maxY4; k++) {
-            const int l = L[(size_t)(y4 + k) * stride + x4 - 1], d = D[(size_t)(y4 + k) * stride + x4 - 1];
-            left = max(left, l); leftAny |= l | d; sgn += d == 2 ? 1 : d == 1 ? -1 : 0;
-        }
-    if (plane == 0) {
-        const int bw = 4 * bw4(bsize), bh = 4 * bh4(bsize);
-        *skipCtx = (bw == w && bh == h) ? 0 : (top == 0 && left == 0) ? 1 : (top == 0 || left == 0) ? 2 + (max(top, left) > 3)
-                 : max(top, left) <= 3 ? 4 : min(top, left) <= 3 ? 5 : 6;
-    } else {
-        const int ub = uv_bsize(bsize, I.ss);
-        *skipCtx = (topAny != 0) + (leftAny != 0) + 7 + ((16 * bw4(ub) * bh4(ub) > w * h) ? 3 : 0);
+    if (hasA && lane < w4 && x4 + lane < maxX4) {
+        const int l = L[(size_t)(y4 - 1) * stride + x4 + lane], d = D[(size_t)(y4 - 1) * stride + x4 + lane];
+        top = l; topAny = (l | d) != 0; sgn += d == 2 ? 1 : d == 1 ? -1 : 0;
     }
-    *dcCtx = sgn < 0 ? 1 : sgn > 0 ? 2 : 0;
+    if (hasL && lane < h4 && y4 + lane < maxY4) {
+        const int l = L[(size_t)(y4 + lane) * stride + x4 - 1], d = D[(size_t)(y4 + lane) * stride + x4 - 1];
+        left = l; leftAny = (l | d) != 0; sgn += d == 2 ? 1 : d == 1 ? -1 : 0;
+    }
+    top = gmaxi(top); left = gmaxi(left); topAny = gmaxi(topAny); leftAny = gmaxi(leftAny); sgn = gsumi(sgn);
+    if (lane == 0) {
+        if (plane == 0) {
+            const int bw = 4 * bw4(bsize), bh = 4 * bh4(bsize);
+            *skipCtx = (bw == w && bh == h) ? 0 : (top == 0 && left == 0) ? 1 : (top == 0 || left == 0) ? 2 + (max(top, left) > 3)
+                     : max(top, left) <= 3 ? 4 : min(top, left) <= 3 ? 5 : 6;
+        } else {
+            const int ub = uv_bsize(bsize, I.ss);
+            *skipCtx = topAny + leftAny + 7 + ((16 * bw4(ub) * bh4(ub) > w * h) ? 3 : 0);
+        }
+        *dcCtx = sgn < 0 ? 1 : sgn > 0 ? 2 : 0;
+    }
 }
 
 // base edges of the tx block of plane p at (px, py) from the reconstruction (group g), DC value and filter type
 template <typename Px>
-__device__ void rc_edges(RcShared &S, const Img &I, int p, int px, int py, int lw, int lh, int haveL, int haveA, int haveAR, int haveBL,
+__device__ void rc_edges(lds<RcShared> &S, const Img &I, int p, int px, int py, int lw, int lh, int haveL, int haveA, int haveAR, int haveBL,
                          int mi_row, int mi_col) {
     const int ss = p ? I.ss : 0;
     const int maxX = ((I.MiCols * 4) >> ss) - 1, maxY = ((I.MiRows * 4) >> ss) - 1;
-    uint16_t *A = S.ea[p] + EDGE_OFF, *L = S.el[p] + EDGE_OFF;
+    lds<uint16_t> *A = S.ea[p] + EDGE_OFF, *L = S.el[p] + EDGE_OFF;
     grp_build_edges(rec_plane<Px>(I, p), I.pitch[p], px, py, lw, lh, haveL, haveA, haveAR, haveBL, maxX, maxY, A, L, I.bd);
     const int dcv = grp_dc_value(A, L, haveL, haveA, lw, lh, I.bd);
     if (glane() == 0) {
@@ -115,12 +136,11 @@ __device__ void rc_edges(RcShared &S, const Img &I, int p, int px, int py, int l
 
 // prediction of candidate c (255 = CfL) of plane p for a W x H tx block, by the group, into res = src - pred; the group's
 // edge buffers receive the prepared (filtered) edges of directional modes.  Returns nothing; S.gup[gid] = flags.
-template <typename Px>
-__device__ void grp_residual(RcShared &S, uint16_t *gea, uint16_t *gel, int p, int c, int lw, int lh, const Px *src, int sp,
-                             int16_t *res, int bd) {
+__device__ void grp_residual(lds<RcShared> &S, lds<uint16_t> *gea, lds<uint16_t> *gel, int p, int c, int lw, int lh,
+                             const lds<uint16_t> *src, int sp, lds<int16_t> *res, int bd) {
     const int W = 1 << lw, H = 1 << lh, lane = glane();
     const int m = c == 255 ? DC_PRED : cand_mode(c), pa = pangle(m, c == 255 ? 0 : cand_delta(c));
-    const uint16_t *EA = S.ea[p] + EDGE_OFF, *EL = S.el[p] + EDGE_OFF;
+    const lds<uint16_t> *EA = S.ea[p] + EDGE_OFF, *EL = S.el[p] + EDGE_OFF;
     int upA = 0, upL = 0;
     const bool filt = c != 255 && is_dir(m) && gea != nullptr;
     if (filt) {
@@ -144,26 +164,132 @@ __device__ void grp_residual(RcShared &S, uint16_t *gea, uint16_t *gel, int p, i
     gsync();
 }
 
-// one RD task (candidate c of plane p with tx type t) by the calling group, on scratch Scr (the group's or a big slot)
-template <int W, int H, typename Px, typename Scr>
-__device__ EvalOut grp_task(RcShared &S, Scr &X, uint16_t *gea, uint16_t *gel, const Img &I, const DevTables &T, int p, int c, int t,
-                            const Px *src, int sp, const SearchParams &P, int edgeFilter) {
-    constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
-    grp_residual<Px>(S, edgeFilter ? gea : nullptr, gel, p, c, LW, LH, src, sp, X.ev.res, I.bd);
-    EvalIn e;
+// one RD task (candidate c of plane p with tx type t) of a tx block up to 16x16 by the calling group
+template <int W, int H>
+__device__ EvalOut grp_task(lds<RcShared> &S, lds<RcGroup> &X, const Img &I, const DevTables &T, int p, int c, int t,
+                            const lds<uint16_t> *src, int sp, const SearchParams &P) {
+    constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : 4, LH = H == 4 ? 2 : H == 8 ? 3 : 4;
+    grp_residual(S, I.edgeFilter ? X.ea : nullptr, X.el, p, c, LW, LH, src, sp, X.ev.res, I.bd);
+    RcEvalIn e;
     e.txSz = tx_of(W, H); e.txType = t; e.plane = p;
     e.dcQ = dc_q(I.qindex, I.bd); e.acQ = ac_q(I.qindex, I.bd);
     e.skipCtx = S.skipCtx[p]; e.dcCtx = S.dcCtx[p]; e.intraDir = p == 0 ? cand_mode(c) : 0; e.bd = I.bd;
     e.rnd = p ? P.rndC : P.rnd; e.zbin = p ? P.zbinC : P.zbin;
-    e.resEnergy = (W == 64 || H == 64) ? grp_res_energy<W, H>(X.ev.res) : 0.f;
-    e.C = &T.cost[I.qctx];
-    return grp_eval<W, H>(X.ev, e, T);
+    e.resEnergy = 0.f;
+    e.C = &S.cost;
+    return grp_eval<W, H>(X.ev, e, S.tab);
+}
+
+// 2-D forward DCT_DCT of a W x H residual with a side of 32 or 64 by the 32 lanes of the group, the residual produced
+// on the fly by res(i, j).  Column pass: lane = column (two columns, j and j + 32, for W = 64), one fmaf chain per kept
+// output over k ascending with the matrix coefficient a wave-uniform scalar operand; row pass: lane = row.  The
+// coefficients (scaled) land in tmp (row i at tmp + i * (W + 1), tw x th kept).  Returns the residual energy sum over the
+// raster order (W or H = 64: the energy the dropped coefficients carry), else 0.
+template <int W, int H, typename Res>
+__device__ float grp_fwd_dct(lds<float> *tmp, const DevTables &T, float scale, Res res) {
+    constexpr int TW = W > 32 ? 32 : W, TH = H > 32 ? 32 : H, TS = W + 1;
+    constexpr int LW = W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
+    constexpr bool E64 = W == 64 || H == 64;
+    constexpr int NC = W == 64 ? 2 : 1;
+    const int lane = glane();
+    cst<DevTables> *TU = uni(&T);
+    cst<float> *FcT = TU->fwdT + TU->fwdOff[K_DCT][LH];
+    cst<float> *FrT = TU->fwdT + TU->fwdOff[K_DCT][LW];
+    float en = 0.f, eo = 0.f;
+    if (lane < W) {
+        float acc[NC][TH];
+#pragma unroll
+        for (int i = 0; i < TH; i++) { acc[0][i] = 0.f; if constexpr (NC == 2) acc[NC - 1][i] = 0.f; }
+#pragma unroll 2
+        for (int k = 0; k < H; k++) {
+            const int r0 = res(k, lane), r1 = NC == 2 ? res(k, lane + 32) : 0;
+            if constexpr (E64) {
+                // raster-order virtual lane l holds elements l, l + 32, ...: W = 64 alternates columns l and l + 32 per
+                // row; W = 32 is column l; W = 16 (16x64) splits column l & 15 into even (lane l) and odd rows (l + 16)
+                if constexpr (W == 64) { en = en + (float)(r0 * r0); en = en + (float)(r1 * r1); }
+                else if constexpr (W == 32) en = en + (float)(r0 * r0);
+                else { if (k & 1) eo = eo + (float)(r0 * r0); else en = en + (float)(r0 * r0); }
+            }
+            const float x0 = (float)r0, x1 = (float)r1;
+            cst<float> *f = FcT + k * H;
+#pragma unroll
+            for (int i = 0; i < TH; i++) {
+                const float fi = f[i];
+                acc[0][i] = fmaf(fi, x0, acc[0][i]);
+                if constexpr (NC == 2) acc[NC - 1][i] = fmaf(fi, x1, acc[NC - 1][i]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < TH; i++) {
+            tmp[i * TS + lane] = acc[0][i];
+            if constexpr (NC == 2) tmp[i * TS + lane + 32] = acc[NC - 1][i];
+        }
+    }
+    if constexpr (E64 && W == 16) {
+        const float o = __shfl_xor(eo, 16, 32);
+        if (lane >= 16) en = o;
+    }
+    gsync();
+    if (lane < TH) {
+        float acc[TW];
+#pragma unroll
+        for (int o = 0; o < TW; o++) acc[o] = 0.f;
+        lds<float> *row = tmp + lane * TS;
+#pragma unroll 2
+        for (int k = 0; k < W; k++) {
+            const float x = row[k];
+            cst<float> *f = FrT + k * W;
+#pragma unroll
+            for (int o = 0; o < TW; o++) acc[o] = fmaf(f[o], x, acc[o]);
+        }
+#pragma unroll
+        for (int o = 0; o < TW; o++) row[o] = acc[o] * scale;
+    }
+    gsync();
+    return E64 ? gsum(en) : 0.f;
+}
+
+// one RD task (candidate c, 255 = CfL, of plane p; DCT_DCT) of a tx block with a side of 32 or 64 by the calling group
+template <int W, int H>
+__device__ EvalOut grp_task_big(lds<RcShared> &S, lds<RcTeam> &X, const Img &I, const DevTables &T, int p, int c,
+                                const lds<uint16_t> *src, int sp, const SearchParams &P) {
+    constexpr int LW = W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
+    const int lane = glane(), bd = I.bd;
+    const int m = c == 255 ? DC_PRED : cand_mode(c), pa = pangle(m, c == 255 ? 0 : cand_delta(c));
+    const lds<uint16_t> *EA = S.ea[p] + EDGE_OFF, *EL = S.el[p] + EDGE_OFF;
+    int upA = 0, upL = 0;
+    if (c != 255 && is_dir(m) && I.edgeFilter) {
+        for (int i = lane; i < EDGE_LEN; i += GW) { X.ea[i] = S.ea[p][i]; X.el[i] = S.el[p][i]; }
+        gsync();
+        grp_edge_prepare(pa, LW, LH, S.ft[p], S.hvL[p], S.hvA[p], S.px[p], S.py[p], S.mx[p], S.my[p], X.ea + EDGE_OFF, X.el + EDGE_OFF, bd,
+                         upA, upL);
+        EA = X.ea + EDGE_OFF;
+        EL = X.el + EDGE_OFF;
+    }
+    if (lane == 0) { S.gup[gid()][0] = upA; S.gup[gid()][1] = upL; }
+    const int dcv = S.dcv[p], alpha = c == 255 ? S.cflAlpha[p - 1] : 0, cavg = S.cflAvg;
+    auto res = [&](int i, int j) -> int {
+        int pr;
+        if (c == 255) pr = clip3(0, (1 << bd) - 1, dcv + round2signed(alpha * (S.cflL[i * W + j] - cavg), 6));
+        else if (m == DC_PRED) pr = dcv;
+        else pr = pred_px(m, pa, i, j, LW, LH, EA, EL, upA, upL);
+        return (int)src[i * sp + j] - pr;
+    };
+    const int txSz = tx_of(W, H);
+    RcEvalIn e;
+    e.resEnergy = grp_fwd_dct<W, H>(X.tmp, T, S.tab.txScale[txSz], res);
+    e.txSz = txSz; e.txType = DCT_DCT; e.plane = p;
+    e.dcQ = dc_q(I.qindex, bd); e.acQ = ac_q(I.qindex, bd);
+    e.skipCtx = S.skipCtx[p]; e.dcCtx = S.dcCtx[p]; e.intraDir = p == 0 ? cand_mode(c) : 0; e.bd = bd;
+    e.rnd = p ? P.rndC : P.rnd; e.zbin = p ? P.zbinC : P.zbin;
+    e.C = &S.cost;
+    return grp_qrate<W, H>(X.tmp, W + 1, X.q, e, S.tab);
 }
 
 // ---- exact inverse transform + reconstruction by the workgroup (spec 7.13.3, 7.12.3 step 3) ----
 template <int LW>
-__device__ uint32_t inv_row(int i, int W, int th, int tw, const int16_t *q, int dcQ, int acQ, int dsh, int bd, bool rect, int rk,
-                            int rowShift, int32_t *res) {
+__device__ uint32_t inv_row(int i, int W, int th, int tw, const lds<int16_t> *q, int dcQ, int acQ, int dsh, int bd, bool rect, int rk,
+                            int rowShift, lds<int32_t> *res) {
     constexpr int N = 1 << LW;
     int32_t Tv[N];
 #pragma unroll
@@ -180,7 +306,7 @@ __device__ uint32_t inv_row(int i, int W, int th, int tw, const int16_t *q, int 
     return err;
 }
 template <int LH>
-__device__ uint32_t inv_col(int j, int W, int ck, int bd, int32_t *res) {
+__device__ uint32_t inv_col(int j, int W, int ck, int bd, lds<int32_t> *res) {
     constexpr int N = 1 << LH;
     int32_t Tv[N];
 #pragma unroll
@@ -194,7 +320,7 @@ __device__ uint32_t inv_col(int j, int W, int ck, int bd, int32_t *res) {
 
 // dequantize + inverse + add the levels q (LDS, modified by the repair) onto dst; returns the number of repairs
 template <typename Px>
-__device__ int wg_inverse_add(RcShared &S, int16_t *q, int txSz, int txType, int bd, int dcQ, int acQ, Px *dst, int pitch) {
+__device__ int wg_inverse_add(lds<RcShared> &S, lds<int16_t> *q, int txSz, int txType, int bd, int dcQ, int acQ, Px *dst, int pitch) {
     const int tid = threadIdx.x, lw = av1t::Tx_Width_Log2[txSz], lh = av1t::Tx_Height_Log2[txSz], W = 1 << lw, H = 1 << lh;
     const int tw = W > 32 ? 32 : W, th = H > 32 ? 32 : H, n = tw * th, den = dq_denom(txSz), dsh = den == 4 ? 2 : den == 2 ? 1 : 0;
     const bool rect = lw - lh == 1 || lh - lw == 1;
@@ -213,7 +339,7 @@ __device__ int wg_inverse_add(RcShared &S, int16_t *q, int txSz, int txType, int
             default: err = inv_row<6>(tid, W, th, tw, q, dcQ, acQ, dsh, bd, rect, rk, rs, S.inv); break;
             }
         }
-        if (err) atomicOr(&S.errFlag, (int)err);
+        if (err) atomicOr(gen(&S.errFlag), (int)err);
         __syncthreads();
         err = 0;
         if (tid < W) {
@@ -225,7 +351,7 @@ __device__ int wg_inverse_add(RcShared &S, int16_t *q, int txSz, int txType, int
             default: err = inv_col<6>(tid, W, ck, bd, S.inv); break;
             }
         }
-        if (err) atomicOr(&S.errFlag, (int)err);
+        if (err) atomicOr(gen(&S.errFlag), (int)err);
         __syncthreads();
         if (!S.errFlag) break;
         fixes++;
@@ -248,12 +374,12 @@ __device__ int wg_inverse_add(RcShared &S, int16_t *q, int txSz, int txType, int
 
 // write the prediction of candidate c of plane p (edges prepared by group g, or the base edges) into dst (workgroup)
 template <typename Px>
-__device__ void wg_write_pred(const RcShared &S, const uint16_t *gea, const uint16_t *gel, int g, int p, int c, int lw, int lh, Px *dst,
+__device__ void wg_write_pred(const lds<RcShared> &S, const lds<uint16_t> *gea, const lds<uint16_t> *gel, int g, int p, int c, int lw, int lh, Px *dst,
                               int pitch, int bd) {
     const int W = 1 << lw, H = 1 << lh;
     const int m = c == 255 ? DC_PRED : cand_mode(c), pa = pangle(m, c == 255 ? 0 : cand_delta(c));
     const bool filt = c != 255 && is_dir(m) && gea != nullptr;
-    const uint16_t *EA = filt ? gea + EDGE_OFF : S.ea[p] + EDGE_OFF, *EL = filt ? gel + EDGE_OFF : S.el[p] + EDGE_OFF;
+    const lds<uint16_t> *EA = filt ? gea + EDGE_OFF : S.ea[p] + EDGE_OFF, *EL = filt ? gel + EDGE_OFF : S.el[p] + EDGE_OFF;
     const int upA = filt ? S.gup[g][0] : 0, upL = filt ? S.gup[g][1] : 0, dcv = S.dcv[p];
     for (int q = threadIdx.x; q < W * H; q += RC_NT) {
         const int i = q >> lw, j = q & (W - 1);
@@ -267,8 +393,8 @@ __device__ void wg_write_pred(const RcShared &S, const uint16_t *gea, const uint
 }
 
 // record the finished tx block with levels q: coefficients, txb entry, context maps, BlockDecoded (workgroup)
-__device__ int wg_record_txb(RcShared &S, const Batch &B, uint32_t sb, const Img &I, int p, int txSz, int txType, int x4, int y4,
-                             const int16_t *q) {
+__device__ int wg_record_txb(lds<RcShared> &S, const Batch &B, uint32_t sb, const Img &I, int p, int txSz, int txType, int x4, int y4,
+                             const lds<int16_t> *q) {
     const int tid = threadIdx.x, n = tx_ncoef(txSz);
     const uint16_t *scan = scan_of(txSz, txType);
     const uint32_t off = (uint32_t)S.nCoef;
@@ -308,70 +434,83 @@ __device__ int wg_record_txb(RcShared &S, const Batch &B, uint32_t sb, const Img
     return eob;
 }
 
-// run S.taskC/T/P[0 .. nTask) on the groups (tx > 16x16: the RC_NBIG big slots); per task J = fmaf(lambda, R + extra, D)
-// with extra = luma mode bits (plane 0) or 0; S.taskJ / taskE.  Then S.win = the first minimum over the tasks.
-template <int W, int H, typename Px>
-__device__ void wg_run_tasks(RcShared &S, const Img &I, const DevTables &T, const SearchParams &P, const Px *const *srcs, int sp, float lambda,
-                             bool lumaBits, bool pickWinner) {
+// source of plane p in the staging buffer
+__device__ __forceinline__ const lds<uint16_t> *stage_src(const lds<RcShared> &S, int p) { return S.srcb + (p == 2 ? 2048 : 0); }
+
+// run S.taskC/T/P[0 .. nTask) on the 8 groups (the source block staged in S.srcb, row stride W); per task
+// J = fmaf(lambda, R + extra, D) with extra = luma mode bits (plane 0) or 0; S.taskJ / taskE.  Then S.win = the first
+// minimum over the tasks.
+template <int W, int H>
+__device__ void wg_run_tasks(lds<RcShared> &S, const Img &I, const DevTables &T, const SearchParams &P, float lambda, bool lumaBits,
+                             bool pickWinner) {
     constexpr bool big = W > 16 || H > 16;
     const int g = gid(), lane = glane();
-    const CostTab &C = T.cost[I.qctx];
-    const int ng = big ? RC_NBIG : RC_NG;
-    if (g < ng) {
-        for (int t = g; t < S.nTask; t += ng) {
-            const int c = S.taskC[t], tt = S.taskT[t], p = S.taskP[t];
-            EvalOut o;
-            if constexpr (big) o = grp_task<W, H, Px>(S, S.big[g], S.big[g].ea, S.big[g].el, I, T, p, c, tt, srcs[p], sp, P, I.edgeFilter);
-            else o = grp_task<W, H, Px>(S, S.g[g], S.g[g].ea, S.g[g].el, I, T, p, c, tt, srcs[p], sp, P, I.edgeFilter);
-            float J;
-            if (lumaBits) {
-                const int m = cand_mode(c);
-                float mb = C.y_mode[S.am][S.lm][m];
-                if (S.bsize >= BLOCK_8X8 && is_dir(m)) mb += C.angle_delta[m - V_PRED][cand_delta(c) + 3];
-                J = fmaf(lambda, o.R + mb, o.D);
-            } else {
-                J = fmaf(lambda, o.R, o.D);
-            }
-            if (lane == 0) { S.taskJ[t] = J; S.taskE[t] = o.eob; }
-            gsync();
+    const lds<CostTab> &C = S.cost;
+    for (int t = g; t < S.nTask; t += RC_NG) {
+        const int c = S.taskC[t], tt = S.taskT[t], p = S.taskP[t];
+        EvalOut o;
+        if constexpr (big) o = grp_task_big<W, H>(S, S.u.t[g], I, T, p, c, stage_src(S, p), W, P);
+        else o = grp_task<W, H>(S, S.u.g[g], I, T, p, c, tt, stage_src(S, p), W, P);
+        float J;
+        if (lumaBits) {
+            const int m = cand_mode(c);
+            float mb = C.y_mode[S.am][S.lm][m];
+            if (S.bsize >= BLOCK_8X8 && is_dir(m)) mb += C.angle_delta[m - V_PRED][cand_delta(c) + 3];
+            J = fmaf(lambda, o.R + mb, o.D);
+        } else {
+            J = fmaf(lambda, o.R, o.D);
         }
+        if (lane == 0) { S.taskJ[t] = J; S.taskE[t] = o.eob; }
+        gsync();
     }
     __syncthreads();
-    if (pickWinner && threadIdx.x == 0) {
-        int w = 0;
-        for (int t = 1; t < S.nTask; t++) if (S.taskJ[t] < S.taskJ[w]) w = t;
-        S.win = w;
+    if (pickWinner && threadIdx.x < GW) {   // the first minimum over the tasks (at most 64)
+        const int n = S.nTask;
+        float v = lane < n ? S.taskJ[lane] : INF;
+        int i = lane;
+        if (lane + GW < n && S.taskJ[lane + GW] < v) { v = S.taskJ[lane + GW]; i = lane + GW; }
+        const int w = gargmin(v, i);
+        if (lane == 0) S.win = w < 0 ? 0 : w;
     }
     __syncthreads();
 }
 
-// re-evaluate task t on group g (big slot for large tx) so its levels and prepared edges are in that group's scratch;
-// returns a pointer to the levels
-template <int W, int H, typename Px>
-__device__ int16_t *wg_final_task(RcShared &S, const Img &I, const DevTables &T, const SearchParams &P, const Px *src, int sp, int t, int g) {
+// re-evaluate task t on group 0 so its levels and prepared edges are in that group's scratch; returns the levels
+template <int W, int H>
+__device__ lds<int16_t> *wg_final_task(lds<RcShared> &S, const Img &I, const DevTables &T, const SearchParams &P, int t) {
     constexpr bool big = W > 16 || H > 16;
-    if (gid() == g) {
-        if constexpr (big) grp_task<W, H, Px>(S, S.big[g], S.big[g].ea, S.big[g].el, I, T, S.taskP[t], S.taskC[t], S.taskT[t], src, sp, P, I.edgeFilter);
-        else grp_task<W, H, Px>(S, S.g[g], S.g[g].ea, S.g[g].el, I, T, S.taskP[t], S.taskC[t], S.taskT[t], src, sp, P, I.edgeFilter);
+    const int p = S.taskP[t];
+    if (gid() == 0) {
+        if constexpr (big) grp_task_big<W, H>(S, S.u.t[0], I, T, p, S.taskC[t], stage_src(S, p), W, P);
+        else grp_task<W, H>(S, S.u.g[0], I, T, p, S.taskC[t], S.taskT[t], stage_src(S, p), W, P);
     }
     __syncthreads();
-    if constexpr (big) return S.big[g].ev.q;
-    else return S.g[g].ev.q;
+    if constexpr (big) return S.u.t[0].q;
+    else return S.u.g[0].ev.q;
 }
 
+// the edge buffers group 0's final evaluation prepared (directional modes with the edge filter on)
+template <int W, int H>
+__device__ __forceinline__ const lds<uint16_t> *final_ea(const lds<RcShared> &S) { return (W > 16 || H > 16) ? S.u.t[0].ea : S.u.g[0].ea; }
+template <int W, int H>
+__device__ __forceinline__ const lds<uint16_t> *final_el(const lds<RcShared> &S) { return (W > 16 || H > 16) ? S.u.t[0].el : S.u.g[0].el; }
+
 template <int W, int H, typename Px>
-__device__ void rc_luma(RcShared &S, const Batch &B, const SearchParams &P, uint32_t sb, const Img &I, int bsize, int x, int y, int availU,
+__device__ void rc_luma(lds<RcShared> &S, const Batch &B, const SearchParams &P, uint32_t sb, const Img &I, int bsize, int x, int y, int availU,
                         int availL, int tx0, int ty0, float lambda) {
     constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
-    constexpr bool big = W > 16 || H > 16;
     const DevTables &T = *B.T;
     const int tid = threadIdx.x, txSz = tx_of(W, H);
     const int sbr4 = (y >> 2) & 15, sbc4 = (x >> 2) & 15, mi_row = y >> 2, mi_col = x >> 2;
+    {
+        const Px *src = src_plane<Px>(I, 0) + (size_t)y * I.spitch[0] + x;
+        for (int q = tid; q < W * H; q += RC_NT) S.srcb[q] = src[(size_t)(q >> LW) * I.spitch[0] + (q & (W - 1))];
+    }
     if (gid() == 0)
         rc_edges<Px>(S, I, 0, x, y, LW, LH, availL, availU, S.bdm[0][sbr4][sbc4 + (W >> 2) + 1], S.bdm[0][sbr4 + (H >> 2) + 1][sbc4],
                      mi_row, mi_col);
+    if (gid() == 2) grp_txb_ctx(I, 0, txSz, mi_col, mi_row, bsize, tx0, ty0, &S.skipCtx[0], &S.dcCtx[0]);
     if (tid == 64) {
-        txb_ctx(I, 0, txSz, mi_col, mi_row, bsize, tx0, ty0, &S.skipCtx[0], &S.dcCtx[0]);
         S.am = av1t::Intra_Mode_Context[availU ? I.mY[(size_t)(mi_row - 1) * I.MiCols + mi_col] : DC_PRED];
         S.lm = av1t::Intra_Mode_Context[availL ? I.mY[(size_t)mi_row * I.MiCols + mi_col - 1] : DC_PRED];
         const BlockRes &br = B.blk[I.shOff[bsize] + (uint32_t)(y / (4 * bh4(bsize))) * I.shNbx[bsize] + x / (4 * bw4(bsize))];
@@ -390,9 +529,7 @@ __device__ void rc_luma(RcShared &S, const Batch &B, const SearchParams &P, uint
         S.bsize = bsize;
     }
     __syncthreads();
-    const Px *src = src_plane<Px>(I, 0) + (size_t)y * I.spitch[0] + x;
-    const Px *srcs[3] = {src, src, src};
-    wg_run_tasks<W, H, Px>(S, I, T, P, srcs, I.spitch[0], lambda, true, true);
+    wg_run_tasks<W, H>(S, I, T, P, lambda, true, true);
     if (!(P.txAllCL && tx_set_intra(txSz) != TX_SET_DCTONLY) && P.txSearchCL && tx_set_intra(txSz) != TX_SET_DCTONLY) {
         // tx-type pass for the pass-1 winner (txAllCL off): the other types of the set
         if (tid == 0) {
@@ -404,14 +541,12 @@ __device__ void rc_luma(RcShared &S, const Batch &B, const SearchParams &P, uint
             S.nTask = n;
         }
         __syncthreads();
-        wg_run_tasks<W, H, Px>(S, I, T, P, srcs, I.spitch[0], lambda, true, true);
+        wg_run_tasks<W, H>(S, I, T, P, lambda, true, true);
     }
     const int win = S.win;
-    int16_t *q = wg_final_task<W, H, Px>(S, I, T, P, src, I.spitch[0], win, 0);
+    lds<int16_t> *q = wg_final_task<W, H>(S, I, T, P, win);
     Px *dst = rec_plane<Px>(I, 0) + (size_t)y * I.pitch[0] + x;
-    const uint16_t *gea = I.edgeFilter ? (big ? S.big[0].ea : S.g[0].ea) : nullptr;
-    const uint16_t *gel = big ? S.big[0].el : S.g[0].el;
-    wg_write_pred<Px>(S, gea, gel, 0, 0, S.taskC[win], LW, LH, dst, I.pitch[0], I.bd);
+    wg_write_pred<Px>(S, I.edgeFilter ? final_ea<W, H>(S) : nullptr, final_el<W, H>(S), 0, 0, S.taskC[win], LW, LH, dst, I.pitch[0], I.bd);
     if (S.taskE[win] > 0) {
         const int fixes = wg_inverse_add<Px>(S, q, txSz, S.taskT[win], I.bd, dc_q(I.qindex, I.bd), ac_q(I.qindex, I.bd), dst, I.pitch[0]);
         if (fixes && tid == 0) atomicAdd(&B.stats[3], 1u);
@@ -422,12 +557,11 @@ __device__ void rc_luma(RcShared &S, const Batch &B, const SearchParams &P, uint
 }
 
 template <int W, int H, typename Px>
-__device__ void rc_chroma(RcShared &S, const Batch &B, const SearchParams &P, uint32_t sb, const Img &I, int bsize, int mi_row, int mi_col,
+__device__ void rc_chroma(lds<RcShared> &S, const Batch &B, const SearchParams &P, uint32_t sb, const Img &I, int bsize, int mi_row, int mi_col,
                           int availUC, int availLC, int tx0, int ty0, float lambda) {
     constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
-    constexpr bool big = W > 16 || H > 16;
     const DevTables &T = *B.T;
-    const CostTab &C = T.cost[I.qctx];
+    const lds<CostTab> &C = S.cost;
     const int tid = threadIdx.x, ss = I.ss, bd = I.bd, txSz = tx_of(W, H);
     const int cx = (mi_col >> ss) * 4, cy = (mi_row >> ss) * 4, x4 = cx >> 2, y4 = cy >> 2;
     const int sbr4 = (mi_row & 15) >> ss, sbc4 = (mi_col & 15) >> ss;
@@ -436,15 +570,20 @@ __device__ void rc_chroma(RcShared &S, const Batch &B, const SearchParams &P, ui
         rc_edges<Px>(S, I, p, cx, cy, LW, LH, availLC, availUC, S.bdm[p][sbr4][sbc4 + (W >> 2) + 1], S.bdm[p][sbr4 + (H >> 2) + 1][sbc4],
                      mi_row, mi_col);
     }
-    if (tid == 64) {
-        txb_ctx(I, 1, txSz, x4, y4, bsize, tx0, ty0, &S.skipCtx[1], &S.dcCtx[1]);
-        txb_ctx(I, 2, txSz, x4, y4, bsize, tx0, ty0, &S.skipCtx[2], &S.dcCtx[2]);
+    if (gid() == 2) grp_txb_ctx(I, 1, txSz, x4, y4, bsize, tx0, ty0, &S.skipCtx[1], &S.dcCtx[1]);
+    if (gid() == 3) grp_txb_ctx(I, 2, txSz, x4, y4, bsize, tx0, ty0, &S.skipCtx[2], &S.dcCtx[2]);
+    {
+        const Px *gU = src_plane<Px>(I, 1) + (size_t)cy * I.spitch[1] + cx, *gV = src_plane<Px>(I, 2) + (size_t)cy * I.spitch[2] + cx;
+        for (int q = tid; q < W * H; q += RC_NT) {
+            S.srcb[q] = gU[(size_t)(q >> LW) * I.spitch[1] + (q & (W - 1))];
+            S.srcb[2048 + q] = gV[(size_t)(q >> LW) * I.spitch[2] + (q & (W - 1))];
+        }
     }
     if (tid < 32) S.satdI[tid] = 0;
     if (tid < 8) S.acc64[tid] = 0;
     __syncthreads();
-    const int sp = I.spitch[1];
-    const Px *srcU = src_plane<Px>(I, 1) + (size_t)cy * sp + cx, *srcV = src_plane<Px>(I, 2) + (size_t)cy * sp + cx;
+    constexpr int sp = W;
+    const lds<uint16_t> *srcU = S.srcb, *srcV = S.srcb + 2048;
     const bool cflOk = P.cfl && cfl_allowed(bsize);
     const float lamS = P.satdLambda * T.lamS[bd == 10][I.qindex];   // the luma lambda's screening weight
     const int yMode = S.mode;
@@ -478,7 +617,7 @@ __device__ void rc_chroma(RcShared &S, const Batch &B, const SearchParams &P, ui
             sum += v;
         }
         sum = gsumi(sum);
-        if (glane() == 0) atomicAdd(&S.satdI[31], sum);
+        if (glane() == 0) atomicAdd(gen(&S.satdI[31]), sum);
         __syncthreads();
         if (tid == 0) S.cflAvg = round2(S.satdI[31], LW + LH);
         __syncthreads();
@@ -490,9 +629,9 @@ __device__ void rc_chroma(RcShared &S, const Batch &B, const SearchParams &P, ui
             nV += l * ((int)srcV[i * sp + j] - S.dcv[2]);
             dd += l * l;
         }
-        atomicAdd(&S.acc64[0], (unsigned long long)nU);
-        atomicAdd(&S.acc64[1], (unsigned long long)nV);
-        atomicAdd(&S.acc64[2], (unsigned long long)dd);
+        atomicAdd(gen(&S.acc64[0]), (unsigned long long)nU);
+        atomicAdd(gen(&S.acc64[1]), (unsigned long long)nV);
+        atomicAdd(gen(&S.acc64[2]), (unsigned long long)dd);
         __syncthreads();
         if (tid == 0) {
             const long long d = (long long)S.acc64[2];
@@ -516,7 +655,7 @@ __device__ void rc_chroma(RcShared &S, const Batch &B, const SearchParams &P, ui
                 }
             }
         }
-        for (int k = 0; k < 6; k++) atomicAdd(&S.acc64[2 + k], (unsigned long long)sse[k]);
+        for (int k = 0; k < 6; k++) atomicAdd(gen(&S.acc64[2 + k]), (unsigned long long)sse[k]);
         __syncthreads();
         if (tid == 0)
             for (int pl = 0; pl < 2; pl++) {
@@ -538,8 +677,8 @@ __device__ void rc_chroma(RcShared &S, const Batch &B, const SearchParams &P, ui
         for (int it = tid; it < items; it += RC_NT) {
             const int ci = c0 + it / (2 * NSUB), rr = it % (2 * NSUB), p = 1 + rr / NSUB, sbk = rr % NSUB;
             const int sx = (sbk % NSX) * 4, sy = (sbk / NSX) * 4, c = S.uvc[ci], m = cand_mode(c), pa = pangle(m, cand_delta(c));
-            const Px *s = p == 1 ? srcU : srcV;
-            const uint16_t *A = S.ea[p] + EDGE_OFF, *L = S.el[p] + EDGE_OFF;
+            const lds<uint16_t> *s = p == 1 ? srcU : srcV;
+            const lds<uint16_t> *A = S.ea[p] + EDGE_OFF, *L = S.el[p] + EDGE_OFF;
             int d[16];
 #pragma unroll
             for (int i = 0; i < 4; i++)
@@ -548,52 +687,47 @@ __device__ void rc_chroma(RcShared &S, const Batch &B, const SearchParams &P, ui
                     const int pr = m == DC_PRED ? S.dcv[p] : pred_px(m, pa, sy + i, sx + j, LW, LH, A, L, 0, 0);
                     d[i * 4 + j] = (int)s[(sy + i) * sp + sx + j] - pr;
                 }
-            atomicAdd(&S.satdI[ci], hadamard4(d));
+            atomicAdd(gen(&S.satdI[ci]), hadamard4(d));
         }
         __syncthreads();
     };
     screen(0, 13);
-    if (tid == 0) {
-        S.nuv = 13;
-        if (P.uvDeltas && bsize >= BLOCK_8X8) {
-            int bm = -1;
-            float bj = INF;
-            for (int m = V_PRED; m <= D67_PRED; m++) {
-                const float j = fmaf(lamS, uvcbits(cand_pack(m, 0)), (float)S.satdI[m] * 0.25f);
-                if (bm < 0 || j < bj) { bj = j; bm = m; }
-            }
-            for (int k = 0; k < 6; k++) S.uvc[13 + k] = cand_pack(bm, k < 3 ? -(k + 1) : k - 2);
-            S.nuv = 19;
+    if (tid < GW) {   // the deltas around the best directional mode (first minimum of the screening cost)
+        const bool dl = P.uvDeltas && bsize >= BLOCK_8X8;
+        if (dl) {
+            const float j = tid < 8 ? fmaf(lamS, uvcbits(cand_pack(V_PRED + tid, 0)), (float)S.satdI[V_PRED + tid] * 0.25f) : INF;
+            const int bm = V_PRED + gargmin(j, tid);
+            if (tid < 6) S.uvc[13 + tid] = cand_pack(bm, tid < 3 ? -(tid + 1) : tid - 2);
         }
+        if (tid == 0) S.nuv = dl ? 19 : 13;
     }
     __syncthreads();
     if (S.nuv > 13) screen(13, 19);
-    if (tid == 0) {
-        float js[19];
-        bool used[19];
-        for (int i = 0; i < S.nuv; i++) { js[i] = fmaf(lamS, uvcbits(S.uvc[i]), (float)S.satdI[i] * 0.25f); used[i] = false; }
-        const int K = max(1, min(P.uvK, 6));
+    if (tid < GW) {   // the uvK candidates of smallest screening cost (first minimum each round), U and V tasks each
+        const int l = tid, nuv = S.nuv, K = max(1, min(P.uvK, 6));
+        float js = l < nuv ? fmaf(lamS, uvcbits(S.uvc[l]), (float)S.satdI[l] * 0.25f) : INF;
         int n = 0;
         for (int k = 0; k < K; k++) {
-            int b = -1;
-            for (int i = 0; i < S.nuv; i++) if (!used[i] && (b < 0 || js[i] < js[b])) b = i;
+            const int b = gargmin(js, l);
             if (b < 0) break;
-            used[b] = true;
-            for (int p = 1; p <= 2; p++) { S.taskC[n] = S.uvc[b]; S.taskT[n] = chroma_tx_type(txSz, cand_mode(S.uvc[b])); S.taskP[n] = p; n++; }
+            if (l == b) js = INF;
+            if (l < 2) { S.taskC[n + l] = S.uvc[b]; S.taskT[n + l] = chroma_tx_type(txSz, cand_mode(S.uvc[b])); S.taskP[n + l] = 1 + l; }
+            n += 2;
         }
-        if (cflOk && (S.cflAlpha[0] || S.cflAlpha[1]))
-            for (int p = 1; p <= 2; p++) { S.taskC[n] = 255; S.taskT[n] = chroma_tx_type(txSz, UV_CFL_PRED); S.taskP[n] = p; n++; }
-        S.nTask = n;
+        if (cflOk && (S.cflAlpha[0] || S.cflAlpha[1])) {
+            if (l < 2) { S.taskC[n + l] = 255; S.taskT[n + l] = chroma_tx_type(txSz, UV_CFL_PRED); S.taskP[n + l] = 1 + l; }
+            n += 2;
+        }
+        if (l == 0) S.nTask = n;
     }
     __syncthreads();
-    const Px *srcs[3] = {srcU, srcU, srcV};
-    wg_run_tasks<W, H, Px>(S, I, T, P, srcs, sp, lambda, false, false);
-    if (tid == 0) {
-        float bJ = INF;
-        int bt = 0;
-        for (int t = 0; t + 1 < S.nTask; t += 2) {
-            const int c = S.taskC[t];
-            float J = S.taskJ[t] + S.taskJ[t + 1];
+    wg_run_tasks<W, H>(S, I, T, P, lambda, false, false);
+    if (tid < GW) {   // pick: the first minimum over the (U, V) task pairs of J_U + J_V + lambda * mode bits
+        const int l = tid;
+        float J = INF;
+        if (2 * l + 1 < S.nTask) {
+            const int t = 2 * l, c = S.taskC[t];
+            J = S.taskJ[t] + S.taskJ[t + 1];
             if (c == 255) {
                 const int aU = S.cflAlpha[0], aV = S.cflAlpha[1];
                 const int sU = aU == 0 ? 0 : aU < 0 ? 1 : 2, sV = aV == 0 ? 0 : aV < 0 ? 1 : 2;
@@ -604,22 +738,22 @@ __device__ void rc_chroma(RcShared &S, const Batch &B, const SearchParams &P, ui
             } else {
                 J = fmaf(lambda, uvcbits(c), J);
             }
-            if (J < bJ) { bJ = J; bt = t; }
         }
-        S.win = bt;
-        const int c = S.taskC[bt];
-        S.uvMode = c == 255 ? UV_CFL_PRED : cand_mode(c);
-        S.uvDelta = c == 255 ? 0 : cand_delta(c);
+        const int w = gargmin(J, l), bt = 2 * (w < 0 ? 0 : w);
+        if (l == 0) {
+            S.win = bt;
+            const int c = S.taskC[bt];
+            S.uvMode = c == 255 ? UV_CFL_PRED : cand_mode(c);
+            S.uvDelta = c == 255 ? 0 : cand_delta(c);
+        }
     }
     __syncthreads();
     const int win = S.win, uvTxType = S.taskT[win];
     for (int p = 1; p <= 2; p++) {
         const int t = win + p - 1;
-        int16_t *q = wg_final_task<W, H, Px>(S, I, T, P, p == 1 ? srcU : srcV, sp, t, 0);
+        lds<int16_t> *q = wg_final_task<W, H>(S, I, T, P, t);
         Px *dst = rec_plane<Px>(I, p) + (size_t)cy * I.pitch[p] + cx;
-        const uint16_t *gea = I.edgeFilter ? (big ? S.big[0].ea : S.g[0].ea) : nullptr;
-        const uint16_t *gel = big ? S.big[0].el : S.g[0].el;
-        wg_write_pred<Px>(S, gea, gel, 0, p, S.taskC[t], LW, LH, dst, I.pitch[p], bd);
+        wg_write_pred<Px>(S, I.edgeFilter ? final_ea<W, H>(S) : nullptr, final_el<W, H>(S), 0, p, S.taskC[t], LW, LH, dst, I.pitch[p], bd);
         if (S.taskE[t] > 0) {
             const int fixes = wg_inverse_add<Px>(S, q, txSz, uvTxType, bd, dc_q(I.qindex, bd), ac_q(I.qindex, bd), dst, I.pitch[p]);
             if (fixes && tid == 0) atomicAdd(&B.stats[3], 1u);
@@ -633,7 +767,7 @@ __device__ void rc_chroma(RcShared &S, const Batch &B, const SearchParams &P, ui
 template <typename Px>
 __global__ void __launch_bounds__(RC_NT) k_recon(Batch B, SearchParams P, const uint32_t *sbList) {
     extern __shared__ __align__(16) char smraw[];
-    RcShared &S = *reinterpret_cast<RcShared *>(smraw);
+    lds<RcShared> &S = *(lds<RcShared> *)(lds<char> *)smraw;
     const uint32_t sb = sbList[blockIdx.x];
     const Img &I = B.img[B.sbImg[sb]];
     const DevTables &T = *B.T;
@@ -655,6 +789,20 @@ __global__ void __launch_bounds__(RC_NT) k_recon(Batch B, SearchParams P, const 
         S.bdm[p][yy + 1][xx + 1] = v;
     }
     if (tid == 0) { S.nTxb = 0; S.nCoef = 0; }
+    // stage the cost tables of the quantizer context and the small forward matrices in LDS
+    {
+        const float *gc = (const float *)&T.cost[I.qctx];
+        lds<float> *lc = (lds<float> *)&S.cost;
+        for (int k = tid; k < (int)(sizeof(CostTab) / 4); k += RC_NT) lc[k] = gc[k];
+        for (int k = tid; k < 3 * 336; k += RC_NT) {
+            const int kind = k / 336, r = k % 336, ln = r < 16 ? 2 : r < 80 ? 3 : 4, e = r - (ln == 2 ? 0 : ln == 3 ? 16 : 80);
+            S.tab.fwdT[k] = T.fwdT[T.fwdOff[kind][ln] + e];
+            S.tab.fwd[k] = T.fwd[T.fwdOff[kind][ln] + e];
+        }
+        if (tid < 21) S.tab.fwdOff[tid / 7][tid % 7] = (tid / 7) * 336 + ((tid % 7) == 3 ? 16 : (tid % 7) == 4 ? 80 : 0);
+        if (tid < 19) S.tab.txScale[tid] = T.txScale[tid];
+        for (int k = tid; k < 19 * 16; k += RC_NT) S.tab.txg[k / 16][k % 16] = T.txg[k / 16][k % 16];
+    }
     __syncthreads();
     const float lambda = T.lambda[I.bd == 10][I.qindex] * P.lambdaMul;
     const int nLeaves = B.leafCount[sb];
