@@ -33,7 +33,7 @@ class _Params(C.Structure):
 
 
 class _Config(C.Structure):
-    _fields_ = [("qindex", C.c_int), ("edgeFilter", C.c_int), ("P", _Params)]
+    _fields_ = [("qindex", C.c_int), ("edgeFilter", C.c_int), ("P", _Params), ("schedule", C.c_int), ("queueBlocks", C.c_int)]
 
 
 class _FrameIn(C.Structure):
@@ -52,7 +52,7 @@ class _FrameOut(C.Structure):
 class _Times(C.Structure):
     _fields_ = [(n, C.c_float) for n in ("setup", "analysis", "dp", "recon", "pack", "download", "total")] + \
                [(n, C.c_uint32) for n in ("steps", "sbs", "blocks", "skipBlocks", "cflBlocks", "rangeFixes")] + \
-               [("deviceBytes", C.c_size_t)]
+               [("deviceBytes", C.c_size_t), ("schedule", C.c_uint32), ("queueBlocks", C.c_uint32)]
 
 
 _lib = None
@@ -124,6 +124,8 @@ class GpConfig:
     qindex: int = 120
     edgeFilter: int = 1
     P: GpParams = field(default_factory=GpParams)
+    schedule: int = 0      # closed loop: 0 persistent queue, 1 one launch per wavefront step (same results)
+    queueBlocks: int = 0   # persistent workgroups, 0 = every resident slot
 
 
 @dataclass
@@ -174,6 +176,8 @@ class GpTimes:
     cflBlocks: int = 0
     rangeFixes: int = 0
     deviceBytes: int = 0
+    schedule: int = 0
+    queueBlocks: int = 0
 
 
 def check_partitions(P: GpParams):
@@ -253,7 +257,7 @@ class GpEncoder:
         if self._h:
             lib().av1mi_destroy(self._h)
             self._h = None
-        c = _Config(cfg.qindex, cfg.edgeFilter, cfg.P._c())
+        c = _Config(cfg.qindex, cfg.edgeFilter, cfg.P._c(), cfg.schedule, cfg.queueBlocks)
         h = C.c_void_p()
         err = C.create_string_buffer(512)
         if lib().av1mi_create(C.byref(c), C.byref(h), err, 512) != 0:

@@ -560,4 +560,14 @@ struct Batch {
     uint32_t *stats;             // [8]: blocks, skip blocks, CfL blocks, range repairs
 };
 
+// the device work queue of the persistent closed-loop schedule
+struct RcQueue {
+    const uint32_t *order;       // [n] superblocks in anti-diagonal wavefront order (step, image)
+    uint32_t n;
+    uint32_t *head;              // next item to take
+    uint32_t *done;              // [nSb] set when a superblock is complete
+    uint32_t *abort;             // raised by a wait that ran out of time
+    uint64_t spinTicks;          // wait limit in wall-clock ticks
+};
+
 }  // namespace av1m

@@ -19,6 +19,8 @@ void build_tables(DevTables &T);
 void launch_analysis(const Batch &B, const SearchParams &P, const uint32_t *shTotal, int bd, hipStream_t st);
 void launch_dp(const Batch &B, const SearchParams &P, hipStream_t st);
 void launch_recon(const Batch &B, const SearchParams &P, const uint32_t *dSbList, uint32_t n, int bd, hipStream_t st);
+void launch_recon_queue(const Batch &B, const SearchParams &P, const RcQueue &Q, uint32_t blocks, int bd, hipStream_t st);
+int recon_queue_slots(int bd);
 int recon_prepare(int bd);
 size_t recon_smem_bytes();
 void launch_pack(const Batch &B, const PackOff *dOff, RecBlock *oB, uint32_t *oBT, RecTxb *oT, int16_t *oC, hipStream_t st);
@@ -92,6 +94,8 @@ struct av1mi_encoder {
     int device = 0;
     DevTables *dT = nullptr;
     hipEvent_t ev[8] = {};
+    int cus = 0, slots[2] = {0, 0};   // compute units; resident closed-loop workgroups per CU (8-bit, 10-bit)
+    uint64_t clockHz = 100000000;     // wall_clock64 rate
 };
 
 extern "C" {
@@ -190,6 +194,12 @@ int av1mi_create(const av1mi_config *cfg, av1mi_encoder **out, char *err, int er
     free(h);
     if (he != hipSuccess) { snprintf(err, errlen, "HIP: %s (table upload)", hipGetErrorString(he)); delete e; return -1; }
     if (recon_prepare(8) || recon_prepare(10)) { snprintf(err, errlen, "closed-loop kernel needs %zu bytes of LDS", recon_smem_bytes()); delete e; return -1; }
+    if (c.schedule < 0 || c.schedule > 1 || c.queueBlocks < 0) { snprintf(err, errlen, "bad schedule %d / queueBlocks %d", c.schedule, c.queueBlocks); delete e; return -1; }
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, e->device) == hipSuccess) e->cus = v;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeWallClockRate, e->device) == hipSuccess && v > 0) e->clockHz = (uint64_t)v * 1000;
+    e->slots[0] = recon_queue_slots(8);
+    e->slots[1] = recon_queue_slots(10);
     for (auto &v : e->ev)
         if ((he = hipEventCreate(&v)) != hipSuccess) { snprintf(err, errlen, "HIP: %s (event)", hipGetErrorString(he)); av1mi_destroy(e); return -1; }
     // keep freed stream-ordered allocations cached between batches
@@ -301,9 +311,10 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     std::vector<uint32_t> stepFlat, stepOff;
     for (auto &v : steps) { stepOff.push_back((uint32_t)stepFlat.size()); stepFlat.insert(stepFlat.end(), v.begin(), v.end()); }
     stepOff.push_back((uint32_t)stepFlat.size());
-    uint32_t *dSteps;
+    if (stepFlat.size() != nSb) { snprintf(err, errlen, "internal: wavefront order covers %zu of %u superblocks", stepFlat.size(), nSb); return -1; }
+    uint32_t *dSteps, *dDone, *dQctl;
     PackOff *dPO;
-    A.add(dSteps, stepFlat.size()); A.add(dPO, nSb);
+    A.add(dSteps, stepFlat.size()); A.add(dPO, nSb); A.add(dDone, nSb); A.add(dQctl, 2);
     HCK(A.alloc(st));
     tm.deviceBytes = A.total;
     for (int i = 0; i < n; i++) {
@@ -322,6 +333,8 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
         HCK(hipMemsetAsync(recMem[i], 0, ysz + 2 * csz, st));
     }
     HCK(hipMemsetAsync(dStats, 0, 8 * 4, st));
+    HCK(hipMemsetAsync(dDone, 0, (size_t)nSb * 4, st));
+    HCK(hipMemsetAsync(dQctl, 0, 2 * 4, st));
     HCK(hipMemcpyAsync(dImg, imgs.data(), n * sizeof(Img), hipMemcpyHostToDevice, st));
     HCK(hipMemcpyAsync(dShStart, shStart.data(), shStart.size() * 4, hipMemcpyHostToDevice, st));
     HCK(hipMemcpyAsync(dSbImg, sbImg.data(), (size_t)nSb * 4, hipMemcpyHostToDevice, st));
@@ -338,17 +351,34 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     launch_dp(B, P, st);
     HCK(hipGetLastError());
     (void)hipEventRecord(enc->ev[3], st);
-    for (size_t s = 0; s + 1 < stepOff.size(); s++) launch_recon(B, P, dSteps + stepOff[s], stepOff[s + 1] - stepOff[s], bd, st);
+    const int slots = enc->slots[bd == 10] > 0 ? enc->slots[bd == 10] : 1;
+    const uint32_t qBlocks = enc->cfg.queueBlocks > 0 ? (uint32_t)enc->cfg.queueBlocks : (uint32_t)std::max(1, enc->cus * slots);
+    if (enc->cfg.schedule == 1) {
+        for (size_t s = 0; s + 1 < stepOff.size(); s++) launch_recon(B, P, dSteps + stepOff[s], stepOff[s + 1] - stepOff[s], bd, st);
+    } else {
+        RcQueue Q;
+        Q.order = dSteps; Q.n = nSb; Q.head = dQctl; Q.abort = dQctl + 1; Q.done = dDone;
+        Q.spinTicks = enc->clockHz * 30;   // no legitimate wait comes near 30 s
+        launch_recon_queue(B, P, Q, std::min<uint32_t>(qBlocks, nSb), bd, st);
+    }
     HCK(hipGetLastError());
+    tm.schedule = (uint32_t)enc->cfg.schedule;
+    tm.queueBlocks = enc->cfg.schedule == 1 ? 0 : std::min<uint32_t>(qBlocks, nSb);
     (void)hipEventRecord(enc->ev[4], st);
     tm.steps = (uint32_t)steps.size();
     tm.sbs = nSb;
     // ---- pack: prefix sums in decode order (tiles raster, SBs raster inside each tile) ----
     std::vector<SbOut> so(nSb);
-    uint32_t stats[8];
+    uint32_t stats[8], qctl[2];
     HCK(hipMemcpyAsync(so.data(), dSbOut, (size_t)nSb * sizeof(SbOut), hipMemcpyDeviceToHost, st));
     HCK(hipMemcpyAsync(stats, dStats, sizeof stats, hipMemcpyDeviceToHost, st));
+    HCK(hipMemcpyAsync(qctl, dQctl, sizeof qctl, hipMemcpyDeviceToHost, st));
     HCK(hipStreamSynchronize(st));
+    if (qctl[1]) {
+        snprintf(err, errlen, "closed-loop queue: a superblock dependency wait timed out (schedule aborted)");
+        A.release(st);
+        return -1;
+    }
     std::vector<PackOff> po(nSb);
     std::vector<uint64_t> imgBlk(n + 1), imgTxb(n + 1), imgCoef(n + 1);
     uint64_t gBlk = 0, gTxb = 0, gCoef = 0;

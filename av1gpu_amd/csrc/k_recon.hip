@@ -64,6 +64,7 @@ struct RcShared {
     int satdI[32], uvc[32];
     int nTxb, nCoef, errFlag, nzFlag, eobOut, culOut;
     int am, lm, mode, delta, uvMode, uvDelta, nuv, bsize;
+    uint32_t qItem;                          // persistent schedule: the superblock taken
 };
 
 template <typename Px> __device__ __forceinline__ Px *rec_plane(const Img &I, int p) { return (Px *)I.rec[p]; }
@@ -764,11 +765,27 @@ __device__ void rc_chroma(lds<RcShared> &S, const Batch &B, const SearchParams &
     }
 }
 
+// stage the cost tables of quantizer context qctx (and, when first, the small forward matrices) in LDS
+__device__ void stage_tables(lds<RcShared> &S, const DevTables &T, int qctx, bool matrices) {
+    const int tid = threadIdx.x;
+    const float *gc = (const float *)&T.cost[qctx];
+    lds<float> *lc = (lds<float> *)&S.cost;
+    for (int k = tid; k < (int)(sizeof(CostTab) / 4); k += RC_NT) lc[k] = gc[k];
+    if (matrices) {
+        for (int k = tid; k < 3 * 336; k += RC_NT) {
+            const int kind = k / 336, r = k % 336, ln = r < 16 ? 2 : r < 80 ? 3 : 4, e = r - (ln == 2 ? 0 : ln == 3 ? 16 : 80);
+            S.tab.fwdT[k] = T.fwdT[T.fwdOff[kind][ln] + e];
+            S.tab.fwd[k] = T.fwd[T.fwdOff[kind][ln] + e];
+        }
+        if (tid < 21) S.tab.fwdOff[tid / 7][tid % 7] = (tid / 7) * 336 + ((tid % 7) == 3 ? 16 : (tid % 7) == 4 ? 80 : 0);
+        if (tid < 19) S.tab.txScale[tid] = T.txScale[tid];
+        for (int k = tid; k < 19 * 16; k += RC_NT) S.tab.txg[k / 16][k % 16] = T.txg[k / 16][k % 16];
+    }
+}
+
+// the closed loop of superblock sb (the workgroup; the tables staged for its image's quantizer context)
 template <typename Px>
-__global__ void __launch_bounds__(RC_NT) k_recon(Batch B, SearchParams P, const uint32_t *sbList) {
-    extern __shared__ __align__(16) char smraw[];
-    lds<RcShared> &S = *(lds<RcShared> *)(lds<char> *)smraw;
-    const uint32_t sb = sbList[blockIdx.x];
+__device__ void recon_sb(lds<RcShared> &S, const Batch &B, const SearchParams &P, uint32_t sb) {
     const Img &I = B.img[B.sbImg[sb]];
     const DevTables &T = *B.T;
     const uint32_t local = sb - I.sbBase;
@@ -789,20 +806,6 @@ __global__ void __launch_bounds__(RC_NT) k_recon(Batch B, SearchParams P, const 
         S.bdm[p][yy + 1][xx + 1] = v;
     }
     if (tid == 0) { S.nTxb = 0; S.nCoef = 0; }
-    // stage the cost tables of the quantizer context and the small forward matrices in LDS
-    {
-        const float *gc = (const float *)&T.cost[I.qctx];
-        lds<float> *lc = (lds<float> *)&S.cost;
-        for (int k = tid; k < (int)(sizeof(CostTab) / 4); k += RC_NT) lc[k] = gc[k];
-        for (int k = tid; k < 3 * 336; k += RC_NT) {
-            const int kind = k / 336, r = k % 336, ln = r < 16 ? 2 : r < 80 ? 3 : 4, e = r - (ln == 2 ? 0 : ln == 3 ? 16 : 80);
-            S.tab.fwdT[k] = T.fwdT[T.fwdOff[kind][ln] + e];
-            S.tab.fwd[k] = T.fwd[T.fwdOff[kind][ln] + e];
-        }
-        if (tid < 21) S.tab.fwdOff[tid / 7][tid % 7] = (tid / 7) * 336 + ((tid % 7) == 3 ? 16 : (tid % 7) == 4 ? 80 : 0);
-        if (tid < 19) S.tab.txScale[tid] = T.txScale[tid];
-        for (int k = tid; k < 19 * 16; k += RC_NT) S.tab.txg[k / 16][k % 16] = T.txg[k / 16][k % 16];
-    }
     __syncthreads();
     const float lambda = T.lambda[I.bd == 10][I.qindex] * P.lambdaMul;
     const int nLeaves = B.leafCount[sb];
@@ -872,18 +875,112 @@ __global__ void __launch_bounds__(RC_NT) k_recon(Batch B, SearchParams P, const 
     }
 }
 
+// one launch per wavefront step: workgroup b takes superblock sbList[b]
+template <typename Px>
+__global__ void __launch_bounds__(RC_NT) k_recon(Batch B, SearchParams P, const uint32_t *sbList) {
+    extern __shared__ __align__(16) char smraw[];
+    lds<RcShared> &S = *(lds<RcShared> *)(lds<char> *)smraw;
+    const uint32_t sb = sbList[blockIdx.x];
+    stage_tables(S, *B.T, B.img[B.sbImg[sb]].qctx, true);
+    __syncthreads();
+    recon_sb<Px>(S, B, P, sb);
+}
+
+// wait (thread 0) until superblock flag f is set; false when the wait ran out of time or another workgroup aborted
+__device__ bool wait_done(const uint32_t *f, const RcQueue &Q) {
+    const uint64_t t0 = wall_clock64();
+    while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        if (__hip_atomic_load(Q.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+        if (wall_clock64() - t0 > Q.spinTicks) {
+            __hip_atomic_store(Q.abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return true;
+}
+
+// Persistent schedule: every workgroup pulls superblocks from the queue (the anti-diagonal wavefront order of all
+// images of the batch) and starts each as soon as its left and above-right (above, at a tile's right edge)
+// neighbours inside its tile are done.  Items are taken in an order where every dependency was taken earlier by a
+// running workgroup, so the waits always end; a wait that exceeds the time limit raises the abort flag, which makes
+// every workgroup stop taking items and return.
+template <typename Px>
+__global__ void __launch_bounds__(RC_NT) k_recon_queue(Batch B, SearchParams P, RcQueue Q) {
+    extern __shared__ __align__(16) char smraw[];
+    lds<RcShared> &S = *(lds<RcShared> *)(lds<char> *)smraw;
+    const int tid = threadIdx.x;
+    int staged = -1;
+    for (;;) {
+        if (tid == 0) {
+            uint32_t it = Q.n;
+            if (!__hip_atomic_load(Q.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                it = __hip_atomic_fetch_add(Q.head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t sb = 0xFFFFFFFFu;
+            if (it < Q.n) {
+                sb = Q.order[it];
+                const Img &I = B.img[B.sbImg[sb]];
+                const uint32_t local = sb - I.sbBase;
+                const int c = (int)(local % I.sbCols), r = (int)(local / I.sbCols);
+                int tc = 0, tr = 0;
+                while (tc + 1 < I.tileCols && c * 16 >= I.tileColStart[tc + 1]) tc++;
+                while (tr + 1 < I.tileRows && r * 16 >= I.tileRowStart[tr + 1]) tr++;
+                const int c0 = I.tileColStart[tc] >> 4, c1 = (I.tileColStart[tc + 1] + 15) >> 4, r0 = I.tileRowStart[tr] >> 4;
+                bool ok = true;
+                if (c > c0) ok = wait_done(Q.done + sb - 1, Q);
+                if (ok && r > r0) ok = wait_done(Q.done + sb - I.sbCols + (c + 1 < c1 ? 1 : 0), Q);
+                if (!ok) sb = 0xFFFFFFFFu;
+            }
+            S.qItem = sb;
+        }
+        __syncthreads();
+        const uint32_t sb = S.qItem;
+        if (sb == 0xFFFFFFFFu) break;
+        // the neighbours' reconstruction, maps and contexts were published with release at agent scope
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const int qctx = B.img[B.sbImg[sb]].qctx;
+        if (qctx != staged) {
+            stage_tables(S, *B.T, qctx, staged < 0);
+            staged = qctx;
+        }
+        __syncthreads();
+        recon_sb<Px>(S, B, P, sb);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(Q.done + sb, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 size_t recon_smem_bytes() { return sizeof(RcShared); }
 
 int recon_prepare(int bd) {
-    const hipError_t e = bd == 10 ? hipFuncSetAttribute((const void *)k_recon<uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(RcShared))
-                                  : hipFuncSetAttribute((const void *)k_recon<uint8_t>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(RcShared));
-    return e == hipSuccess ? 0 : -1;
+    const int sz = (int)sizeof(RcShared);
+    const hipError_t e = bd == 10 ? hipFuncSetAttribute((const void *)k_recon<uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize, sz)
+                                  : hipFuncSetAttribute((const void *)k_recon<uint8_t>, hipFuncAttributeMaxDynamicSharedMemorySize, sz);
+    const hipError_t f = bd == 10 ? hipFuncSetAttribute((const void *)k_recon_queue<uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize, sz)
+                                  : hipFuncSetAttribute((const void *)k_recon_queue<uint8_t>, hipFuncAttributeMaxDynamicSharedMemorySize, sz);
+    return e == hipSuccess && f == hipSuccess ? 0 : -1;
 }
 
 void launch_recon(const Batch &B, const SearchParams &P, const uint32_t *dSbList, uint32_t n, int bd, hipStream_t st) {
     if (!n) return;
     if (bd == 10) hipLaunchKernelGGL(k_recon<uint16_t>, dim3(n), dim3(RC_NT), sizeof(RcShared), st, B, P, dSbList);
     else hipLaunchKernelGGL(k_recon<uint8_t>, dim3(n), dim3(RC_NT), sizeof(RcShared), st, B, P, dSbList);
+}
+
+// resident workgroups of the persistent kernel per compute unit
+int recon_queue_slots(int bd) {
+    int n = 0;
+    const hipError_t e = bd == 10 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_recon_queue<uint16_t>, RC_NT, sizeof(RcShared))
+                                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_recon_queue<uint8_t>, RC_NT, sizeof(RcShared));
+    return e == hipSuccess ? n : 0;
+}
+
+void launch_recon_queue(const Batch &B, const SearchParams &P, const RcQueue &Q, uint32_t blocks, int bd, hipStream_t st) {
+    if (!Q.n || !blocks) return;
+    if (bd == 10) hipLaunchKernelGGL(k_recon_queue<uint16_t>, dim3(blocks), dim3(RC_NT), sizeof(RcShared), st, B, P, Q);
+    else hipLaunchKernelGGL(k_recon_queue<uint8_t>, dim3(blocks), dim3(RC_NT), sizeof(RcShared), st, B, P, Q);
 }
 
 }  // namespace av1m

@@ -6,9 +6,9 @@ exhaustive search on one 4000 x 3000 8-bit 4:2:0 synthetic image (seed 1000) at 
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 
-roofline: the dominant kernel's algorithmic HBM bytes (source planes read once + reconstruction written once +
-2 bytes per coefficient + 16 bytes per block and tx-block record) per launch over its average launch duration, timed
-with HIP events on the encode stream.  cpu_baseline: the CPU restatement (oracle/liborc.so) on a bounded crop of the
+roofline: the dominant kernel's (the closed loop's) algorithmic HBM bytes (source planes read once + reconstruction
+written once + 2 bytes per coefficient + 16 bytes per block and tx-block record) per launch over its average launch
+duration, timed with HIP events on the encode stream.  cpu_baseline: the CPU restatement (oracle/liborc.so) on a bounded crop of the
 same image with all host threads.
 """
 import argparse
@@ -33,6 +33,7 @@ def main():
     ap.add_argument("--height", type=int, default=3000)
     ap.add_argument("--qindex", type=int, default=120)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--schedule", type=int, default=0, help="closed loop: 0 persistent queue, 1 one launch per wavefront step")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only (the path shards by image / tile with no collective); "
@@ -43,7 +44,7 @@ def main():
     W, H = a.width, a.height
     planes = A.synth_image(1000, W, H)
     frame = A.DeviceFrame(planes, W, H, qindex=a.qindex)
-    enc = A.GpEncoder().init(A.GpConfig(qindex=a.qindex))
+    enc = A.GpEncoder().init(A.GpConfig(qindex=a.qindex, schedule=a.schedule))
     for _ in range(a.warmup):
         enc.encode([frame], want_recon=True)
     A.lib().av1mi_device_sync()
@@ -64,9 +65,11 @@ def main():
     recon_ms = sum(s.recon for s in stage) / len(stage)
     analysis_ms = sum(s.analysis for s in stage) / len(stage)
     steps = stage[-1].steps
-    # dominant kernel: the closed loop (one launch per wavefront step); per launch: the image's bytes / steps
-    per_launch_ms = recon_ms / max(steps, 1)
-    achieved = (alg_bytes / max(steps, 1)) / (per_launch_ms * 1e-3) / 1e9
+    # dominant kernel: the closed loop, one persistent launch per encode (queue schedule) or one per wavefront step
+    launches = 1 if stage[-1].schedule == 0 else max(steps, 1)
+    kernel = "k_recon_queue" if stage[-1].schedule == 0 else "k_recon"
+    per_launch_ms = recon_ms / launches
+    achieved = (alg_bytes / launches) / (per_launch_ms * 1e-3) / 1e9
     line = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -88,9 +91,11 @@ def main():
                       "download": round(sum(s.download for s in stage) / len(stage), 3),
                       "total_events": round(sum(s.total for s in stage) / len(stage), 3)},
         "record": {"blocks": n_blocks, "txb": n_txb, "coeffs": n_coeffs, "wavefront_steps": steps, "sbs": stage[-1].sbs},
-        "roofline": {"bound": "hbm", "kernel": "k_recon", "achieved": round(achieved, 4), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "schedule": {"closed_loop": "persistent queue" if stage[-1].schedule == 0 else "launch per wavefront step",
+                     "workgroups": stage[-1].queueBlocks},
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 4), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "per_launch_ms": round(per_launch_ms, 4), "alg_bytes_per_launch": round(alg_bytes / max(steps, 1))},
+                     "per_launch_ms": round(per_launch_ms, 4), "alg_bytes_per_launch": round(alg_bytes / launches)},
     }
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(planes, W, H, a.qindex)

@@ -52,6 +52,11 @@ typedef struct av1mi_config {
     int qindex;        /* base_q_idx 1..255 of frames that do not set their own */
     int edgeFilter;    /* enable_intra_edge_filter */
     av1mi_params P;
+    /* closed-loop schedule (no counterpart in the reference, which launches once per wavefront step,
+     * pipeline.cpp:815-827): 0 = persistent queue (default), 1 = one launch per wavefront step.  queueBlocks:
+     * workgroups of the persistent kernel, 0 = every resident slot of the device.  Results do not depend on either. */
+    int schedule;
+    int queueBlocks;
 } av1mi_config;
 
 /* GpFrameIn (pipeline.h:149-163): device planes padded (edge replicated) to aw x ah, multiples of 64; chroma
@@ -86,6 +91,7 @@ typedef struct av1mi_times {
     float setup, analysis, dp, recon, pack, download, total;
     uint32_t steps, sbs, blocks, skipBlocks, cflBlocks, rangeFixes;
     size_t deviceBytes;
+    uint32_t schedule, queueBlocks;   /* the closed-loop schedule used (0 queue, 1 steps) and its workgroups */
 } av1mi_times;
 
 typedef struct av1mi_encoder av1mi_encoder;

@@ -96,3 +96,22 @@ def test_gpu_full_size_record_decodes(enc, ref):
     g = g[0]
     assert g.tile_rows == 2 and t.sbs == 63 * 47
     _decodes(ref, g, W, H, 120)
+
+
+def test_gpu_schedules_agree(enc):
+    """The persistent queue (every resident slot, and only 3 workgroups, which makes most superblocks wait on their
+    neighbours) and one launch per wavefront step give identical records for a batch of mixed sizes and qindex."""
+    specs = [(1000, 700, 1080, 120), (300, 200, 1081, 80), (64, 64, 1082, 160), (520, 130, 1083, 120)]
+    frames = [A.DeviceFrame(A.synth_image(s, W, H), W, H, qindex=q) for W, H, s, q in specs]
+    base, tq = enc.encode(frames)
+    assert tq.schedule == 0 and tq.queueBlocks > 0
+    for sched, qb in ((1, 0), (0, 3)):
+        e = A.GpEncoder().init(A.GpConfig(qindex=120, schedule=sched, queueBlocks=qb))
+        print("schedule", sched, "queueBlocks", qb, flush=True)
+        out, t = e.encode(frames)
+        assert t.schedule == sched
+        for a, b in zip(base, out):
+            assert np.array_equal(a.blocks, b.blocks) and np.array_equal(a.txb, b.txb) and np.array_equal(a.coeffs, b.coeffs)
+            for p in range(3):
+                assert np.array_equal(a.recon[p], b.recon[p])
+        e.close()
