@@ -212,6 +212,22 @@ def synth_image(seed, W, H, ssx=1, bd=8):
     return [y, u, v]
 
 
+def batch_spec(n=256, seed=1000):
+    """The mixed-size batch workload: n images with sizes log-uniform over 0.18-24 MP, aspect ratio (W / H) uniform over
+    [0.5, 2], sides rounded to multiples of 8, qindex drawn from {80, 120, 160}; image i is the synthetic image of seed
+    seed + i.  Returns [(W, H, qindex, seed)]."""
+    import math
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        mp = math.exp(rng.uniform(math.log(0.18), math.log(24.0)))
+        a = rng.uniform(0.5, 2.0)
+        W = max(8, int(round(math.sqrt(mp * 1e6 * a) / 8.0)) * 8)
+        H = max(8, int(round(math.sqrt(mp * 1e6 / a) / 8.0)) * 8)
+        out.append((W, H, int(rng.choice([80, 120, 160])), seed + i))
+    return out
+
+
 class DeviceFrame:
     """Padded planes of one frame resident in device memory (owned; freed on close / garbage collection)."""
 

@@ -52,7 +52,7 @@ AV1M_HD int cand_delta(int c) { return (c & 7) - 3; }
 AV1M_HD int cand_pack(int m, int d) { return (m << 3) | (d + 3); }
 AV1M_HD int pangle(int mode, int delta) { return is_dir(mode) ? av1t::Mode_To_Angle[mode] + 3 * delta : 0; }
 
-AV1M_HD int tx_set_intra(int t) {
+AV1M_HD constexpr int tx_set_intra(int t) {
     if (av1t::Tx_Size_Sqr_Up[t] >= TX_32X32) return TX_SET_DCTONLY;
     return av1t::Tx_Size_Sqr[t] == TX_16X16 ? TX_SET_INTRA_2 : TX_SET_INTRA_1;
 }
@@ -62,7 +62,7 @@ AV1M_HD int chroma_tx_type(int uvTx, int uvMode) {
     const int t = av1t::Mode_To_Txfm[uvMode];
     return tx_in_set(tx_set_intra(uvTx), t) ? t : DCT_DCT;
 }
-AV1M_HD int tx_class(int t) {
+AV1M_HD constexpr int tx_class(int t) {
     return (t == V_DCT || t == V_ADST || t == V_FLIPADST) ? TX_CLASS_VERT : (t == H_DCT || t == H_ADST || t == H_FLIPADST) ? TX_CLASS_HORIZ : TX_CLASS_2D;
 }
 // 1-D kinds of the row (horizontal) and column (vertical) passes; flips
@@ -110,6 +110,16 @@ AV1M_HD const uint16_t *scan_of(int t, int txType) {
          : t == TX_4X16 ? Default_Scan_4x16 : t == TX_16X4 ? Default_Scan_16x4 : t == TX_8X32 ? Default_Scan_8x32
          : t == TX_32X8 ? Default_Scan_32x8 : Default_Scan_32x32;
 }
+
+// the default (zig-zag) scans of the tx sizes without a 64 side, concatenated in tx-size order: offset of size t
+constexpr int DSCAN_TOTAL = 3344;
+AV1M_HD constexpr int dscan_off(int t) {
+    return t == TX_4X4 ? 0 : t == TX_8X8 ? 16 : t == TX_16X16 ? 80 : t == TX_32X32 ? 336 : t == TX_4X8 ? 1360 : t == TX_8X4 ? 1392
+         : t == TX_8X16 ? 1424 : t == TX_16X8 ? 1552 : t == TX_16X32 ? 1680 : t == TX_32X16 ? 2192 : t == TX_4X16 ? 2704
+         : t == TX_16X4 ? 2768 : t == TX_8X32 ? 2832 : 3088;
+}
+// the coefficient context of tx size t (get_tx_size_context)
+AV1M_HD constexpr int txsz_ctx_of(int t) { return (av1t::Tx_Size_Sqr[t] + av1t::Tx_Size_Sqr_Up[t] + 1) >> 1; }
 
 // ---- block geometry ----
 AV1M_HD int has_chroma(int bs, int mi_row, int mi_col, int mono, int ss) {

@@ -13,6 +13,14 @@ namespace av1m {
 
 constexpr int GW = 32;   // lanes per group
 
+constexpr int tx_of(int W, int H) {
+    return W == H ? (W == 4 ? TX_4X4 : W == 8 ? TX_8X8 : W == 16 ? TX_16X16 : W == 32 ? TX_32X32 : TX_64X64)
+         : (W == 4 && H == 8) ? TX_4X8 : (W == 8 && H == 4) ? TX_8X4 : (W == 8 && H == 16) ? TX_8X16 : (W == 16 && H == 8) ? TX_16X8
+         : (W == 16 && H == 32) ? TX_16X32 : (W == 32 && H == 16) ? TX_32X16 : (W == 32 && H == 64) ? TX_32X64
+         : (W == 64 && H == 32) ? TX_64X32 : (W == 4 && H == 16) ? TX_4X16 : (W == 16 && H == 4) ? TX_16X4
+         : (W == 8 && H == 32) ? TX_8X32 : (W == 32 && H == 8) ? TX_32X8 : (W == 16 && H == 64) ? TX_16X64 : TX_64X16;
+}
+
 // LDS view of a type: accesses through it compile to ds_* instructions, also inside functions the compiler keeps out
 // of line (where a plain pointer would lose its address space and go through flat instructions)
 template <typename T> using lds = __attribute__((address_space(3))) T;
@@ -46,6 +54,11 @@ __device__ __forceinline__ int gsumi(int v) {
     for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
     return v;
 }
+__device__ __forceinline__ long long gsuml(long long v) {
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+    return v;
+}
 __device__ __forceinline__ int gmaxi(int v) {
 #pragma unroll
     for (int o = 16; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 32));
@@ -74,27 +87,30 @@ struct EvalOut { float D, R; int eob; };
 
 __device__ __forceinline__ int txsz_ctx(int t) { return (av1t::Tx_Size_Sqr[t] + av1t::Tx_Size_Sqr_Up[t] + 1) >> 1; }
 
-// txb_skip, luma tx type and eob position bits
-template <typename E>
+// txb_skip, luma tx type and eob position bits of a tx block of size TSZ
+template <int TSZ, typename E>
 __device__ float base_bits(const E &e, int eob) {
     const auto &C = *e.C;
-    const int t = e.txSz, ctx = txsz_ctx(t);
+    constexpr int t = TSZ, ctx = txsz_ctx_of(t);
     if (eob == 0) return C.txb_skip[ctx][e.skipCtx][1];
     float b = C.txb_skip[ctx][e.skipCtx][0];
     const int pt = e.plane > 0;
     if (e.plane == 0) {
-        const int set = tx_set_intra(t), sqr = av1t::Tx_Size_Sqr[t];
-        if (set == TX_SET_INTRA_1) {
+        constexpr int set = tx_set_intra(t), sqr = av1t::Tx_Size_Sqr[t];
+        if constexpr (set == TX_SET_INTRA_1) {
             int idx = 0;
+#pragma unroll
             for (int i = 0; i < 7; i++) idx = av1t::Tx_Type_Intra_Inv_Set1[i] == e.txType ? i : idx;
             b += C.tx_set1[sqr][e.intraDir][idx];
-        } else if (set == TX_SET_INTRA_2) {
+        } else if constexpr (set == TX_SET_INTRA_2) {
             int idx = 0;
+#pragma unroll
             for (int i = 0; i < 5; i++) idx = av1t::Tx_Type_Intra_Inv_Set2[i] == e.txType ? i : idx;
             b += C.tx_set2[sqr][e.intraDir][idx];
         }
     }
-    const int ms = min((int)av1t::Tx_Width_Log2[t], 5) + min((int)av1t::Tx_Height_Log2[t], 5) - 4;
+    constexpr int lw = av1t::Tx_Width_Log2[t], lh = av1t::Tx_Height_Log2[t];
+    constexpr int ms = (lw < 5 ? lw : 5) + (lh < 5 ? lh : 5) - 4;
     const int eobPt = eob <= 2 ? eob : 2 + (31 - __clz(eob - 1));
     const int ec = tx_class(e.txType) == TX_CLASS_2D ? 0 : 1;
     const float p = ms == 0 ? C.eob_pt16[pt][ec][eobPt - 1] : ms == 1 ? C.eob_pt32[pt][ec][eobPt - 1]
@@ -109,12 +125,22 @@ __device__ float base_bits(const E &e, int eob) {
     return b;
 }
 
-// bits of the coefficient at scan position c (< eob) given the levels q (natural order, stride = adjusted width)
-template <typename QP, typename CT>
-__device__ float coeff_bits(int c, int eob, QP q, const uint16_t *scan, int t, int cls, int pt, int dcCtx, const CT &C) {
-    const int ctx = txsz_ctx(t), adj = av1t::Adjusted_Tx_Size[t];
-    const int bwl = av1t::Tx_Width_Log2[adj], width = 1 << bwl, height = av1t::Tx_Height[adj];
-    const int pos = scan[c], qv = q[pos], v = qv < 0 ? -qv : qv, level = v < 15 ? v : 15;
+// scan position of index c in a tx block of the adjusted size TW x TH: class VERT scans rows (raster order), HORIZ
+// columns (Mrow / Mcol scans), 2D the default zig-zag scan dscan
+template <int TW, int TH, int CLS, typename SP>
+__device__ __forceinline__ int scan_pos(int c, SP dscan) {
+    if constexpr (CLS == TX_CLASS_VERT) return c;
+    else if constexpr (CLS == TX_CLASS_HORIZ) return (c % TH) * TW + c / TH;
+    else return dscan[c];
+}
+
+// bits of the coefficient at scan position c (< eob) of a tx block of size TSZ and class CLS given the levels q
+// (natural order, stride = adjusted width); cbOff: Coeff_Base_Ctx_Offset[TSZ] (5 x 5)
+template <int TSZ, int CLS, typename QP, typename SP, typename OP, typename CT>
+__device__ float coeff_bits(int c, int eob, QP q, SP dscan, OP cbOff, int pt, int dcCtx, const CT &C) {
+    constexpr int ctx = txsz_ctx_of(TSZ), adj = av1t::Adjusted_Tx_Size[TSZ];
+    constexpr int bwl = av1t::Tx_Width_Log2[adj], width = 1 << bwl, height = av1t::Tx_Height[adj];
+    const int pos = scan_pos<width, height, CLS>(c, dscan), qv = q[pos], v = qv < 0 ? -qv : qv, level = v < 15 ? v : 15;
     const int row = pos >> bwl, col = pos - (row << bwl);
     float b;
     if (c == eob - 1) {
@@ -124,24 +150,25 @@ __device__ float coeff_bits(int c, int eob, QP q, const uint16_t *scan, int t, i
         int mag = 0;
 #pragma unroll
         for (int k = 0; k < 5; k++) {
-            const int rr = row + av1t::Sig_Ref_Diff_Offset[cls][k][0], cc = col + av1t::Sig_Ref_Diff_Offset[cls][k][1];
+            const int rr = row + av1t::Sig_Ref_Diff_Offset[CLS][k][0], cc = col + av1t::Sig_Ref_Diff_Offset[CLS][k][1];
             if (rr < height && cc < width) { const int a = q[(rr << bwl) + cc]; mag += min(a < 0 ? -a : a, 3); }
         }
         int bctx = min((mag + 1) >> 1, 4);
-        if (cls == TX_CLASS_2D) bctx = (row == 0 && col == 0) ? 0 : bctx + av1t::Coeff_Base_Ctx_Offset[t][min(row, 4)][min(col, 4)];
-        else bctx += av1t::Coeff_Base_Pos_Ctx_Offset[min(cls == TX_CLASS_VERT ? row : col, 2)];
+        if constexpr (CLS == TX_CLASS_2D) bctx = (row == 0 && col == 0) ? 0 : bctx + cbOff[min(row, 4) * 5 + min(col, 4)];
+        else bctx += min(CLS == TX_CLASS_VERT ? row : col, 2) == 0 ? av1t::Coeff_Base_Pos_Ctx_Offset[0]
+                   : min(CLS == TX_CLASS_VERT ? row : col, 2) == 1 ? av1t::Coeff_Base_Pos_Ctx_Offset[1] : av1t::Coeff_Base_Pos_Ctx_Offset[2];
         b = C.coeff_base[ctx][pt][bctx][min(level, 3)];
     }
     if (level > 2) {
         int mag = 0;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            const int rr = row + av1t::Mag_Ref_Offset_With_Tx_Class[cls][k][0], cc = col + av1t::Mag_Ref_Offset_With_Tx_Class[cls][k][1];
+            const int rr = row + av1t::Mag_Ref_Offset_With_Tx_Class[CLS][k][0], cc = col + av1t::Mag_Ref_Offset_With_Tx_Class[CLS][k][1];
             if (rr < height && cc < width) { const int a = q[rr * width + cc]; mag += min(a < 0 ? -a : a, 15); }
         }
         mag = min((mag + 1) >> 1, 6);
-        const int brctx = pos == 0 ? mag : cls == TX_CLASS_2D ? ((row < 2 && col < 2) ? mag + 7 : mag + 14)
-                        : cls == TX_CLASS_HORIZ ? (col == 0 ? mag + 7 : mag + 14) : (row == 0 ? mag + 7 : mag + 14);
+        const int brctx = pos == 0 ? mag : CLS == TX_CLASS_2D ? ((row < 2 && col < 2) ? mag + 7 : mag + 14)
+                        : CLS == TX_CLASS_HORIZ ? (col == 0 ? mag + 7 : mag + 14) : (row == 0 ? mag + 7 : mag + 14);
         const auto *br = C.coeff_br[min(ctx, (int)TX_32X32)][pt][brctx];
         int rem = level - 3;
         for (int i = 0; i < 4; i++) {
@@ -158,6 +185,28 @@ __device__ float coeff_bits(int c, int eob, QP q, const uint16_t *scan, int t, i
     return b;
 }
 
+// eob and rate of the levels q of a tx block of size TSZ, class CLS, by the group (all lanes return both)
+template <int TSZ, int CLS, typename QP, typename SP, typename OP, typename E>
+__device__ void grp_rate(QP q, SP dscan, OP cbOff, const E &e, int &eobOut, float &bitsOut) {
+    constexpr int adj = av1t::Adjusted_Tx_Size[TSZ];
+    constexpr int TW = 1 << av1t::Tx_Width_Log2[adj], TH = av1t::Tx_Height[adj], N = TW * TH;
+    const int lane = glane();
+    int last = 0;
+    for (int c = lane; c < N; c += GW)
+        if (q[scan_pos<TW, TH, CLS>(c, dscan)]) last = c + 1;
+    const int eob = gmaxi(last);
+    float bits = lane == 0 ? base_bits<TSZ>(e, eob) : 0.f;
+    for (int c = lane; c < eob; c += GW) bits += coeff_bits<TSZ, CLS>(c, eob, q, dscan, cbOff, e.plane > 0, e.dcCtx, *e.C);
+    eobOut = eob;
+    bitsOut = gsum(bits);
+}
+
+// the default scan and the base-context offsets of tx size TSZ (global tables; the closed loop reads LDS copies)
+template <int TSZ>
+__device__ __forceinline__ const uint16_t *dscan_ptr(const DevTables &) { return scan_of(TSZ, DCT_DCT); }
+template <int TSZ>
+__device__ __forceinline__ const uint8_t *cboff_ptr(const DevTables &) { return &av1t::Coeff_Base_Ctx_Offset[TSZ][0][0]; }
+
 // Evaluation scratch of one group in LDS, for tx blocks up to W x H
 template <int W, int H>
 struct EvalScr {
@@ -170,11 +219,12 @@ struct EvalScr {
 template <int W, int H, typename FP, typename QP, typename E, typename TT>
 __device__ EvalOut grp_qrate(FP tmp, int ts, QP q, const E &e, const TT &T);
 
-// Forward transform + quantization + distortion + rate of the residual in S.res (W x H, row-major), by the 32 lanes of
-// the calling group.  Levels in S.q (natural order, tw x th).  All lanes return the same values.
-template <int W, int H, typename Scr, typename E, typename TT>
-__device__ EvalOut grp_eval(Scr &S, const E &e, const TT &T) {
-    constexpr int TW = W > 32 ? 32 : W, TH = H > 32 ? 32 : H, TS = Scr::TS;
+// Forward transform + quantization + distortion + rate of the residual res (W x H, row-major), by the 32 lanes of
+// the calling group, with tmp (th x (W + 1) floats) as scratch.  Levels into q (natural order, tw x th).  All lanes
+// return the same values.
+template <int W, int H, typename RP, typename FP, typename QP, typename E, typename TT>
+__device__ EvalOut grp_eval(RP res, FP tmp, QP q, const E &e, const TT &T) {
+    constexpr int TW = W > 32 ? 32 : W, TH = H > 32 ? 32 : H, TS = W + 1;
     constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
     const int lane = glane();
     const int rk = row_kind(e.txType), ck = col_kind(e.txType), fud = flip_ud(e.txType), flr = flip_lr(e.txType);
@@ -188,20 +238,20 @@ __device__ EvalOut grp_eval(Scr &S, const E &e, const TT &T) {
             float acc[OC];
             if (ck == K_IDT) {
 #pragma unroll
-                for (int o = 0; o < OC; o++) acc[o] = Fc[(i0 + o) * H + i0 + o] * (float)S.res[(fud ? H - 1 - (i0 + o) : i0 + o) * W + jj];
+                for (int o = 0; o < OC; o++) acc[o] = Fc[(i0 + o) * H + i0 + o] * (float)res[(fud ? H - 1 - (i0 + o) : i0 + o) * W + jj];
             } else {
 #pragma unroll
                 for (int o = 0; o < OC; o++) acc[o] = 0.f;
 #pragma unroll 4
                 for (int k = 0; k < H; k++) {
-                    const float x = (float)S.res[(fud ? H - 1 - k : k) * W + jj];
+                    const float x = (float)res[(fud ? H - 1 - k : k) * W + jj];
                     const auto *f = FcT + k * H + i0;
 #pragma unroll
                     for (int o = 0; o < OC; o++) acc[o] = fmaf(f[o], x, acc[o]);
                 }
             }
 #pragma unroll
-            for (int o = 0; o < OC; o++) S.tmp[(i0 + o) * TS + j] = acc[o];
+            for (int o = 0; o < OC; o++) tmp[(i0 + o) * TS + j] = acc[o];
         }
     }
     gsync();
@@ -212,7 +262,7 @@ __device__ EvalOut grp_eval(Scr &S, const E &e, const TT &T) {
         const float scale = T.txScale[e.txSz];
         if (lane < TH) {
             float acc[TW];
-            auto *row = S.tmp + lane * TS;
+            auto *row = tmp + lane * TS;
             if (rk == K_IDT) {
 #pragma unroll
                 for (int o = 0; o < TW; o++) acc[o] = Fr[o * W + o] * row[o];
@@ -232,7 +282,7 @@ __device__ EvalOut grp_eval(Scr &S, const E &e, const TT &T) {
         }
     }
     gsync();
-    return grp_qrate<W, H>(S.tmp, TS, S.q, e, T);
+    return grp_qrate<W, H>(tmp, TS, q, e, T);
 }
 
 // Quantization, distortion and rate of the coefficients of a W x H tx block held in tmp (row i of the kept tw x th
@@ -262,16 +312,15 @@ __device__ EvalOut grp_qrate(FP tmp, int ts, QP q, const E &e, const TT &T) {
     }
     const float distS = gsum(dist);
     gsync();
-    const uint16_t *scan = scan_of(e.txSz, e.txType);
-    int last = 0;
-    for (int c = lane; c < N; c += GW)
-        if (q[scan[c]]) last = c + 1;
-    const int eob = gmaxi(last);
-    float bits = lane == 0 ? base_bits(e, eob) : 0.f;
+    constexpr int TSZ = tx_of(W, H);
+    const auto dscan = dscan_ptr<TSZ>(T);
+    const auto cbOff = cboff_ptr<TSZ>(T);
     const int cls = tx_class(e.txType);
-    for (int c = lane; c < eob; c += GW) bits += coeff_bits(c, eob, q, scan, e.txSz, cls, e.plane > 0, e.dcCtx, *e.C);
+    int eob;
     EvalOut o;
-    o.R = gsum(bits);
+    if (W > 16 || H > 16 || cls == TX_CLASS_2D) grp_rate<TSZ, TX_CLASS_2D>(q, dscan, cbOff, e, eob, o.R);
+    else if (cls == TX_CLASS_HORIZ) grp_rate<TSZ, TX_CLASS_HORIZ>(q, dscan, cbOff, e, eob, o.R);
+    else grp_rate<TSZ, TX_CLASS_VERT>(q, dscan, cbOff, e, eob, o.R);
     const float g = T.txg[e.txSz][e.txType];
     o.D = g * distS;
     if (W == 64 || H == 64) {
@@ -412,13 +461,6 @@ __device__ __forceinline__ int hadamard4(int *d) {
     return sum;
 }
 
-constexpr int tx_of(int W, int H) {
-    return W == H ? (W == 4 ? TX_4X4 : W == 8 ? TX_8X8 : W == 16 ? TX_16X16 : W == 32 ? TX_32X32 : TX_64X64)
-         : (W == 4 && H == 8) ? TX_4X8 : (W == 8 && H == 4) ? TX_8X4 : (W == 8 && H == 16) ? TX_8X16 : (W == 16 && H == 8) ? TX_16X8
-         : (W == 16 && H == 32) ? TX_16X32 : (W == 32 && H == 16) ? TX_32X16 : (W == 32 && H == 64) ? TX_32X64
-         : (W == 64 && H == 32) ? TX_64X32 : (W == 4 && H == 16) ? TX_4X16 : (W == 16 && H == 4) ? TX_16X4
-         : (W == 8 && H == 32) ? TX_8X32 : (W == 32 && H == 8) ? TX_32X8 : (W == 16 && H == 64) ? TX_16X64 : TX_64X16;
-}
 
 // runtime tx-size dispatch to a template taking <W, H>
 #define AV1M_TX_SWITCH(txSz, M)                                                                                   \

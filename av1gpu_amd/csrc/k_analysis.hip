@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) k_analysis(Batch B, Sear
             }
             e.txType = t;
             e.intraDir = m;
-            const EvalOut o = grp_eval<W, H>(S.ev, e, T);
+            const EvalOut o = grp_eval<W, H>(S.ev.res, S.ev.tmp, S.ev.q, e, T);
             const float J = fmaf(lambda, (o.R + mbits(c)) + overhead, o.D);
             if (J < bestJ) {
                 bestJ = J;

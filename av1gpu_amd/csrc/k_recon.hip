@@ -17,9 +17,18 @@ namespace av1m {
 
 constexpr int RC_NT = 256, RC_NG = 8;
 
+constexpr int RC_SMALL_TASKS = 48, RC_SMALL_RES = 16;
+// per-group scratch of the evaluations of transform blocks up to 16x16
 struct RcGroup {
-    EvalScr<16, 16> ev;
+    float tmp[16 * 17];
     uint16_t ea[EDGE_LEN], el[EDGE_LEN];
+};
+// blocks up to 16x16: the residual of each candidate (luma) or task (chroma) and the levels of every task, kept until
+// the decision so the winner is neither predicted nor quantized again
+struct RcSmall {
+    RcGroup g[RC_NG];
+    int16_t res[RC_SMALL_RES][256];
+    int16_t lev[RC_SMALL_TASKS][256];
 };
 // scratch of one group evaluating a transform block with a side of 32 or 64 (DCT_DCT only)
 struct RcTeam {
@@ -34,11 +43,17 @@ struct RcTabs {
     int fwdOff[3][7];
     float txScale[19];
     float txg[19][16];
+    uint16_t dscan[DSCAN_TOTAL];   // default scans (offsets dscan_off)
+    uint8_t cbOff[19][25];         // Coeff_Base_Ctx_Offset
 };
 using RcEvalIn = EvalInT<const lds<CostTab> *>;
+template <int TSZ>
+__device__ __forceinline__ const lds<uint16_t> *dscan_ptr(const lds<RcTabs> &T) { return T.dscan + dscan_off(av1t::Adjusted_Tx_Size[TSZ]); }
+template <int TSZ>
+__device__ __forceinline__ const lds<uint8_t> *cboff_ptr(const lds<RcTabs> &T) { return T.cbOff[TSZ]; }
 
 union RcScr {
-    RcGroup g[RC_NG];
+    RcSmall s;
     RcTeam t[RC_NG];
 };
 struct RcShared {
@@ -52,7 +67,8 @@ struct RcShared {
     int skipCtx[3], dcCtx[3];
     int gup[RC_NG][2];                       // upsampling flags of each group's prepared edges
     uint8_t bdm[3][18][18];                  // BlockDecoded[plane][y + 1][x + 1]
-    int taskC[64], taskT[64], taskP[64], taskE[64];
+    int taskC[64], taskT[64], taskP[64], taskE[64], taskK[64];   // candidate, tx type, plane, eob, residual slot
+    int candC[8], nCand;                     // luma: the re-checked candidates
     float taskJ[64];
     int nTask;
     float gBestJ[RC_NG];
@@ -165,12 +181,11 @@ __device__ void grp_residual(lds<RcShared> &S, lds<uint16_t> *gea, lds<uint16_t>
     gsync();
 }
 
-// one RD task (candidate c of plane p with tx type t) of a tx block up to 16x16 by the calling group
+// one RD task (candidate c of plane p with tx type t) of a tx block up to 16x16 by the calling group: the residual
+// res is ready; the levels go to lev
 template <int W, int H>
-__device__ EvalOut grp_task(lds<RcShared> &S, lds<RcGroup> &X, const Img &I, const DevTables &T, int p, int c, int t,
-                            const lds<uint16_t> *src, int sp, const SearchParams &P) {
-    constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : 4, LH = H == 4 ? 2 : H == 8 ? 3 : 4;
-    grp_residual(S, I.edgeFilter ? X.ea : nullptr, X.el, p, c, LW, LH, src, sp, X.ev.res, I.bd);
+__device__ EvalOut grp_task(lds<RcShared> &S, lds<RcGroup> &X, const Img &I, int p, int c, int t, const lds<int16_t> *res,
+                            lds<int16_t> *lev, const SearchParams &P) {
     RcEvalIn e;
     e.txSz = tx_of(W, H); e.txType = t; e.plane = p;
     e.dcQ = dc_q(I.qindex, I.bd); e.acQ = ac_q(I.qindex, I.bd);
@@ -178,7 +193,7 @@ __device__ EvalOut grp_task(lds<RcShared> &S, lds<RcGroup> &X, const Img &I, con
     e.rnd = p ? P.rndC : P.rnd; e.zbin = p ? P.zbinC : P.zbin;
     e.resEnergy = 0.f;
     e.C = &S.cost;
-    return grp_eval<W, H>(X.ev, e, S.tab);
+    return grp_eval<W, H>(res, X.tmp, lev, e, S.tab);
 }
 
 // 2-D forward DCT_DCT of a W x H residual with a side of 32 or 64 by the 32 lanes of the group, the residual produced
@@ -440,18 +455,26 @@ __device__ __forceinline__ const lds<uint16_t> *stage_src(const lds<RcShared> &S
 
 // run S.taskC/T/P[0 .. nTask) on the 8 groups (the source block staged in S.srcb, row stride W); per task
 // J = fmaf(lambda, R + extra, D) with extra = luma mode bits (plane 0) or 0; S.taskJ / taskE.  Then S.win = the first
-// minimum over the tasks.
+// minimum over the tasks.  Blocks up to 16x16: luma tasks read the residual of their candidate (S.taskK), made by
+// wg_luma_residuals; chroma tasks make their own in slot t; levels of task t go to S.u.s.lev[t].  Larger blocks:
+// team g keeps the levels and prepared edges of its last task.
 template <int W, int H>
 __device__ void wg_run_tasks(lds<RcShared> &S, const Img &I, const DevTables &T, const SearchParams &P, float lambda, bool lumaBits,
                              bool pickWinner) {
     constexpr bool big = W > 16 || H > 16;
+    constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
     const int g = gid(), lane = glane();
     const lds<CostTab> &C = S.cost;
     for (int t = g; t < S.nTask; t += RC_NG) {
         const int c = S.taskC[t], tt = S.taskT[t], p = S.taskP[t];
         EvalOut o;
-        if constexpr (big) o = grp_task_big<W, H>(S, S.u.t[g], I, T, p, c, stage_src(S, p), W, P);
-        else o = grp_task<W, H>(S, S.u.g[g], I, T, p, c, tt, stage_src(S, p), W, P);
+        if constexpr (big) {
+            o = grp_task_big<W, H>(S, S.u.t[g], I, T, p, c, stage_src(S, p), W, P);
+        } else {
+            if (!lumaBits)   // chroma: this task's own residual
+                grp_residual(S, I.edgeFilter ? S.u.s.g[g].ea : nullptr, S.u.s.g[g].el, p, c, LW, LH, stage_src(S, p), W, S.u.s.res[t], I.bd);
+            o = grp_task<W, H>(S, S.u.s.g[g], I, p, c, tt, S.u.s.res[lumaBits ? S.taskK[t] : t], S.u.s.lev[t], P);
+        }
         float J;
         if (lumaBits) {
             const int m = cand_mode(c);
@@ -476,25 +499,47 @@ __device__ void wg_run_tasks(lds<RcShared> &S, const Img &I, const DevTables &T,
     __syncthreads();
 }
 
-// re-evaluate task t on group 0 so its levels and prepared edges are in that group's scratch; returns the levels
+// luma blocks up to 16x16: the residual of each candidate S.candC[k] into S.u.s.res[k] (group k)
 template <int W, int H>
-__device__ lds<int16_t> *wg_final_task(lds<RcShared> &S, const Img &I, const DevTables &T, const SearchParams &P, int t) {
-    constexpr bool big = W > 16 || H > 16;
-    const int p = S.taskP[t];
-    if (gid() == 0) {
-        if constexpr (big) grp_task_big<W, H>(S, S.u.t[0], I, T, p, S.taskC[t], stage_src(S, p), W, P);
-        else grp_task<W, H>(S, S.u.g[0], I, T, p, S.taskC[t], S.taskT[t], stage_src(S, p), W, P);
-    }
+__device__ void wg_luma_residuals(lds<RcShared> &S, const Img &I) {
+    constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : 4, LH = H == 4 ? 2 : H == 8 ? 3 : 4;
+    const int g = gid();
+    if (g < S.nCand)
+        grp_residual(S, I.edgeFilter ? S.u.s.g[g].ea : nullptr, S.u.s.g[g].el, 0, S.candC[g], LW, LH, stage_src(S, 0), W, S.u.s.res[g], I.bd);
     __syncthreads();
-    if constexpr (big) return S.u.t[0].q;
-    else return S.u.g[0].ev.q;
 }
 
-// the edge buffers group 0's final evaluation prepared (directional modes with the edge filter on)
+// the levels of the winning task t, and (larger blocks) the team holding them with its prepared edges: its last task
+// is t, or it evaluates t again
 template <int W, int H>
-__device__ __forceinline__ const lds<uint16_t> *final_ea(const lds<RcShared> &S) { return (W > 16 || H > 16) ? S.u.t[0].ea : S.u.g[0].ea; }
-template <int W, int H>
-__device__ __forceinline__ const lds<uint16_t> *final_el(const lds<RcShared> &S) { return (W > 16 || H > 16) ? S.u.t[0].el : S.u.g[0].el; }
+__device__ lds<int16_t> *wg_final_levels(lds<RcShared> &S, const Img &I, const DevTables &T, const SearchParams &P, int t, int &team) {
+    team = t % RC_NG;
+    if constexpr (W > 16 || H > 16) {
+        if (t + RC_NG < S.nTask) {   // overwritten by a later task of the same team
+            const int p = S.taskP[t];
+            if (gid() == team) grp_task_big<W, H>(S, S.u.t[team], I, T, p, S.taskC[t], stage_src(S, p), W, P);
+            __syncthreads();
+        }
+        return S.u.t[team].q;
+    } else {
+        return S.u.s.lev[t];
+    }
+}
+
+// write the prediction of task t of plane p (into dst): larger blocks predict again from team g's prepared edges;
+// blocks up to 16x16 take source minus the kept residual (slot k)
+template <int W, int H, typename Px>
+__device__ void wg_final_pred(const lds<RcShared> &S, const Img &I, int p, int t, int k, int team, Px *dst, int pitch) {
+    constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
+    if constexpr (W > 16 || H > 16) {
+        wg_write_pred<Px>(S, I.edgeFilter ? S.u.t[team].ea : nullptr, S.u.t[team].el, team, p, S.taskC[t], LW, LH, dst, pitch, I.bd);
+    } else {
+        const lds<uint16_t> *src = stage_src(S, p);
+        const lds<int16_t> *res = S.u.s.res[k];
+        for (int q = threadIdx.x; q < W * H; q += RC_NT) dst[(size_t)(q >> LW) * pitch + (q & (W - 1))] = (Px)((int)src[q] - res[q]);
+        __syncthreads();
+    }
+}
 
 template <int W, int H, typename Px>
 __device__ void rc_luma(lds<RcShared> &S, const Batch &B, const SearchParams &P, uint32_t sb, const Img &I, int bsize, int x, int y, int availU,
@@ -518,36 +563,42 @@ __device__ void rc_luma(lds<RcShared> &S, const Batch &B, const SearchParams &P,
         const int nc = max(1, min((int)br.ncand, P.Krecheck));
         const int set = tx_set_intra(txSz);
         int n = 0;
+        for (int k = 0; k < nc; k++) S.candC[k] = br.cand[k];
+        S.nCand = nc;
         if (P.txAllCL && set != TX_SET_DCTONLY) {
             for (int k = 0; k < nc; k++)
                 for (int t = 0; t < 16; t++)
-                    if (tx_in_set(set, t)) { S.taskC[n] = br.cand[k]; S.taskT[n] = t; S.taskP[n] = 0; n++; }
+                    if (tx_in_set(set, t)) { S.taskC[n] = br.cand[k]; S.taskT[n] = t; S.taskP[n] = 0; S.taskK[n] = k; n++; }
         } else {
-            for (int k = 0; k < nc; k++) { S.taskC[n] = br.cand[k]; S.taskT[n] = DCT_DCT; S.taskP[n] = 0; n++; }
-            if (br.tx != DCT_DCT) { S.taskC[n] = br.cand[0]; S.taskT[n] = br.tx; S.taskP[n] = 0; n++; }
+            for (int k = 0; k < nc; k++) { S.taskC[n] = br.cand[k]; S.taskT[n] = DCT_DCT; S.taskP[n] = 0; S.taskK[n] = k; n++; }
+            if (br.tx != DCT_DCT) { S.taskC[n] = br.cand[0]; S.taskT[n] = br.tx; S.taskP[n] = 0; S.taskK[n] = 0; n++; }
         }
         S.nTask = n;
         S.bsize = bsize;
     }
     __syncthreads();
+    if constexpr (W <= 16 && H <= 16) wg_luma_residuals<W, H>(S, I);
     wg_run_tasks<W, H>(S, I, T, P, lambda, true, true);
     if (!(P.txAllCL && tx_set_intra(txSz) != TX_SET_DCTONLY) && P.txSearchCL && tx_set_intra(txSz) != TX_SET_DCTONLY) {
         // tx-type pass for the pass-1 winner (txAllCL off): the other types of the set
         if (tid == 0) {
-            const int c0 = S.taskC[S.win], n0 = S.nTask;
+            const int c0 = S.taskC[S.win], k0 = S.taskK[S.win], n0 = S.nTask;
             const BlockRes &br = B.blk[I.shOff[bsize] + (uint32_t)(y / (4 * bh4(bsize))) * I.shNbx[bsize] + x / (4 * bw4(bsize))];
             int n = n0;
             for (int t = 1; t < 16; t++)
-                if (tx_in_set(tx_set_intra(txSz), t) && !(c0 == br.cand[0] && t == br.tx)) { S.taskC[n] = c0; S.taskT[n] = t; S.taskP[n] = 0; n++; }
+                if (tx_in_set(tx_set_intra(txSz), t) && !(c0 == br.cand[0] && t == br.tx)) {
+                    S.taskC[n] = c0; S.taskT[n] = t; S.taskP[n] = 0; S.taskK[n] = k0; n++;
+                }
             S.nTask = n;
         }
         __syncthreads();
         wg_run_tasks<W, H>(S, I, T, P, lambda, true, true);
     }
     const int win = S.win;
-    lds<int16_t> *q = wg_final_task<W, H>(S, I, T, P, win);
+    int team;
+    lds<int16_t> *q = wg_final_levels<W, H>(S, I, T, P, win, team);
     Px *dst = rec_plane<Px>(I, 0) + (size_t)y * I.pitch[0] + x;
-    wg_write_pred<Px>(S, I.edgeFilter ? final_ea<W, H>(S) : nullptr, final_el<W, H>(S), 0, 0, S.taskC[win], LW, LH, dst, I.pitch[0], I.bd);
+    wg_final_pred<W, H, Px>(S, I, 0, win, S.taskK[win], team, dst, I.pitch[0]);
     if (S.taskE[win] > 0) {
         const int fixes = wg_inverse_add<Px>(S, q, txSz, S.taskT[win], I.bd, dc_q(I.qindex, I.bd), ac_q(I.qindex, I.bd), dst, I.pitch[0]);
         if (fixes && tid == 0) atomicAdd(&B.stats[3], 1u);
@@ -630,9 +681,12 @@ __device__ void rc_chroma(lds<RcShared> &S, const Batch &B, const SearchParams &
             nV += l * ((int)srcV[i * sp + j] - S.dcv[2]);
             dd += l * l;
         }
-        atomicAdd(gen(&S.acc64[0]), (unsigned long long)nU);
-        atomicAdd(gen(&S.acc64[1]), (unsigned long long)nV);
-        atomicAdd(gen(&S.acc64[2]), (unsigned long long)dd);
+        nU = gsuml(nU); nV = gsuml(nV); dd = gsuml(dd);
+        if (glane() == 0) {
+            atomicAdd(gen(&S.acc64[0]), (unsigned long long)nU);
+            atomicAdd(gen(&S.acc64[1]), (unsigned long long)nV);
+            atomicAdd(gen(&S.acc64[2]), (unsigned long long)dd);
+        }
         __syncthreads();
         if (tid == 0) {
             const long long d = (long long)S.acc64[2];
@@ -647,8 +701,10 @@ __device__ void rc_chroma(lds<RcShared> &S, const Batch &B, const SearchParams &
         long long sse[6] = {0, 0, 0, 0, 0, 0};
         for (int q = tid; q < W * H; q += RC_NT) {
             const int i = q >> LW, j = q & (W - 1), l = S.cflL[q] - S.cflAvg;
+#pragma unroll
             for (int pl = 0; pl < 2; pl++) {
                 const int s = pl == 0 ? srcU[i * sp + j] : srcV[i * sp + j];
+#pragma unroll
                 for (int k = 0; k < 3; k++) {
                     const int a = max(-16, min(16, S.cflAlpha[pl] + k - 1));
                     const int d = s - clip3(0, (1 << bd) - 1, S.dcv[pl + 1] + round2signed(a * l, 6));
@@ -656,7 +712,11 @@ __device__ void rc_chroma(lds<RcShared> &S, const Batch &B, const SearchParams &
                 }
             }
         }
-        for (int k = 0; k < 6; k++) atomicAdd(gen(&S.acc64[2 + k]), (unsigned long long)sse[k]);
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            const long long v = gsuml(sse[k]);
+            if (glane() == 0) atomicAdd(gen(&S.acc64[2 + k]), (unsigned long long)v);
+        }
         __syncthreads();
         if (tid == 0)
             for (int pl = 0; pl < 2; pl++) {
@@ -673,22 +733,33 @@ __device__ void rc_chroma(lds<RcShared> &S, const Batch &B, const SearchParams &
     if (tid < 13) S.uvc[tid] = cand_pack(tid, 0);
     __syncthreads();
     constexpr int NSUB = (W / 4) * (H / 4), NSX = W / 4;
+    // screening: item = (candidate, plane, 4x4); a candidate's 2 NSUB items are consecutive, so each segment of
+    // min(2 NSUB, 32) lanes sums one candidate's Hadamard costs with shuffles before one LDS atomic
     auto screen = [&](int c0, int c1) {
+        constexpr int SEG = 2 * NSUB < GW ? 2 * NSUB : GW;
         const int items = (c1 - c0) * NSUB * 2;
-        for (int it = tid; it < items; it += RC_NT) {
-            const int ci = c0 + it / (2 * NSUB), rr = it % (2 * NSUB), p = 1 + rr / NSUB, sbk = rr % NSUB;
-            const int sx = (sbk % NSX) * 4, sy = (sbk / NSX) * 4, c = S.uvc[ci], m = cand_mode(c), pa = pangle(m, cand_delta(c));
-            const lds<uint16_t> *s = p == 1 ? srcU : srcV;
-            const lds<uint16_t> *A = S.ea[p] + EDGE_OFF, *L = S.el[p] + EDGE_OFF;
-            int d[16];
+        for (int base = 0; base < items; base += RC_NT) {
+            const int it = base + tid;
+            int v = 0, ci = c0;
+            if (it < items) {
+                ci = c0 + it / (2 * NSUB);
+                const int rr = it % (2 * NSUB), p = 1 + rr / NSUB, sbk = rr % NSUB;
+                const int sx = (sbk % NSX) * 4, sy = (sbk / NSX) * 4, c = S.uvc[ci], m = cand_mode(c), pa = pangle(m, cand_delta(c));
+                const lds<uint16_t> *s = p == 1 ? srcU : srcV;
+                const lds<uint16_t> *A = S.ea[p] + EDGE_OFF, *L = S.el[p] + EDGE_OFF;
+                int d[16];
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+                for (int i = 0; i < 4; i++)
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int pr = m == DC_PRED ? S.dcv[p] : pred_px(m, pa, sy + i, sx + j, LW, LH, A, L, 0, 0);
-                    d[i * 4 + j] = (int)s[(sy + i) * sp + sx + j] - pr;
-                }
-            atomicAdd(gen(&S.satdI[ci]), hadamard4(d));
+                    for (int j = 0; j < 4; j++) {
+                        const int pr = m == DC_PRED ? S.dcv[p] : pred_px(m, pa, sy + i, sx + j, LW, LH, A, L, 0, 0);
+                        d[i * 4 + j] = (int)s[(sy + i) * sp + sx + j] - pr;
+                    }
+                v = hadamard4(d);
+            }
+#pragma unroll
+            for (int o = SEG / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+            if (it < items && (it & (SEG - 1)) == 0) atomicAdd(gen(&S.satdI[ci]), v);
         }
         __syncthreads();
     };
@@ -752,9 +823,10 @@ __device__ void rc_chroma(lds<RcShared> &S, const Batch &B, const SearchParams &
     const int win = S.win, uvTxType = S.taskT[win];
     for (int p = 1; p <= 2; p++) {
         const int t = win + p - 1;
-        lds<int16_t> *q = wg_final_task<W, H>(S, I, T, P, t);
+        int team;
+        lds<int16_t> *q = wg_final_levels<W, H>(S, I, T, P, t, team);
         Px *dst = rec_plane<Px>(I, p) + (size_t)cy * I.pitch[p] + cx;
-        wg_write_pred<Px>(S, I.edgeFilter ? final_ea<W, H>(S) : nullptr, final_el<W, H>(S), 0, p, S.taskC[t], LW, LH, dst, I.pitch[p], bd);
+        wg_final_pred<W, H, Px>(S, I, p, t, t, team, dst, I.pitch[p]);
         if (S.taskE[t] > 0) {
             const int fixes = wg_inverse_add<Px>(S, q, txSz, uvTxType, bd, dc_q(I.qindex, bd), ac_q(I.qindex, bd), dst, I.pitch[p]);
             if (fixes && tid == 0) atomicAdd(&B.stats[3], 1u);
@@ -780,6 +852,13 @@ __device__ void stage_tables(lds<RcShared> &S, const DevTables &T, int qctx, boo
         if (tid < 21) S.tab.fwdOff[tid / 7][tid % 7] = (tid / 7) * 336 + ((tid % 7) == 3 ? 16 : (tid % 7) == 4 ? 80 : 0);
         if (tid < 19) S.tab.txScale[tid] = T.txScale[tid];
         for (int k = tid; k < 19 * 16; k += RC_NT) S.tab.txg[k / 16][k % 16] = T.txg[k / 16][k % 16];
+        for (int t = 0; t < 19; t++) {
+            if (av1t::Tx_Width[t] == 64 || av1t::Tx_Height[t] == 64) continue;
+            const uint16_t *src = scan_of(t, DCT_DCT);
+            const int n = av1t::Tx_Width[t] * av1t::Tx_Height[t], off = dscan_off(t);
+            for (int k = tid; k < n; k += RC_NT) S.tab.dscan[off + k] = src[k];
+        }
+        for (int k = tid; k < 19 * 25; k += RC_NT) S.tab.cbOff[k / 25][k % 25] = (&av1t::Coeff_Base_Ctx_Offset[0][0][0])[k];
     }
 }
 
@@ -875,9 +954,13 @@ __device__ void recon_sb(lds<RcShared> &S, const Batch &B, const SearchParams &P
     }
 }
 
+// One workgroup per compute unit (the LDS footprint allows no more), so the compiler is told to spend registers on
+// latency (batched LDS loads) instead of keeping them free for occupancy it cannot have.
+#define AV1M_RC_KERNEL __global__ void __launch_bounds__(RC_NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
+
 // one launch per wavefront step: workgroup b takes superblock sbList[b]
 template <typename Px>
-__global__ void __launch_bounds__(RC_NT) k_recon(Batch B, SearchParams P, const uint32_t *sbList) {
+AV1M_RC_KERNEL k_recon(Batch B, SearchParams P, const uint32_t *sbList) {
     extern __shared__ __align__(16) char smraw[];
     lds<RcShared> &S = *(lds<RcShared> *)(lds<char> *)smraw;
     const uint32_t sb = sbList[blockIdx.x];
@@ -907,7 +990,7 @@ __device__ bool wait_done(const uint32_t *f, const RcQueue &Q) {
 // running workgroup, so the waits always end; a wait that exceeds the time limit raises the abort flag, which makes
 // every workgroup stop taking items and return.
 template <typename Px>
-__global__ void __launch_bounds__(RC_NT) k_recon_queue(Batch B, SearchParams P, RcQueue Q) {
+AV1M_RC_KERNEL k_recon_queue(Batch B, SearchParams P, RcQueue Q) {
     extern __shared__ __align__(16) char smraw[];
     lds<RcShared> &S = *(lds<RcShared> *)(lds<char> *)smraw;
     const int tid = threadIdx.x;

@@ -34,6 +34,7 @@ def main():
     ap.add_argument("--qindex", type=int, default=120)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--schedule", type=int, default=0, help="closed loop: 0 persistent queue, 1 one launch per wavefront step")
+    ap.add_argument("--batch", type=int, default=256, help="images of the mixed-size batch leg (0 = skip it)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only (the path shards by image / tile with no collective); "
@@ -97,11 +98,43 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "per_launch_ms": round(per_launch_ms, 4), "alg_bytes_per_launch": round(alg_bytes / launches)},
     }
+    frame.close()
+    if a.batch > 0:
+        line["batched"] = batch_leg(A, a.batch, a.schedule, value)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(planes, W, H, a.qindex)
     print(json.dumps(line), flush=True)
-    frame.close()
     enc.close()
+
+
+def batch_leg(A, n, schedule, plain_value):
+    """The mixed-size batch (A.batch_spec: n images, 0.18-24 MP, qindex 80/120/160) in one encode call, inputs
+    resident in device memory; one warm-up call on the first 8 images, then one timed call on all n."""
+    from concurrent.futures import ThreadPoolExecutor
+    spec = A.batch_spec(n)
+    with ThreadPoolExecutor(16) as ex:
+        planes = list(ex.map(lambda s: A.synth_image(s[3], s[0], s[1]), spec))
+    frames = [A.DeviceFrame(p, W, H, qindex=q) for p, (W, H, q, _) in zip(planes, spec)]
+    del planes
+    enc = A.GpEncoder().init(A.GpConfig(qindex=120, schedule=schedule))
+    enc.encode(frames[:8], want_recon=True)
+    A.lib().av1mi_device_sync()
+    t0 = time.perf_counter()
+    out, t = enc.encode(frames, want_recon=True)
+    A.lib().av1mi_device_sync()
+    dt = time.perf_counter() - t0
+    mp = sum(W * H for W, H, _, _ in spec) / 1e6
+    res = {"workload": "%d synthetic 8-bit 4:2:0 images, 0.18-24 MP log-uniform, aspect 0.5-2, qindex 80/120/160, "
+                       "one encode call (seeds 1000..%d)" % (n, 1000 + n - 1),
+           "images": n, "megapixels": round(mp, 3), "value": round(mp / dt, 4), "unit": "MP/s", "seconds": round(dt, 3),
+           "vs_plain_single_image": round(mp / dt / plain_value, 3),
+           "stages_ms": {"analysis": round(t.analysis, 3), "dp": round(t.dp, 3), "recon": round(t.recon, 3),
+                         "pack": round(t.pack, 3), "download": round(t.download, 3)},
+           "sbs": t.sbs, "blocks": t.blocks, "device_bytes": t.deviceBytes}
+    for f in frames:
+        f.close()
+    enc.close()
+    return res
 
 
 def cpu_baseline(planes, W, H, qindex):
