@@ -76,6 +76,18 @@ def lib():
         L.av1mi_dev_alloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
         L.av1mi_dev_free.argtypes = [C.c_void_p]
         L.av1mi_copy_h2d.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.av1mi_dev_alloc_async.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_void_p]
+        L.av1mi_dev_free_async.argtypes = [C.c_void_p, C.c_void_p]
+        L.av1mi_copy_h2d_async.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
+        L.av1mi_stream_create.argtypes = [C.POINTER(C.c_void_p)]
+        L.av1mi_stream_destroy.argtypes = [C.c_void_p]
+        L.av1mi_stream_sync.argtypes = [C.c_void_p]
+        L.av1mi_frame_bytes.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.av1mi_frame_bytes.restype = C.c_size_t
+        L.av1mi_compute_units.argtypes = []
+        for fn in (L.av1mi_write_obus, L.av1mi_write_avif):
+            fn.argtypes = [C.POINTER(_FrameOut), C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t), C.c_char_p, C.c_int]
+        L.av1mi_buffer_free.argtypes = [C.c_void_p]
         L.av1mi_synth_image.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.POINTER(C.c_int)]
         L.av1mi_fwd_matrix.argtypes = [C.c_int, C.c_int, C.c_void_p]
@@ -212,6 +224,36 @@ def synth_image(seed, W, H, ssx=1, bd=8):
     return [y, u, v]
 
 
+def write_bitstream(rec, edge_filter=1, avif=True, monochrome=0):
+    """The AV1 bitstream of a record (GpFrameOut, or a dict with width, height, base_q_idx, bit_depth, blocks,
+    block_txb, txb, coeffs, tile_cols, tile_rows): an AVIF file (avif=True) or the bare OBUs.  Host only."""
+    g = (lambda k: rec[k]) if isinstance(rec, dict) else (lambda k: getattr(rec, k))
+    blocks = np.ascontiguousarray(g("blocks"))
+    bt = np.ascontiguousarray(g("block_txb"), dtype=np.uint32)
+    txb = np.ascontiguousarray(g("txb"))
+    co = np.ascontiguousarray(g("coeffs"), dtype=np.int16)
+    o = _FrameOut()
+    o.width, o.height, o.base_q_idx, o.bit_depth = g("width"), g("height"), g("base_q_idx"), g("bit_depth")
+    o.chroma_format, o.monochrome = 0, monochrome
+    o.n_blocks, o.n_txb, o.n_coeffs = len(blocks), len(txb), len(co)
+    o.blocks, o.block_txb, o.txb, o.coeffs = blocks.ctypes.data, bt.ctypes.data, txb.ctypes.data, co.ctypes.data
+    o.tile_cols, o.tile_rows = g("tile_cols"), g("tile_rows")
+    data, size = C.c_void_p(), C.c_size_t()
+    err = C.create_string_buffer(512)
+    fn = lib().av1mi_write_avif if avif else lib().av1mi_write_obus
+    if fn(C.byref(o), edge_filter, C.byref(data), C.byref(size), err, 512) != 0:
+        raise ValueError("write_bitstream: " + err.value.decode())
+    try:
+        return C.string_at(data, size.value)
+    finally:
+        lib().av1mi_buffer_free(data)
+
+
+def device_compute_units():
+    """Compute units of the current HIP device (0 without one)."""
+    return int(lib().av1mi_compute_units())
+
+
 def batch_spec(n=256, seed=1000):
     """The mixed-size batch workload: n images with sizes log-uniform over 0.18-24 MP, aspect ratio (W / H) uniform over
     [0.5, 2], sides rounded to multiples of 8, qindex drawn from {80, 120, 160}; image i is the synthetic image of seed
@@ -231,19 +273,23 @@ def batch_spec(n=256, seed=1000):
 class DeviceFrame:
     """Padded planes of one frame resident in device memory (owned; freed on close / garbage collection)."""
 
-    def __init__(self, planes, W, H, bd=8, qindex=0):
+    def __init__(self, planes, W, H, bd=8, qindex=0, stream=None):
+        """stream: upload with stream-ordered allocation and copies on that stream (freed on it too)."""
         padded, aw, ah = pad_planes(planes, W, H)
         self.W, self.H, self.aw, self.ah, self.bd, self.qindex = W, H, aw, ah, bd, qindex
-        self.ptrs, self.pitch = [], []
+        self.ptrs, self.pitch, self.stream = [], [], stream
+        self._host = padded if stream else None   # pageable sources must outlive the asynchronous copies
         for a in padded:
             p = C.c_void_p()
-            if lib().av1mi_dev_alloc(C.byref(p), a.nbytes) != 0:
+            ok = (lib().av1mi_dev_alloc_async(C.byref(p), a.nbytes, stream) if stream else lib().av1mi_dev_alloc(C.byref(p), a.nbytes)) == 0
+            if not ok:
                 self.close()
                 raise MemoryError("device allocation of %d bytes failed" % a.nbytes)
-            if lib().av1mi_copy_h2d(p, a.ctypes.data, a.nbytes) != 0:
+            self.ptrs.append(p.value)
+            ok = (lib().av1mi_copy_h2d_async(p, a.ctypes.data, a.nbytes, stream) if stream else lib().av1mi_copy_h2d(p, a.ctypes.data, a.nbytes)) == 0
+            if not ok:
                 self.close()
                 raise RuntimeError("host-to-device copy failed")
-            self.ptrs.append(p.value)
             self.pitch.append(a.shape[1])
 
     def frame_in(self):
@@ -252,8 +298,12 @@ class DeviceFrame:
 
     def close(self):
         for p in self.ptrs:
-            lib().av1mi_dev_free(p)
+            if self.stream:
+                lib().av1mi_dev_free_async(p, self.stream)
+            else:
+                lib().av1mi_dev_free(p)
         self.ptrs = []
+        self._host = None
 
     def __del__(self):
         try:

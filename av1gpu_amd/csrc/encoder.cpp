@@ -454,6 +454,47 @@ void av1mi_frame_out_free(av1mi_frame_out *o) {
 }
 
 int av1mi_dev_alloc(void **ptr, size_t bytes) { return hipMalloc(ptr, bytes) == hipSuccess ? 0 : -1; }
+int av1mi_dev_alloc_async(void **ptr, size_t bytes, void *stream) {
+    return hipMallocAsync(ptr, bytes, (hipStream_t)stream) == hipSuccess ? 0 : -1;
+}
+int av1mi_dev_free_async(void *ptr, void *stream) { return hipFreeAsync(ptr, (hipStream_t)stream) == hipSuccess ? 0 : -1; }
+int av1mi_copy_h2d_async(void *dst, const void *src, size_t bytes, void *stream) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream) == hipSuccess ? 0 : -1;
+}
+int av1mi_stream_create(void **stream) {
+    hipStream_t s;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return -1;
+    *stream = s;
+    return 0;
+}
+int av1mi_stream_destroy(void *stream) { return hipStreamDestroy((hipStream_t)stream) == hipSuccess ? 0 : -1; }
+int av1mi_stream_sync(void *stream) { return hipStreamSynchronize((hipStream_t)stream) == hipSuccess ? 0 : -1; }
+
+int av1mi_compute_units(void) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return v;
+}
+
+size_t av1mi_frame_bytes(int W, int H, int bitDepth) {
+    if (W <= 0 || H <= 0) return 0;
+    const size_t bps = bitDepth == 10 ? 2 : 1;
+    const size_t MiCols = 2 * (size_t)((W + 7) >> 3), MiRows = 2 * (size_t)((H + 7) >> 3);
+    const size_t sbCols = (MiCols + 15) >> 4, sbRows = (MiRows + 15) >> 4, nSb = sbCols * sbRows;
+    const size_t aw = sbCols * 64, ah = sbRows * 64, mi = MiCols * MiRows, c1 = mi / 4;
+    size_t blocks = 0;
+    for (int s = 0; s < BLOCK_INVALID; s++) {
+        if (s >= 13 && s <= 15) continue;
+        const size_t w = 4 * bw4(s), h = 4 * bh4(s);
+        blocks += ((MiCols * 4 + w - 1) / w) * ((MiRows * 4 + h - 1) / h);
+    }
+    const size_t planes = aw * ah * bps * 3 / 2;
+    const size_t arena = planes + 5 * mi + 2 * (mi + 2 * c1) + blocks * sizeof(BlockRes) +
+                         nSb * (4 + 2 + MAX_LEAVES * sizeof(Leaf) + 4 + sizeof(SbOut) + MAX_LEAVES * (sizeof(RecBlock) + 2) +
+                                SB_TXB_CAP * sizeof(RecTxb) + SB_COEF_CAP * 2 + 4 + sizeof(PackOff) + 4);
+    const size_t records = nSb * (MAX_LEAVES * (sizeof(RecBlock) + 4) + SB_TXB_CAP * sizeof(RecTxb) + SB_COEF_CAP * 2);
+    return planes + arena + records + 64 * 1024;
+}
 int av1mi_dev_free(void *ptr) { return hipFree(ptr) == hipSuccess ? 0 : -1; }
 int av1mi_copy_h2d(void *dst, const void *src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : -1; }
 int av1mi_device_sync(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : -1; }

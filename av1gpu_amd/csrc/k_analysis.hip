@@ -96,20 +96,31 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) k_analysis(Batch B, Sear
         S.satdI[lane] = 0;
         S.cand[lane] = lane < 13 ? cand_pack(lane, 0) : 0;
         gsync();
+        // item = (candidate, 4x4); a candidate's NSUB items are consecutive, so each segment of min(NSUB, 32) lanes
+        // sums its Hadamard costs with shuffles before one LDS atomic
         auto screen = [&](int c0, int c1) {
+            constexpr int SEG = NSUB < GW ? NSUB : GW;
             const int items = (c1 - c0) * NSUB;
-            for (int it = lane; it < items; it += GW) {
-                const int ci = c0 + it / NSUB, sb = it % NSUB, sx = (sb % NSX) * 4, sy = (sb / NSX) * 4;
-                const int c = S.cand[ci], m = cand_mode(c), pa = pangle(m, cand_delta(c));
-                int d[16];
+            for (int base = 0; base < items; base += GW) {
+                const int it = base + lane;
+                int v = 0, ci = c0;
+                if (it < items) {
+                    ci = c0 + it / NSUB;
+                    const int sb = it % NSUB, sx = (sb % NSX) * 4, sy = (sb / NSX) * 4;
+                    const int c = S.cand[ci], m = cand_mode(c), pa = pangle(m, cand_delta(c));
+                    int d[16];
 #pragma unroll
-                for (int i = 0; i < 4; i++)
+                    for (int i = 0; i < 4; i++)
 #pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const int p = m == DC_PRED ? dcv : pred_px(m, pa, sy + i, sx + j, LW, LH, A, L, 0, 0);
-                        d[i * 4 + j] = (int)src[(sy + i) * sp + sx + j] - p;
-                    }
-                atomicAdd(&S.satdI[ci], hadamard4(d));
+                        for (int j = 0; j < 4; j++) {
+                            const int p = m == DC_PRED ? dcv : pred_px(m, pa, sy + i, sx + j, LW, LH, A, L, 0, 0);
+                            d[i * 4 + j] = (int)src[(sy + i) * sp + sx + j] - p;
+                        }
+                    v = hadamard4(d);
+                }
+#pragma unroll
+                for (int o = SEG / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+                if (it < items && (it & (SEG - 1)) == 0) atomicAdd(&S.satdI[ci], v);
             }
             gsync();
         };

@@ -35,6 +35,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--schedule", type=int, default=0, help="closed loop: 0 persistent queue, 1 one launch per wavefront step")
     ap.add_argument("--batch", type=int, default=256, help="images of the mixed-size batch leg (0 = skip it)")
+    ap.add_argument("--serve", type=int, default=64, help="requests of the serving leg (0 = skip it)")
+    ap.add_argument("--rate", type=float, default=50.0, help="serving leg: Poisson arrival rate, requests per second")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only (the path shards by image / tile with no collective); "
@@ -101,6 +103,8 @@ def main():
     frame.close()
     if a.batch > 0:
         line["batched"] = batch_leg(A, a.batch, a.schedule, value)
+    if a.serve > 0:
+        line["serving"] = serving_leg(A, a.serve, a.rate)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(planes, W, H, a.qindex)
     print(json.dumps(line), flush=True)
@@ -135,6 +139,29 @@ def batch_leg(A, n, schedule, plain_value):
         f.close()
     enc.close()
     return res
+
+
+def serving_leg(A, n, rate):
+    """n single-image requests (serve.serving_spec: 0.2-24 MP, 1 % at 24 MP, qindex 120) arriving as a Poisson
+    process at `rate` per second into the continuous-batching server; latency = submission to result on the host."""
+    from concurrent.futures import ThreadPoolExecutor
+    import numpy as np
+    from av1gpu_amd import serve
+    spec = serve.serving_spec(n)
+    with ThreadPoolExecutor(16) as ex:
+        planes = list(ex.map(lambda s: A.synth_image(s[3], s[0], s[1]), spec))
+    images = [(p, W, H, q) for p, (W, H, q, _) in zip(planes, spec)]
+    srv = serve.Server(workers=4, hbm_budget=96 << 30, max_batch=16)
+    serve.poisson_run(srv, images[:4], rate)          # warm-up
+    nb0 = len(srv.batches)
+    reqs, wall = serve.poisson_run(srv, images, rate)
+    srv.close()
+    lat = np.array([r.latency for r in reqs]) * 1e3
+    mp = sum(W * H for W, H, _, _ in spec) / 1e6
+    return {"workload": "%d requests, 8-bit 4:2:0, 0.2-24 MP log-uniform with 1%% at 24 MP, qindex 120, Poisson %.0f/s"
+                        % (n, rate), "requests": n, "megapixels": round(mp, 3), "value": round(mp / wall, 4), "unit": "MP/s",
+            "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+            "workers": 4, "batches": len(srv.batches) - nb0}
 
 
 def cpu_baseline(planes, W, H, qindex):

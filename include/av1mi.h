@@ -112,11 +112,32 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
                  int want_recon, char *err, int errlen);
 void av1mi_frame_out_free(av1mi_frame_out *out);
 
+/* The bitstream of a record (the reference's host writer, av1core/src/encoder.cpp:1072-1659 with avif.cpp):
+ * av1mi_write_obus gives the sequence header and frame OBUs of the still picture (default CDFs, no post-filters),
+ * av1mi_write_avif the same wrapped in an AVIF file.  edge_filter: enable_intra_edge_filter of the encode.  *data is
+ * malloc'd (release with av1mi_buffer_free).  0, or -1 with a message when the record is not codable. */
+int av1mi_write_obus(const av1mi_frame_out *rec, int edge_filter, void **data, size_t *size, char *err, int errlen);
+int av1mi_write_avif(const av1mi_frame_out *rec, int edge_filter, void **data, size_t *size, char *err, int errlen);
+void av1mi_buffer_free(void *p);
+
 /* Device-buffer helpers for callers without their own allocator (hipMalloc / hipFree / hipMemcpy). */
 int av1mi_dev_alloc(void **ptr, size_t bytes);
 int av1mi_dev_free(void *ptr);
 int av1mi_copy_h2d(void *dst, const void *src, size_t bytes);
 int av1mi_device_sync(void);
+/* Stream-ordered variants and non-blocking streams, for callers that overlap several encodes on one device (the
+ * serving loop): allocation / free from the device's stream-ordered pool, asynchronous host-to-device copy. */
+int av1mi_dev_alloc_async(void **ptr, size_t bytes, void *stream);
+int av1mi_dev_free_async(void *ptr, void *stream);
+int av1mi_copy_h2d_async(void *dst, const void *src, size_t bytes, void *stream);
+int av1mi_stream_create(void **stream);
+int av1mi_stream_destroy(void *stream);
+int av1mi_stream_sync(void *stream);
+/* Upper bound of the device bytes one frame of W x H occupies during av1mi_encode (its padded input planes, the
+ * encode's arena share and its compacted records): the admission unit of a serving loop. */
+size_t av1mi_frame_bytes(int W, int H, int bitDepth);
+/* compute units of the current device (0 if there is none) */
+int av1mi_compute_units(void);
 
 /* Seeded synthetic test image (the benchmark generator): W x H planes, chroma ((W + ssx) >> ssx) x ((H + ssx) >> ssx);
  * bd 8: uint8 samples, bd 10: uint16; strides in samples.  Host memory. */

@@ -158,6 +158,11 @@ def ref_available():
     return os.path.exists(REF_SO)
 
 
+class Parsed(C.Structure):
+    _fields_ = [("rec", DecisionRecord), ("seq_level_idx", C.c_uint32), ("n_tiles", C.c_uint32), ("tile_bytes", C.c_void_p),
+                ("tile_offsets", C.c_void_p), ("error", C.c_char * 256), ("priv", C.c_void_p)]
+
+
 class Ref:
     def __init__(self):
         self.lib = C.CDLL(REF_SO)
@@ -171,6 +176,29 @@ class Ref:
         L.ref_dequant_inverse.argtypes = [C.c_void_p] + [C.c_int] * 5 + [C.c_void_p]
         L.ref_predict.argtypes = [C.c_int] * 11 + [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         L.ref_build_edges.argtypes = [C.c_void_p] + [C.c_int] * 11 + [C.c_void_p, C.c_void_p]
+        L.av1g_parse.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.POINTER(Parsed))]
+        L.av1g_parsed_free.argtypes = [C.POINTER(Parsed)]
+
+    def parse(self, obus):
+        """The reference's OBU parser (av1core/src/encoder.cpp:1683-1969): the record a stream codes, as a dict."""
+        p = C.POINTER(Parsed)()
+        if self.lib.av1g_parse(obus, len(obus), C.byref(p)) != 0:
+            msg = p.contents.error.decode() if p else "?"
+            if p:
+                self.lib.av1g_parsed_free(p)
+            raise AssertionError("av1g_parse: " + msg)
+        try:
+            r = p.contents.rec
+            nb, nt, nc = r.n_blocks, r.n_txb, r.n_coeffs
+            out = dict(blocks=np.frombuffer(C.string_at(r.blocks, nb * 16), BLOCK_DTYPE).copy(),
+                       block_txb=np.frombuffer(C.string_at(r.block_txb, (nb + 1) * 4), np.uint32).copy(),
+                       txb=np.frombuffer(C.string_at(r.txb, nt * 16), TXB_DTYPE).copy(),
+                       coeffs=np.frombuffer(C.string_at(r.coeffs, nc * 2), np.int16).copy(),
+                       base_q_idx=r.fp.base_q_idx, width=r.fp.width, height=r.fp.height, bit_depth=r.fp.bit_depth,
+                       edge_filter=r.fp.enable_intra_edge_filter, n_tiles=p.contents.n_tiles)
+            return out
+        finally:
+            self.lib.av1g_parsed_free(p)
 
     @staticmethod
     def frame_params(W, H, qindex, ssx=1, mono=0, bd=8, edge_filter=1):

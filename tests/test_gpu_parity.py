@@ -75,10 +75,15 @@ def _decodes(ref, g, W, H, q):
     for p in range(3):
         assert np.array_equal(g.recon[p][:dec[p].shape[0], :dec[p].shape[1]], dec[p]), "plane %d" % p
     assert np.array_equal(info["eob"], g.txb["eob"])
+    # this build's own writer: its stream parses back to the GPU record in the reference's parser
+    parsed = ref.parse(A.write_bitstream(g, avif=False))
+    for k in ("blocks", "block_txb", "txb", "coeffs"):
+        assert np.array_equal(parsed[k], getattr(g, k)), k
     if refcheck.dav1d_available():
-        d = refcheck.dav1d_decode_yuv(avif)
-        for p in range(3):
-            assert np.array_equal(d[p], dec[p]), "dav1d plane %d" % p
+        for f in (avif, A.write_bitstream(g)):
+            d = refcheck.dav1d_decode_yuv(f)
+            for p in range(3):
+                assert np.array_equal(d[p], dec[p]), "dav1d plane %d" % p
 
 
 def test_gpu_record_decodes_to_gpu_reconstruction(enc, ref):
