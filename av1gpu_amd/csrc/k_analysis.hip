@@ -47,8 +47,12 @@ __device__ __forceinline__ int ol_decoded(int px, int py, int x, int y, const Ti
     return zorder16(((px - sbx) >> 2) & 15, ((py - sby) >> 2) & 15) < zorder16(((x - sbx) >> 2) & 15, ((y - sby) >> 2) & 15);
 }
 
+// Occupancy target 4 waves per SIMD: the compiler then keeps the kernels under 128 VGPRs, and the extra waves hide the
+// LDS and table latency (measured on MI355X: analysis of the 12 MP image 68.5 -> 56.7 ms).  The shapes of 32x32 and
+// up are limited by LDS instead (the compiler notes that it cannot reach the target there; the build silences it).
 template <int W, int H, int BS, typename Px>
-__global__ void __launch_bounds__(an_groups(W, H) * 32) k_analysis(Batch B, SearchParams P, uint32_t nTotal) {
+__global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_waves_per_eu(4))) k_analysis(Batch B, SearchParams P,
+                                                                                                      uint32_t nTotal) {
     constexpr int NG = an_groups(W, H), TXS = tx_of(W, H);
     constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
     constexpr int NSUB = (W / 4) * (H / 4), NSX = W / 4;
