@@ -21,6 +21,10 @@ void launch_dp(const Batch &B, const SearchParams &P, hipStream_t st);
 void launch_recon(const Batch &B, const SearchParams &P, const uint32_t *dSbList, uint32_t n, int bd, hipStream_t st);
 void launch_recon_queue(const Batch &B, const SearchParams &P, const RcQueue &Q, uint32_t blocks, int bd, hipStream_t st);
 int recon_queue_slots(int bd);
+struct LnNet;
+LnNet *ln_new(const float *params);
+void ln_delete(LnNet *N);
+int ln_masks(LnNet &N, const Img *dImg, const uint32_t *dSbImg, uint32_t nSb, float conf, uint32_t chunk, uint32_t *dMask, hipStream_t st);
 int recon_prepare(int bd);
 size_t recon_smem_bytes();
 void launch_pack(const Batch &B, const PackOff *dOff, RecBlock *oB, uint32_t *oBT, RecTxb *oT, int16_t *oC, hipStream_t st);
@@ -96,6 +100,9 @@ struct av1mi_encoder {
     hipEvent_t ev[8] = {};
     int cus = 0, slots[2] = {0, 0};   // compute units; resident closed-loop workgroups per CU (8-bit, 10-bit)
     uint64_t clockHz = 100000000;     // wall_clock64 rate
+    LnNet *ln = nullptr;              // learned pruning network (null: exhaustive search)
+    float lnConf = 0.8f;
+    uint32_t lnChunk = 65536;         // superblocks per network pass (bounds the activation memory, ~100 KB each)
 };
 
 extern "C" {
@@ -212,8 +219,22 @@ int av1mi_create(const av1mi_config *cfg, av1mi_encoder **out, char *err, int er
     return 0;
 }
 
+int av1mi_set_shortlist(av1mi_encoder *e, const float *params, float confidence, char *err, int errlen) {
+    ln_delete(e->ln);
+    e->ln = nullptr;
+    if (!params) return 0;
+    if (!(confidence > 0.5f && confidence <= 1.0f)) { snprintf(err, errlen, "confidence %g outside (0.5, 1]", confidence); return -1; }
+    e->ln = ln_new(params);
+    if (!e->ln) { snprintf(err, errlen, "HIP: could not upload the pruning network"); return -1; }
+    e->lnConf = confidence;
+    const char *c = getenv("AV1MI_LN_CHUNK");
+    e->lnChunk = c && atoi(c) > 0 ? (uint32_t)atoi(c) : 65536u;
+    return 0;
+}
+
 void av1mi_destroy(av1mi_encoder *e) {
     if (!e) return;
+    ln_delete(e->ln);
     if (e->dT) (void)hipFree(e->dT);
     for (auto &v : e->ev) if (v) (void)hipEventDestroy(v);
     delete e;
@@ -312,9 +333,10 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     for (auto &v : steps) { stepOff.push_back((uint32_t)stepFlat.size()); stepFlat.insert(stepFlat.end(), v.begin(), v.end()); }
     stepOff.push_back((uint32_t)stepFlat.size());
     if (stepFlat.size() != nSb) { snprintf(err, errlen, "internal: wavefront order covers %zu of %u superblocks", stepFlat.size(), nSb); return -1; }
-    uint32_t *dSteps, *dDone, *dQctl;
+    uint32_t *dSteps, *dDone, *dQctl, *dLnMask = nullptr;
     PackOff *dPO;
     A.add(dSteps, stepFlat.size()); A.add(dPO, nSb); A.add(dDone, nSb); A.add(dQctl, 2);
+    if (enc->ln) A.add(dLnMask, (size_t)nSb * 4);
     HCK(A.alloc(st));
     tm.deviceBytes = A.total;
     for (int i = 0; i < n; i++) {
@@ -345,6 +367,10 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     B.leafCount = dLeafCount; B.leaves = dLeaves; B.sbJ = dSbJ; B.sbOut = dSbOut; B.oBlocks = dOB; B.oBlockTxb = dOBT;
     B.oTxb = dOT; B.oCoeffs = dOC; B.stats = dStats;
     (void)hipEventRecord(enc->ev[1], st);
+    if (enc->ln) {   // the pruning network runs first (its time counts as analysis)
+        if (ln_masks(*enc->ln, dImg, dSbImg, nSb, enc->lnConf, enc->lnChunk, dLnMask, st)) { snprintf(err, errlen, "pruning network failed"); A.release(st); return -1; }
+        B.lnMask = dLnMask;
+    }
     launch_analysis(B, P, shTotal, bd, st);
     HCK(hipGetLastError());
     (void)hipEventRecord(enc->ev[2], st);

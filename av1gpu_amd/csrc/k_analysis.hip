@@ -77,6 +77,22 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
             if (lane == 0) { out.J = INF; out.ncand = 0; }
             continue;
         }
+        // learned pruning: a block belongs to the node of its larger side; it is analysed when that node's parent was
+        // expanded (the 64x64 node's own blocks always are); the screened luma modes are the superblock's shortlist
+        uint32_t modeMask = 0x1FFFu;
+        if (B.lnMask) {
+            const uint32_t *mk = B.lnMask + (size_t)(I.sbBase + (uint32_t)(y >> 6) * I.sbCols + (uint32_t)(x >> 6)) * 4;
+            constexpr int M = W > H ? W : H, PN = 2 * M;
+            if constexpr (M < 64) {
+                constexpr int LV = PN == 64 ? 0 : PN == 32 ? 1 : PN == 16 ? 2 : 3, BASE = LV == 0 ? 0 : LV == 1 ? 1 : LV == 2 ? 5 : 21;
+                const int node = BASE + ((y & 63) / PN) * (64 / PN) + (x & 63) / PN;
+                if (!((mk[node >> 5] >> (node & 31)) & 1)) {
+                    if (lane == 0) { out.J = INF; out.ncand = 0; }
+                    continue;
+                }
+            }
+            modeMask = mk[3] & 0x1FFFu;
+        }
         const int bd = I.bd;
         const TileBox tb = tile_box(I, x, y);
         const int haveL = x > tb.x0, haveA = y > tb.y0;
@@ -98,7 +114,9 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
         };
         // ---- SATD screening (unfiltered edges) ----
         S.satdI[lane] = 0;
-        S.cand[lane] = lane < 13 ? cand_pack(lane, 0) : 0;
+        // the screened modes in mode order (all 13 without a shortlist)
+        const int n13 = __popc(modeMask);
+        if (lane < 13 && ((modeMask >> lane) & 1)) S.cand[__popc(modeMask & ((1u << lane) - 1))] = cand_pack(lane, 0);
         gsync();
         // item = (candidate, 4x4); a candidate's NSUB items are consecutive, so each segment of min(NSUB, 32) lanes
         // sums its Hadamard costs with shuffles before one LDS atomic
@@ -128,19 +146,20 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
             }
             gsync();
         };
-        screen(0, 13);
-        float js = lane < 13 ? fmaf(lamS, mbits(S.cand[lane]), (float)S.satdI[lane] * 0.25f) : INF;
+        screen(0, n13);
+        float js = lane < n13 ? fmaf(lamS, mbits(S.cand[lane]), (float)S.satdI[lane] * 0.25f) : INF;
         if (P.deltas && BS >= BLOCK_8X8) {
-            const float jd = (lane >= V_PRED && lane <= D67_PRED) ? js : INF;
+            const float jd = (lane < n13 && is_dir(cand_mode(S.cand[lane]))) ? js : INF;
             const int b0 = gargmin(jd, lane);
             const int b1 = gargmin(lane == b0 ? INF : jd, lane);
-            if (lane < 12) {
+            const int nd = b0 < 0 ? 0 : b1 < 0 ? 6 : 12;   // deltas around the (up to) two best directional modes
+            if (lane < nd) {
                 const int k = lane % 6;
-                S.cand[13 + lane] = cand_pack(lane < 6 ? b0 : b1, k < 3 ? -(k + 1) : k - 2);
+                S.cand[13 + lane] = cand_pack(cand_mode(S.cand[lane < 6 ? b0 : b1]), k < 3 ? -(k + 1) : k - 2);
             }
             gsync();
-            screen(13, 25);
-            if (lane >= 13 && lane < 25) js = fmaf(lamS, mbits(S.cand[lane]), (float)S.satdI[lane] * 0.25f);
+            screen(13, 13 + nd);
+            if (lane >= 13 && lane < 13 + nd) js = fmaf(lamS, mbits(S.cand[lane]), (float)S.satdI[lane] * 0.25f);
         }
         const int K = min(P.K, MAXK);
         int top[MAXK];

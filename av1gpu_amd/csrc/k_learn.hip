@@ -1,0 +1,521 @@
+// Learned candidate pruning: a small BF16 CNN per 64x64 superblock, run on MFMA with FP32 accumulation, that predicts
+// which square partition nodes need their children analysed and which luma modes to screen.  No counterpart in the
+// reference (its pruning stages are hand-written heuristics: encoder/src/k_leafprep.cu:1-14, an_stage.h:1-15, the
+// screener hook gp.h:136-143).  The network only removes candidates from the exhaustive search, so every decision it
+// leads to is still an ordinary, decodable AV1 decision.
+//
+// Network (per superblock):  x0 = [64 x 64 luma / max - 0.5, qindex / 255]  (NHWC, 2 channels, BF16)
+//   L1..L4: 3x3 convolution, stride 2, zero padding 1, 32 output channels, bias, ReLU   (64 -> 32 -> 16 -> 8 -> 4)
+//   L5: fully connected 512 -> 98 logits: 85 "expand" logits (node 0: the 64x64 node, 1-4: 32x32 nodes, 5-20: 16x16,
+//       21-84: 8x8, raster order per level) and 13 luma-mode logits (each mode's share of the superblock's area).
+// Convolutions are implicit GEMMs on v_mfma_f32_16x16x16_bf16 (positions x input taps times taps x channels), weights
+// and activations BF16, accumulation and biases FP32.  Training runs the same forward kernels, then FP32 backward
+// kernels (convolution weight gradients accumulated with float atomics) and Adam on FP32 master weights.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+#include "dev_eval.h"
+
+namespace av1m {
+
+typedef uint16_t bf16_t;
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+constexpr int LN_OUT = 98, LN_SPLIT = 85, LN_FEAT = 512, LN_C = 32;
+
+__device__ __forceinline__ bf16_t to_bf16(float f) {   // round to nearest even
+    const uint32_t u = __float_as_uint(f);
+    return (bf16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float from_bf16(bf16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// ---- input tiles: 64 x 64 luma of each superblock (padded planes) and the qindex plane ----
+// superblocks s0 .. s0 + n - 1 of the batch go to slots 0 .. n - 1
+__global__ void __launch_bounds__(256) k_ln_gather(const Img *imgs, const uint32_t *sbImg, uint32_t s0, uint32_t n, bf16_t *x0) {
+    const uint32_t slot = blockIdx.x, sb = s0 + slot;
+    if (slot >= n) return;
+    const Img &I = imgs[sbImg[sb]];
+    const uint32_t local = sb - I.sbBase;
+    const int sx = (int)(local % I.sbCols) * 64, sy = (int)(local / I.sbCols) * 64;
+    const float inv = 1.0f / (float)((1 << I.bd) - 1), q = (float)I.qindex / 255.0f;
+    bf16_t *o = x0 + (size_t)slot * 64 * 64 * 2;
+    for (int p = threadIdx.x; p < 64 * 64; p += 256) {
+        const int y = p >> 6, x = p & 63;
+        const size_t off = (size_t)(sy + y) * I.spitch[0] + sx + x;
+        const int v = I.bd == 8 ? ((const uint8_t *)I.src[0])[off] : ((const uint16_t *)I.src[0])[off];
+        o[2 * p] = to_bf16((float)v * inv - 0.5f);
+        o[2 * p + 1] = to_bf16(q);
+    }
+}
+
+// ---- input tiles from 64 x 64 8-bit luma tiles (training data) and their qindex ----
+__global__ void __launch_bounds__(256) k_ln_tiles(const uint8_t *tiles, const uint8_t *q, uint32_t n, bf16_t *x0) {
+    const uint32_t it = blockIdx.x;
+    if (it >= n) return;
+    const float qq = (float)q[it] / 255.0f;
+    for (int p = threadIdx.x; p < 64 * 64; p += 256) {
+        x0[((size_t)it * 4096 + p) * 2] = to_bf16((float)tiles[(size_t)it * 4096 + p] * (1.0f / 255.0f) - 0.5f);
+        x0[((size_t)it * 4096 + p) * 2 + 1] = to_bf16(qq);
+    }
+}
+
+// ---- convolution forward on MFMA: one wave computes 16 output positions x 32 channels of one superblock ----
+// A (16 x 16): lane l holds positions l % 16, taps kb + 4 (l / 16) + 0..3; B (16 x 16): taps likewise, channel l % 16;
+// D: lane l holds channel l % 16 of positions 4 (l / 16) + 0..3.
+template <int CIN, int HIN>
+__global__ void __launch_bounds__(64) k_ln_conv(const bf16_t *in, const bf16_t *W, const float *bias, bf16_t *out, uint32_t n) {
+    constexpr int HOUT = HIN / 2, NPOS = HOUT * HOUT, K = 9 * CIN, KP = (K + 15) / 16 * 16, TILES = NPOS / 16;
+    const uint32_t item = blockIdx.x / TILES, tile = blockIdx.x % TILES;
+    if (item >= n) return;
+    const int l = threadIdx.x, row = l & 15, kq = l >> 4;
+    const int pos = (int)tile * 16 + row, oy = pos / HOUT, ox = pos % HOUT;
+    const bf16_t *x = in + (size_t)item * HIN * HIN * CIN;
+    float4_t acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    for (int kb = 0; kb < KP; kb += 16) {
+        short4_t a;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int k = kb + 4 * kq + j;
+            bf16_t v = 0;
+            if (k < K) {
+                const int t = k / CIN, ci = k % CIN, iy = 2 * oy - 1 + t / 3, ix = 2 * ox - 1 + t % 3;
+                if (iy >= 0 && iy < HIN && ix >= 0 && ix < HIN) v = x[((size_t)iy * HIN + ix) * CIN + ci];
+            }
+            a[j] = (short)v;
+        }
+        short4_t b0, b1;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            b0[j] = (short)W[(size_t)(l & 15) * KP + kb + 4 * kq + j];
+            b1[j] = (short)W[(size_t)(16 + (l & 15)) * KP + kb + 4 * kq + j];
+        }
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b0, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b1, acc1, 0, 0, 0);
+    }
+    bf16_t *o = out + (size_t)item * NPOS * LN_C;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int p = (int)tile * 16 + 4 * kq + r, c = l & 15;
+        o[(size_t)p * LN_C + c] = to_bf16(fmaxf(acc0[r] + bias[c], 0.f));
+        o[(size_t)p * LN_C + 16 + c] = to_bf16(fmaxf(acc1[r] + bias[16 + c], 0.f));
+    }
+}
+
+// ---- fully connected head: logits[item][o] = b[o] + sum_f W[o][f] a4[item][f] (FP32, f ascending) ----
+__global__ void __launch_bounds__(128) k_ln_fc(const bf16_t *a4, const float *W, const float *b, float *logits, uint32_t n) {
+    const uint32_t item = blockIdx.x;
+    if (item >= n) return;
+    const bf16_t *f = a4 + (size_t)item * LN_FEAT;
+    for (int o = threadIdx.x; o < LN_OUT; o += 128) {
+        const float *w = W + (size_t)o * LN_FEAT;
+        float s = b[o];
+        for (int k = 0; k < LN_FEAT; k++) s = fmaf(w[k], from_bf16(f[k]), s);
+        logits[(size_t)item * LN_OUT + o] = s;
+    }
+}
+
+// ---- decisions: per superblock the nodes whose children are analysed and the luma modes screened ----
+// A node is expanded unless the network is confident (p <= 1 - conf) that the search would not use its children, and
+// always where it reaches past the frame (the partition syntax may force a split there).  The modes are the 8 most
+// likely when they carry at least conf of the total probability, else all 13.  mask: 3 words of node bits, 1 of modes.
+// logits of slots 0 .. n - 1 belong to superblocks s0 .. s0 + n - 1 of the batch; mask is indexed by superblock
+__global__ void __launch_bounds__(64) k_ln_decide(const float *logits, const Img *imgs, const uint32_t *sbImg, uint32_t s0, uint32_t n,
+                                                  float conf, uint32_t *mask) {
+    const uint32_t slot = blockIdx.x * 64 + threadIdx.x, sb = s0 + slot;
+    if (slot >= n) return;
+    const Img &I = imgs[sbImg[sb]];
+    const uint32_t local = sb - I.sbBase;
+    const int sx = (int)(local % I.sbCols) * 64, sy = (int)(local / I.sbCols) * 64;
+    const int FW = I.MiCols * 4, FH = I.MiRows * 4;
+    const float *z = logits + (size_t)slot * LN_OUT;
+    uint32_t m[4] = {0, 0, 0, 0};
+    int node = 0;
+    for (int lv = 0; lv < 4; lv++) {
+        const int n = 64 >> lv, per = 1 << lv;
+        for (int ny = 0; ny < per; ny++)
+            for (int nx = 0; nx < per; nx++, node++) {
+                const bool edge = sx + nx * n + n > FW || sy + ny * n + n > FH;
+                const float p = 1.f / (1.f + expf(-z[node]));
+                if (edge || p > 1.f - conf) m[node >> 5] |= 1u << (node & 31);
+            }
+    }
+    float p[13], tot = 0.f;
+    for (int k = 0; k < 13; k++) { p[k] = 1.f / (1.f + expf(-z[LN_SPLIT + k])); tot += p[k]; }
+    uint32_t top = 0;
+    float mass = 0.f;
+    for (int r = 0; r < 8; r++) {
+        int b = -1;
+        for (int k = 0; k < 13; k++) if (!((top >> k) & 1) && (b < 0 || p[k] > p[b])) b = k;
+        top |= 1u << b;
+        mass += p[b];
+    }
+    m[3] = mass >= conf * tot ? top : 0x1FFFu;
+    for (int k = 0; k < 4; k++) mask[(size_t)sb * 4 + k] = m[k];
+}
+
+// ================================ training ================================
+
+// loss (binary cross-entropy with logits, masked) and its gradient / count
+__global__ void __launch_bounds__(128) k_ln_loss(const float *logits, const float *label, const float *lmask, uint32_t n, float scale,
+                                                 float *dlog, float *lossOut) {
+    const uint32_t i = blockIdx.x * 128 + threadIdx.x;
+    float l = 0.f;
+    if (i < n * LN_OUT) {
+        const float z = logits[i], y = label[i], w = lmask[i];
+        const float s = 1.f / (1.f + expf(-z));
+        dlog[i] = w * (s - y) * scale;
+        l = w * (fmaxf(z, 0.f) - z * y + log1pf(expf(-fabsf(z))));
+    }
+    for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o, 64);
+    if ((threadIdx.x & 63) == 0 && l != 0.f) atomicAdd(lossOut, l * scale);
+}
+
+// FC backward, inputs: da4[item][f] = sum_o dlog[item][o] W[o][f], masked by the ReLU of a4
+__global__ void __launch_bounds__(256) k_ln_fc_bwd_in(const bf16_t *a4, const float *W, const float *dlog, uint32_t n, float *da4) {
+    const uint32_t item = blockIdx.x;
+    if (item >= n) return;
+    const float *g = dlog + (size_t)item * LN_OUT;
+    for (int k = threadIdx.x; k < LN_FEAT; k += 256) {
+        float s = 0.f;
+        for (int o = 0; o < LN_OUT; o++) s = fmaf(g[o], W[(size_t)o * LN_FEAT + k], s);
+        da4[(size_t)item * LN_FEAT + k] = from_bf16(a4[(size_t)item * LN_FEAT + k]) > 0.f ? s : 0.f;
+    }
+}
+// FC backward, weights: one thread per (o, f) over the items (no atomics); the bias row uses f = LN_FEAT
+__global__ void __launch_bounds__(256) k_ln_fc_bwd_w(const bf16_t *a4, const float *dlog, uint32_t n, float *dW, float *db) {
+    const uint32_t e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= (uint32_t)LN_OUT * (LN_FEAT + 1)) return;
+    const int o = (int)(e / (LN_FEAT + 1)), k = (int)(e % (LN_FEAT + 1));
+    float s = 0.f;
+    for (uint32_t it = 0; it < n; it++) {
+        const float g = dlog[(size_t)it * LN_OUT + o];
+        s = fmaf(g, k < LN_FEAT ? from_bf16(a4[(size_t)it * LN_FEAT + k]) : 1.f, s);
+    }
+    if (k < LN_FEAT) dW[(size_t)o * LN_FEAT + k] += s;
+    else db[o] += s;
+}
+
+// convolution backward.  dz: gradient at the convolution's output before the ReLU (already masked), FP32 NHWC.
+// Weights: dW[co][k] += sum over items and output positions of dz[pos][co] * im2col(in)[pos][k]; db[co] += sum dz.
+template <int CIN, int HIN>
+__global__ void __launch_bounds__(256) k_ln_conv_bwd_w(const bf16_t *in, const float *dz, uint32_t n, float *dW, float *db) {
+    constexpr int HOUT = HIN / 2, NPOS = HOUT * HOUT, K = 9 * CIN, KP = (K + 15) / 16 * 16;
+    const uint32_t item = blockIdx.x;
+    if (item >= n) return;
+    const bf16_t *x = in + (size_t)item * HIN * HIN * CIN;
+    const float *g = dz + (size_t)item * NPOS * LN_C;
+    for (int e = threadIdx.x; e < LN_C * K; e += 256) {
+        const int co = e / K, k = e % K, t = k / CIN, ci = k % CIN, ky = t / 3, kx = t % 3;
+        float s = 0.f;
+        for (int pos = 0; pos < NPOS; pos++) {
+            const int oy = pos / HOUT, ox = pos % HOUT, iy = 2 * oy - 1 + ky, ix = 2 * ox - 1 + kx;
+            if (iy < 0 || iy >= HIN || ix < 0 || ix >= HIN) continue;
+            s = fmaf(g[(size_t)pos * LN_C + co], from_bf16(x[((size_t)iy * HIN + ix) * CIN + ci]), s);
+        }
+        if (s != 0.f) atomicAdd(&dW[(size_t)co * KP + k], s);
+    }
+    for (int co = threadIdx.x; co < LN_C; co += 256) {
+        float s = 0.f;
+        for (int pos = 0; pos < NPOS; pos++) s += g[(size_t)pos * LN_C + co];
+        if (s != 0.f) atomicAdd(&db[co], s);
+    }
+}
+// Input: dIn[iy][ix][ci] = sum over taps (ky, kx) and channels co of dz[oy][ox][co] W[co][tap][ci] with
+// iy = 2 oy - 1 + ky, then masked by the ReLU of that input (the previous layer's output > 0): the previous dz.
+template <int CIN, int HIN>
+__global__ void __launch_bounds__(256) k_ln_conv_bwd_in(const bf16_t *in, const float *W, const float *dz, uint32_t n, float *dzPrev) {
+    constexpr int HOUT = HIN / 2, K = 9 * CIN, KP = (K + 15) / 16 * 16;
+    const uint32_t item = blockIdx.x;
+    if (item >= n) return;
+    const bf16_t *x = in + (size_t)item * HIN * HIN * CIN;
+    const float *g = dz + (size_t)item * HOUT * HOUT * LN_C;
+    float *d = dzPrev + (size_t)item * HIN * HIN * CIN;
+    for (int e = threadIdx.x; e < HIN * HIN * CIN; e += 256) {
+        const int ci = e % CIN, p = e / CIN, iy = p / HIN, ix = p % HIN;
+        float s = 0.f;
+        if (from_bf16(x[e]) > 0.f) {
+            for (int ky = 0; ky < 3; ky++) {
+                const int ty = iy + 1 - ky;
+                if (ty < 0 || (ty & 1) || (ty >> 1) >= HOUT) continue;
+                for (int kx = 0; kx < 3; kx++) {
+                    const int tx = ix + 1 - kx;
+                    if (tx < 0 || (tx & 1) || (tx >> 1) >= HOUT) continue;
+                    const float *gz = g + ((size_t)(ty >> 1) * HOUT + (tx >> 1)) * LN_C;
+                    const int k = (ky * 3 + kx) * CIN + ci;
+                    for (int co = 0; co < LN_C; co++) s = fmaf(gz[co], W[(size_t)co * KP + k], s);
+                }
+            }
+        }
+        d[e] = s;
+    }
+}
+
+// Adam on the FP32 master weights; refreshes the BF16 copy (bf may be null)
+__global__ void __launch_bounds__(256) k_ln_adam(float *w, const float *g, float *m, float *v, bf16_t *bf, uint32_t n, float lr, float b1,
+                                                 float b2, float eps, float c1, float c2, float wd) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i] + wd * w[i];
+    m[i] = b1 * m[i] + (1.f - b1) * gi;
+    v[i] = b2 * v[i] + (1.f - b2) * gi * gi;
+    w[i] -= lr * (m[i] / c1) / (sqrtf(v[i] / c2) + eps);
+    if (bf) bf[i] = to_bf16(w[i]);
+}
+
+__global__ void __launch_bounds__(256) k_ln_to_bf16(const float *w, bf16_t *bf, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) bf[i] = to_bf16(w[i]);
+}
+
+// ================================ host side ================================
+
+// parameter layout (FP32 masters): per convolution l = 1..4 the weights [32][KP_l] (taps past 9 Cin are zero) and the
+// bias [32], then the FC weights [98][512] and bias [98]
+constexpr int LN_KP1 = 32, LN_KP = 288;
+constexpr size_t LN_W_OFF[4] = {0, 32 * LN_KP1 + 32, 32 * LN_KP1 + 32 + 32 * LN_KP + 32, 32 * LN_KP1 + 32 + 2 * (32 * LN_KP + 32)};
+constexpr size_t LN_FC_OFF = 32 * LN_KP1 + 32 + 3 * (32 * LN_KP + 32);
+constexpr size_t LN_NPARAM = LN_FC_OFF + (size_t)LN_OUT * LN_FEAT + LN_OUT;
+constexpr size_t LN_NBF = 32 * LN_KP1 + 3 * 32 * LN_KP;   // the BF16 copies of the convolution weights
+inline size_t ln_kp(int l) { return l == 0 ? LN_KP1 : LN_KP; }
+inline size_t ln_bf_off(int l) { return l == 0 ? 0 : 32 * LN_KP1 + (size_t)(l - 1) * 32 * LN_KP; }
+
+struct LnNet {
+    float *w = nullptr;            // FP32 parameters (LN_NPARAM)
+    bf16_t *wb = nullptr;          // BF16 convolution weights (LN_NBF)
+    // activations for up to cap items
+    uint32_t cap = 0;
+    bf16_t *x0 = nullptr, *a1 = nullptr, *a2 = nullptr, *a3 = nullptr, *a4 = nullptr;
+    float *logits = nullptr;
+    // training state (allocated on first use)
+    float *g = nullptr, *m = nullptr, *v = nullptr, *dlog = nullptr, *d4 = nullptr, *d3 = nullptr, *d2 = nullptr, *d1 = nullptr, *loss = nullptr;
+    float *label = nullptr, *lmask = nullptr;
+    uint8_t *tiles = nullptr, *q = nullptr;
+    uint32_t dcap = 0;             // items the gradient buffers hold
+    uint32_t ndata = 0;            // items of training data on the device
+    int step = 0;
+};
+
+static hipError_t ln_alloc_acts(LnNet &N, uint32_t n) {
+    if (n <= N.cap) return hipSuccess;
+    (void)hipFree(N.x0); (void)hipFree(N.a1); (void)hipFree(N.a2); (void)hipFree(N.a3); (void)hipFree(N.a4); (void)hipFree(N.logits);
+    N.x0 = N.a1 = N.a2 = N.a3 = N.a4 = nullptr; N.logits = nullptr; N.cap = 0;
+    hipError_t e;
+    if ((e = hipMalloc(&N.x0, (size_t)n * 64 * 64 * 2 * 2)) != hipSuccess) return e;
+    if ((e = hipMalloc(&N.a1, (size_t)n * 32 * 32 * 32 * 2)) != hipSuccess) return e;
+    if ((e = hipMalloc(&N.a2, (size_t)n * 16 * 16 * 32 * 2)) != hipSuccess) return e;
+    if ((e = hipMalloc(&N.a3, (size_t)n * 8 * 8 * 32 * 2)) != hipSuccess) return e;
+    if ((e = hipMalloc(&N.a4, (size_t)n * LN_FEAT * 2)) != hipSuccess) return e;
+    if ((e = hipMalloc(&N.logits, (size_t)n * LN_OUT * 4)) != hipSuccess) return e;
+    N.cap = n;
+    return hipSuccess;
+}
+
+// forward of n items whose inputs are in N.x0
+static void ln_forward(LnNet &N, uint32_t n, hipStream_t st) {
+    const float *w = N.w;
+    hipLaunchKernelGGL((k_ln_conv<2, 64>), dim3(n * 64), dim3(64), 0, st, N.x0, N.wb + ln_bf_off(0), w + LN_W_OFF[0] + 32 * LN_KP1, N.a1, n);
+    hipLaunchKernelGGL((k_ln_conv<32, 32>), dim3(n * 16), dim3(64), 0, st, N.a1, N.wb + ln_bf_off(1), w + LN_W_OFF[1] + 32 * LN_KP, N.a2, n);
+    hipLaunchKernelGGL((k_ln_conv<32, 16>), dim3(n * 4), dim3(64), 0, st, N.a2, N.wb + ln_bf_off(2), w + LN_W_OFF[2] + 32 * LN_KP, N.a3, n);
+    hipLaunchKernelGGL((k_ln_conv<32, 8>), dim3(n), dim3(64), 0, st, N.a3, N.wb + ln_bf_off(3), w + LN_W_OFF[3] + 32 * LN_KP, N.a4, n);
+    hipLaunchKernelGGL(k_ln_fc, dim3(n), dim3(128), 0, st, N.a4, w + LN_FC_OFF, w + LN_FC_OFF + (size_t)LN_OUT * LN_FEAT, N.logits, n);
+}
+
+static void ln_refresh_bf16(LnNet &N, hipStream_t st) {
+    for (int l = 0; l < 4; l++)
+        hipLaunchKernelGGL(k_ln_to_bf16, dim3((unsigned)((32 * ln_kp(l) + 255) / 256)), dim3(256), 0, st, N.w + LN_W_OFF[l],
+                           N.wb + ln_bf_off(l), (uint32_t)(32 * ln_kp(l)));
+}
+
+static hipError_t ln_upload(LnNet &N, const float *params) {
+    hipError_t e;
+    if (!N.w && (e = hipMalloc(&N.w, LN_NPARAM * 4)) != hipSuccess) return e;
+    if (!N.wb && (e = hipMalloc(&N.wb, LN_NBF * 2)) != hipSuccess) return e;
+    if ((e = hipMemcpy(N.w, params, LN_NPARAM * 4, hipMemcpyHostToDevice)) != hipSuccess) return e;
+    ln_refresh_bf16(N, 0);
+    return hipDeviceSynchronize();
+}
+
+static void ln_free(LnNet &N) {
+    void *p[] = {N.w, N.wb, N.x0, N.a1, N.a2, N.a3, N.a4, N.logits, N.g, N.m, N.v, N.dlog, N.d4, N.d3, N.d2, N.d1, N.loss, N.label, N.lmask,
+                 N.tiles, N.q};
+    for (void *x : p) if (x) (void)hipFree(x);
+    N = LnNet();
+}
+
+LnNet *ln_new(const float *params) {
+    LnNet *N = new LnNet();
+    if (ln_upload(*N, params) != hipSuccess) { ln_free(*N); delete N; return nullptr; }
+    return N;
+}
+void ln_delete(LnNet *N) {
+    if (!N) return;
+    ln_free(*N);
+    delete N;
+}
+
+// learned masks of a batch (encoder integration): mask[nSb][4]
+int ln_masks(LnNet &N, const Img *dImg, const uint32_t *dSbImg, uint32_t nSb, float conf, uint32_t chunk, uint32_t *dMask, hipStream_t st) {
+    for (uint32_t s0 = 0; s0 < nSb; s0 += chunk) {
+        const uint32_t n = nSb - s0 < chunk ? nSb - s0 : chunk;
+        if (ln_alloc_acts(N, n) != hipSuccess) return -1;
+        hipLaunchKernelGGL(k_ln_gather, dim3(n), dim3(256), 0, st, dImg, dSbImg, s0, n, N.x0);
+        ln_forward(N, n, st);
+        hipLaunchKernelGGL(k_ln_decide, dim3((n + 63) / 64), dim3(64), 0, st, N.logits, dImg, dSbImg, s0, n, conf, dMask);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace av1m
+
+// ================================ C ABI ================================
+using namespace av1m;
+
+struct av1mi_learner { LnNet net; };
+
+extern "C" {
+
+size_t av1mi_ln_param_count(void) { return LN_NPARAM; }
+
+// He-normal initialisation from a seed (Box-Muller over splitmix64), zero biases and zero padding taps
+void av1mi_ln_init_params(float *p, uint64_t seed) {
+    auto next = [&]() {
+        seed += 0x9E3779B97F4A7C15ull;
+        uint64_t z = seed;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return (double)((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+    };
+    auto gauss = [&]() { return sqrt(-2.0 * log(next() + 1e-300)) * cos(6.283185307179586 * next()); };
+    memset(p, 0, LN_NPARAM * 4);
+    for (int l = 0; l < 4; l++) {
+        const int cin = l == 0 ? 2 : 32, K = 9 * cin;
+        const double sd = sqrt(2.0 / K);
+        for (int co = 0; co < 32; co++)
+            for (int k = 0; k < K; k++) p[LN_W_OFF[l] + (size_t)co * ln_kp(l) + k] = (float)(sd * gauss());
+    }
+    const double sd = sqrt(1.0 / LN_FEAT);
+    for (size_t i = 0; i < (size_t)LN_OUT * LN_FEAT; i++) p[LN_FC_OFF + i] = (float)(sd * gauss());
+}
+
+int av1mi_ln_create(const float *params, av1mi_learner **out) {
+    av1mi_learner *L = new av1mi_learner();
+    if (ln_upload(L->net, params) != hipSuccess) { ln_free(L->net); delete L; return -1; }
+    *out = L;
+    return 0;
+}
+void av1mi_ln_destroy(av1mi_learner *L) {
+    if (!L) return;
+    ln_free(L->net);
+    delete L;
+}
+int av1mi_ln_get_params(av1mi_learner *L, float *params) {
+    return hipMemcpy(params, L->net.w, LN_NPARAM * 4, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
+}
+
+// forward of n 64x64 8-bit luma tiles (host arrays: tiles n x 4096, q n) -> logits (n x 98)
+int av1mi_ln_forward(av1mi_learner *L, const uint8_t *tiles, const uint8_t *q, uint32_t n, float *logits) {
+    LnNet &N = L->net;
+    if (n == 0) return 0;
+    if (ln_alloc_acts(N, n) != hipSuccess) return -1;
+    uint8_t *dt = nullptr, *dq = nullptr;
+    if (hipMalloc(&dt, (size_t)n * 4096) != hipSuccess || hipMalloc(&dq, n) != hipSuccess) { (void)hipFree(dt); return -1; }
+    (void)hipMemcpy(dt, tiles, (size_t)n * 4096, hipMemcpyHostToDevice);
+    (void)hipMemcpy(dq, q, n, hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_ln_tiles, dim3(n), dim3(256), 0, 0, dt, dq, n, N.x0);
+    ln_forward(N, n, 0);
+    const hipError_t e = hipMemcpy(logits, N.logits, (size_t)n * LN_OUT * 4, hipMemcpyDeviceToHost);
+    (void)hipFree(dt); (void)hipFree(dq);
+    return e == hipSuccess ? 0 : -1;
+}
+
+// the activations of the last forward (for the checks against the CPU): layer 0 = x0 .. 4 = a4, as BF16 bits
+int av1mi_ln_activations(av1mi_learner *L, int layer, uint32_t n, uint16_t *out) {
+    LnNet &N = L->net;
+    const size_t per[5] = {64 * 64 * 2, 32 * 32 * 32, 16 * 16 * 32, 8 * 8 * 32, LN_FEAT};
+    const bf16_t *src[5] = {N.x0, N.a1, N.a2, N.a3, N.a4};
+    if (layer < 0 || layer > 4 || n > N.cap) return -1;
+    return hipMemcpy(out, src[layer], n * per[layer] * 2, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
+}
+
+// Training data on the device: n tiles (n x 4096 8-bit luma), their qindex, labels and label masks (n x 98 each).
+int av1mi_ln_set_data(av1mi_learner *L, const uint8_t *tiles, const uint8_t *q, const float *label, const float *lmask, uint32_t n) {
+    LnNet &N = L->net;
+    (void)hipFree(N.tiles); (void)hipFree(N.q); (void)hipFree(N.label); (void)hipFree(N.lmask);
+    N.tiles = N.q = nullptr; N.label = N.lmask = nullptr; N.ndata = 0;
+    if (n == 0) return -1;
+    if (hipMalloc(&N.tiles, (size_t)n * 4096) != hipSuccess || hipMalloc(&N.q, n) != hipSuccess ||
+        hipMalloc(&N.label, (size_t)n * LN_OUT * 4) != hipSuccess || hipMalloc(&N.lmask, (size_t)n * LN_OUT * 4) != hipSuccess)
+        return -1;
+    (void)hipMemcpy(N.tiles, tiles, (size_t)n * 4096, hipMemcpyHostToDevice);
+    (void)hipMemcpy(N.q, q, n, hipMemcpyHostToDevice);
+    (void)hipMemcpy(N.label, label, (size_t)n * LN_OUT * 4, hipMemcpyHostToDevice);
+    if (hipMemcpy(N.lmask, lmask, (size_t)n * LN_OUT * 4, hipMemcpyHostToDevice) != hipSuccess) return -1;
+    N.ndata = n;
+    return 0;
+}
+
+// Gradient of the mean masked BCE over items [first, first + n) of the data into N.g (and, with grads != null, to the
+// host); returns the loss through *loss.
+static int ln_grad(LnNet &N, uint32_t first, uint32_t n, float *loss, hipStream_t st) {
+    if (n == 0 || first >= N.ndata || n > N.ndata - first) return -1;   // items past the data on the device
+    if (ln_alloc_acts(N, n) != hipSuccess) return -1;
+    if (!N.g) {
+        if (hipMalloc(&N.g, LN_NPARAM * 4) != hipSuccess || hipMalloc(&N.m, LN_NPARAM * 4) != hipSuccess ||
+            hipMalloc(&N.v, LN_NPARAM * 4) != hipSuccess || hipMalloc(&N.loss, 4) != hipSuccess)
+            return -1;
+        (void)hipMemset(N.m, 0, LN_NPARAM * 4);
+        (void)hipMemset(N.v, 0, LN_NPARAM * 4);
+    }
+    if (n > N.dcap || !N.dlog) {
+        (void)hipFree(N.dlog); (void)hipFree(N.d4); (void)hipFree(N.d3); (void)hipFree(N.d2); (void)hipFree(N.d1);
+        if (hipMalloc(&N.dlog, (size_t)n * LN_OUT * 4) != hipSuccess || hipMalloc(&N.d4, (size_t)n * LN_FEAT * 4) != hipSuccess ||
+            hipMalloc(&N.d3, (size_t)n * 8 * 8 * 32 * 4) != hipSuccess || hipMalloc(&N.d2, (size_t)n * 16 * 16 * 32 * 4) != hipSuccess ||
+            hipMalloc(&N.d1, (size_t)n * 32 * 32 * 32 * 4) != hipSuccess)
+            return -1;
+        N.dcap = n;
+    }
+    (void)hipMemsetAsync(N.g, 0, LN_NPARAM * 4, st);
+    (void)hipMemsetAsync(N.loss, 0, 4, st);
+    hipLaunchKernelGGL(k_ln_tiles, dim3(n), dim3(256), 0, st, N.tiles + (size_t)first * 4096, N.q + first, n, N.x0);
+    ln_forward(N, n, st);
+    const float scale = 1.0f / (float)n;
+    hipLaunchKernelGGL(k_ln_loss, dim3((n * LN_OUT + 127) / 128), dim3(128), 0, st, N.logits, N.label + (size_t)first * LN_OUT,
+                       N.lmask + (size_t)first * LN_OUT, n, scale, N.dlog, N.loss);
+    float *g = N.g;
+    const float *w = N.w;
+    hipLaunchKernelGGL(k_ln_fc_bwd_w, dim3((LN_OUT * (LN_FEAT + 1) + 255) / 256), dim3(256), 0, st, N.a4, N.dlog, n, g + LN_FC_OFF,
+                       g + LN_FC_OFF + (size_t)LN_OUT * LN_FEAT);
+    hipLaunchKernelGGL(k_ln_fc_bwd_in, dim3(n), dim3(256), 0, st, N.a4, w + LN_FC_OFF, N.dlog, n, N.d4);
+    // layer 4 (input a3, 8x8): weights, then the gradient at a3 (masked by its ReLU)
+    hipLaunchKernelGGL((k_ln_conv_bwd_w<32, 8>), dim3(n), dim3(256), 0, st, N.a3, N.d4, n, g + LN_W_OFF[3], g + LN_W_OFF[3] + 32 * LN_KP);
+    hipLaunchKernelGGL((k_ln_conv_bwd_in<32, 8>), dim3(n), dim3(256), 0, st, N.a3, w + LN_W_OFF[3], N.d4, n, N.d3);
+    hipLaunchKernelGGL((k_ln_conv_bwd_w<32, 16>), dim3(n), dim3(256), 0, st, N.a2, N.d3, n, g + LN_W_OFF[2], g + LN_W_OFF[2] + 32 * LN_KP);
+    hipLaunchKernelGGL((k_ln_conv_bwd_in<32, 16>), dim3(n), dim3(256), 0, st, N.a2, w + LN_W_OFF[2], N.d3, n, N.d2);
+    hipLaunchKernelGGL((k_ln_conv_bwd_w<32, 32>), dim3(n), dim3(256), 0, st, N.a1, N.d2, n, g + LN_W_OFF[1], g + LN_W_OFF[1] + 32 * LN_KP);
+    hipLaunchKernelGGL((k_ln_conv_bwd_in<32, 32>), dim3(n), dim3(256), 0, st, N.a1, w + LN_W_OFF[1], N.d2, n, N.d1);
+    hipLaunchKernelGGL((k_ln_conv_bwd_w<2, 64>), dim3(n), dim3(256), 0, st, N.x0, N.d1, n, g + LN_W_OFF[0], g + LN_W_OFF[0] + 32 * LN_KP1);
+    if (hipGetLastError() != hipSuccess) return -1;
+    if (loss && hipMemcpyAsync(loss, N.loss, 4, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
+    return hipStreamSynchronize(st) == hipSuccess ? 0 : -1;
+}
+
+int av1mi_ln_gradient(av1mi_learner *L, uint32_t first, uint32_t n, float *loss, float *grads) {
+    if (ln_grad(L->net, first, n, loss, 0)) return -1;
+    return hipMemcpy(grads, L->net.g, LN_NPARAM * 4, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
+}
+
+// one Adam step on items [first, first + n) of the data
+int av1mi_ln_train_step(av1mi_learner *L, uint32_t first, uint32_t n, float lr, float wd, float *loss) {
+    LnNet &N = L->net;
+    if (ln_grad(N, first, n, loss, 0)) return -1;
+    N.step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - powf(b1, (float)N.step), c2 = 1.f - powf(b2, (float)N.step);
+    hipLaunchKernelGGL(k_ln_adam, dim3((unsigned)((LN_NPARAM + 255) / 256)), dim3(256), 0, 0, N.w, N.g, N.m, N.v, (bf16_t *)nullptr,
+                       (uint32_t)LN_NPARAM, lr, b1, b2, 1e-8f, c1, c2, wd);
+    ln_refresh_bf16(N, 0);
+    return hipDeviceSynchronize() == hipSuccess ? 0 : -1;
+}
+
+}  // extern "C"

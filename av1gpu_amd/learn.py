@@ -1,0 +1,287 @@
+"""Learned candidate pruning: labels, training, inference helpers and the checks of the network kernels.
+
+The network (av1gpu_amd/csrc/k_learn.hip) sees the 64x64 luma of a superblock and the qindex and predicts
+  * 85 "expand" outputs, one per square partition node (the 64x64 node, four 32x32, sixteen 16x16, sixty-four 8x8, raster
+    order per level): 1 when the exhaustive search's leaves inside the node are at most half its size in both dimensions
+    (its children are used);
+  * 13 luma-mode outputs: the share of the superblock's area each luma mode covers in the exhaustive result.
+The encoder then analyses a node's children only where the network is not confident (1 - conf) that they are unused,
+and screens the 8 most likely modes only where they carry at least conf of the probability mass.
+
+Training data are the exhaustive search's own decisions on seeded synthetic images; training runs on the GPU with the
+library's forward (MFMA) and backward kernels.  ``ref_conv`` / ``ref_fc`` / ``ref_grads`` restate the same network in
+float64 NumPy with the BF16 rounding at the same points; the tests check the kernels' activations and gradients with
+them.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+import av1gpu_amd as A
+
+N_OUT, N_SPLIT, N_FEAT = 98, 85, 512
+LEVEL_BASE = (0, 1, 5, 21)
+KP = (32, 288, 288, 288)
+CIN = (2, 32, 32, 32)
+
+
+def _sigs():
+    L = A.lib()
+    if getattr(L, "_ln_sigs", False):
+        return L
+    P, U8, F = C.c_void_p, C.c_void_p, C.c_void_p
+    L.av1mi_ln_param_count.restype = C.c_size_t
+    L.av1mi_ln_init_params.argtypes = [F, C.c_uint64]
+    L.av1mi_ln_create.argtypes = [F, C.POINTER(C.c_void_p)]
+    L.av1mi_ln_destroy.argtypes = [P]
+    L.av1mi_ln_get_params.argtypes = [P, F]
+    L.av1mi_ln_forward.argtypes = [P, U8, U8, C.c_uint32, F]
+    L.av1mi_ln_activations.argtypes = [P, C.c_int, C.c_uint32, P]
+    L.av1mi_ln_set_data.argtypes = [P, U8, U8, F, F, C.c_uint32]
+    L.av1mi_ln_gradient.argtypes = [P, C.c_uint32, C.c_uint32, C.POINTER(C.c_float), F]
+    L.av1mi_ln_train_step.argtypes = [P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.POINTER(C.c_float)]
+    L.av1mi_set_shortlist.argtypes = [P, F, C.c_float, C.c_char_p, C.c_int]
+    L._ln_sigs = True
+    return L
+
+
+def param_count():
+    return int(_sigs().av1mi_ln_param_count())
+
+
+def init_params(seed=1):
+    p = np.zeros(param_count(), np.float32)
+    _sigs().av1mi_ln_init_params(p.ctypes.data, seed)
+    return p
+
+
+def unpack(p):
+    """Views of the parameter vector: [(W_l [32][KP_l], b_l [32]) for l in 0..3], W5 [98][512], b5 [98]."""
+    out, off = [], 0
+    for l in range(4):
+        w = p[off:off + 32 * KP[l]].reshape(32, KP[l])
+        off += 32 * KP[l]
+        b = p[off:off + 32]
+        off += 32
+        out.append((w, b))
+    w5 = p[off:off + N_OUT * N_FEAT].reshape(N_OUT, N_FEAT)
+    off += N_OUT * N_FEAT
+    return out, w5, p[off:off + N_OUT]
+
+
+# ---------------------------------------------------------------- data
+def sb_tiles(y, W, H):
+    """64x64 luma tiles of every superblock (edge-replicated padding), raster order: [nSb, 4096] uint8."""
+    aw, ah = (W + 63) // 64 * 64, (H + 63) // 64 * 64
+    p = np.pad(y[:H, :W], ((0, ah - H), (0, aw - W)), mode="edge")
+    t = p.reshape(ah // 64, 64, aw // 64, 64).transpose(0, 2, 1, 3).reshape(-1, 4096)
+    return np.ascontiguousarray(t)
+
+
+def sb_labels(blocks, W, H):
+    """Labels and label masks [nSb, 98] of a record's leaves (see the module docstring); nodes reaching past the frame
+    are masked out (the encoder expands them regardless)."""
+    MiCols, MiRows = 2 * ((W + 7) >> 3), 2 * ((H + 7) >> 3)
+    sbc, sbr = (MiCols + 15) >> 4, (MiRows + 15) >> 4
+    lab = np.zeros((sbr * sbc, N_OUT), np.float32)
+    msk = np.ones((sbr * sbc, N_OUT), np.float32)
+    FW, FH = MiCols * 4, MiRows * 4
+    bw = np.array([4 * (1 << v) for v in _MI_W_LOG2])[blocks["bsize"]]
+    bh = np.array([4 * (1 << v) for v in _MI_H_LOG2])[blocks["bsize"]]
+    x, y = blocks["mi_col"].astype(np.int64) * 4, blocks["mi_row"].astype(np.int64) * 4
+    sb = (y // 64) * sbc + x // 64
+    lx, ly = x % 64, y % 64
+    m = np.maximum(bw, bh)
+    for lv in range(4):
+        n = 64 >> lv
+        sel = n >= 2 * m
+        node = LEVEL_BASE[lv] + (ly // n) * (64 // n) + lx // n
+        lab[sb[sel], node[sel]] = 1.0
+        per = 1 << lv
+        for ny in range(per):
+            for nx in range(per):
+                nd = LEVEL_BASE[lv] + ny * per + nx
+                sxs = (np.arange(sbr * sbc) % sbc) * 64 + nx * n
+                sys = (np.arange(sbr * sbc) // sbc) * 64 + ny * n
+                msk[(sxs + n > FW) | (sys + n > FH), nd] = 0.0
+    # modes: the share of the superblock's (visible) area each luma mode covers
+    area = (np.minimum(x + bw, FW) - x) * (np.minimum(y + bh, FH) - y)
+    np.add.at(lab, (sb, N_SPLIT + blocks["y_mode"].astype(np.int64)), area.astype(np.float32))
+    tot = lab[:, N_SPLIT:].sum(1, keepdims=True)
+    lab[:, N_SPLIT:] /= np.maximum(tot, 1.0)
+    return lab, msk
+
+
+_MI_W_LOG2 = (0, 0, 1, 1, 1, 2, 2, 2, 3, 3, 3, 4, 4, 4, 5, 5, 0, 2, 1, 3, 2, 4)
+_MI_H_LOG2 = (0, 1, 0, 1, 2, 1, 2, 3, 2, 3, 4, 3, 4, 5, 4, 5, 2, 0, 3, 1, 4, 2)
+
+
+# ---------------------------------------------------------------- device learner
+class Learner:
+    def __init__(self, params):
+        self.L = _sigs()
+        self.h = C.c_void_p()
+        p = np.ascontiguousarray(params, np.float32)
+        if self.L.av1mi_ln_create(p.ctypes.data, C.byref(self.h)) != 0:
+            raise RuntimeError("av1mi_ln_create failed")
+        self.n = 0
+
+    def params(self):
+        p = np.zeros(param_count(), np.float32)
+        if self.L.av1mi_ln_get_params(self.h, p.ctypes.data) != 0:
+            raise RuntimeError("av1mi_ln_get_params failed")
+        return p
+
+    def forward(self, tiles, q):
+        tiles = np.ascontiguousarray(tiles, np.uint8)
+        q = np.ascontiguousarray(q, np.uint8)
+        out = np.zeros((len(tiles), N_OUT), np.float32)
+        if self.L.av1mi_ln_forward(self.h, tiles.ctypes.data, q.ctypes.data, len(tiles), out.ctypes.data) != 0:
+            raise RuntimeError("av1mi_ln_forward failed")
+        return out
+
+    def activations(self, layer, n):
+        per = (64 * 64 * 2, 32 * 32 * 32, 16 * 16 * 32, 8 * 8 * 32, N_FEAT)[layer]
+        a = np.zeros(n * per, np.uint16)
+        if self.L.av1mi_ln_activations(self.h, layer, n, a.ctypes.data) != 0:
+            raise RuntimeError("av1mi_ln_activations failed")
+        return (a.astype(np.uint32) << 16).view(np.float32)
+
+    def set_data(self, tiles, q, label, lmask):
+        self._keep = [np.ascontiguousarray(tiles, np.uint8), np.ascontiguousarray(q, np.uint8),
+                      np.ascontiguousarray(label, np.float32), np.ascontiguousarray(lmask, np.float32)]
+        t, qq, lb, lm = self._keep
+        if self.L.av1mi_ln_set_data(self.h, t.ctypes.data, qq.ctypes.data, lb.ctypes.data, lm.ctypes.data, len(t)) != 0:
+            raise RuntimeError("av1mi_ln_set_data failed")
+        self.n = len(t)
+
+    def gradient(self, first, n):
+        g = np.zeros(param_count(), np.float32)
+        loss = C.c_float()
+        if self.L.av1mi_ln_gradient(self.h, first, n, C.byref(loss), g.ctypes.data) != 0:
+            raise RuntimeError("av1mi_ln_gradient failed")
+        return loss.value, g
+
+    def step(self, first, n, lr, wd=1e-5):
+        loss = C.c_float()
+        if self.L.av1mi_ln_train_step(self.h, first, n, lr, wd, C.byref(loss)) != 0:
+            raise RuntimeError("av1mi_ln_train_step failed")
+        return loss.value
+
+    def close(self):
+        if self.h:
+            self.L.av1mi_ln_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def set_shortlist(enc, params, confidence=0.8):
+    """Enable (params) or disable (None) the learned pruning of an encoder."""
+    L = _sigs()
+    err = C.create_string_buffer(256)
+    p = None if params is None else np.ascontiguousarray(params, np.float32)
+    if L.av1mi_set_shortlist(enc._h, None if p is None else p.ctypes.data, confidence, err, 256) != 0:
+        raise RuntimeError("set_shortlist: " + err.value.decode())
+
+
+# ---------------------------------------------------------------- float64 restatement (checks)
+def bf16(x):
+    """Round float32 values to BF16 (nearest even), returned as float32."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+def input_tensor(tiles, q):
+    t = np.asarray(tiles, np.float32).reshape(-1, 64, 64)
+    x0 = np.empty((len(t), 64, 64, 2), np.float32)
+    x0[..., 0] = bf16(t * np.float32(1.0 / 255.0) - np.float32(0.5))
+    x0[..., 1] = bf16(np.asarray(q, np.float32) / np.float32(255.0))[:, None, None]
+    return x0
+
+
+def _im2col(x, cin):
+    """x [n, H, H, cin] -> [n, H/2 * H/2, 9 cin] for the 3x3 stride-2 pad-1 convolution (tap-major, then channel)."""
+    n, H = x.shape[0], x.shape[1]
+    Ho = H // 2
+    xp = np.zeros((n, H + 2, H + 2, cin), x.dtype)
+    xp[:, 1:H + 1, 1:H + 1] = x
+    cols = [xp[:, ky:ky + 2 * Ho:2, kx:kx + 2 * Ho:2, :] for ky in range(3) for kx in range(3)]
+    return np.stack(cols, axis=3).reshape(n, Ho * Ho, 9 * cin)
+
+
+def ref_conv(x, w, b, l):
+    """One convolution + ReLU in float64 on BF16 inputs and weights; output rounded to BF16 as the kernel stores it."""
+    K = 9 * CIN[l]
+    col = _im2col(x.astype(np.float64), CIN[l])
+    wt = bf16(w[:, :K]).astype(np.float64)
+    z = col @ wt.T + b.astype(np.float64)
+    H = x.shape[1] // 2
+    return bf16(np.maximum(z, 0.0).astype(np.float32)).reshape(x.shape[0], H, H, 32)
+
+
+def ref_fc(a4, w5, b5):
+    return a4.reshape(len(a4), -1).astype(np.float64) @ w5.astype(np.float64).T + b5
+
+
+def ref_grads(params, acts, label, lmask):
+    """Float64 gradient of the mean masked BCE, backpropagated through the given (kernel) activations
+    acts = [x0, a1, a2, a3, a4] (NHWC float32 holding BF16 values)."""
+    convs, w5, b5 = unpack(params)
+    n = len(label)
+    z = ref_fc(acts[4], w5, b5)
+    s = 1.0 / (1.0 + np.exp(-z))
+    dlog = lmask * (s - label) / n
+    g = np.zeros(param_count(), np.float64)
+    gconv, gw5, gb5 = unpack(g)
+    a4 = acts[4].reshape(n, -1).astype(np.float64)
+    gw5[:] = dlog.T @ a4
+    gb5[:] = dlog.sum(0)
+    d = (dlog @ w5.astype(np.float64)) * (a4 > 0)
+    d = d.reshape(n, 4, 4, 32)
+    for l in (3, 2, 1, 0):
+        x = acts[l].astype(np.float64)
+        K = 9 * CIN[l]
+        col = _im2col(x, CIN[l])
+        dz = d.reshape(n, -1, 32)
+        gconv[l][0][:, :K] = np.einsum("npc,npk->ck", dz, col)
+        gconv[l][1][:] = dz.sum((0, 1))
+        if l > 0:
+            wk = convs[l][0][:, :K].astype(np.float64)
+            dcol = dz @ wk                                   # [n, P, 9 cin]
+            H = x.shape[1]
+            Ho = H // 2
+            dx = np.zeros((n, H + 2, H + 2, CIN[l]))
+            dcol = dcol.reshape(n, Ho, Ho, 9, CIN[l])
+            for t in range(9):
+                ky, kx = t // 3, t % 3
+                dx[:, ky:ky + 2 * Ho:2, kx:kx + 2 * Ho:2, :] += dcol[:, :, :, t, :]
+            d = dx[:, 1:H + 1, 1:H + 1] * (x > 0)
+    loss = float(np.sum(lmask * (np.maximum(z, 0) - z * label + np.log1p(np.exp(-np.abs(z))))) / n)
+    return loss, g
+
+
+# ---------------------------------------------------------------- rate / distortion
+def psnr_y(src, rec, W, H, bd=8):
+    d = src[:H, :W].astype(np.float64) - rec[:H, :W].astype(np.float64)
+    mse = float(np.mean(d * d))
+    peak = (1 << bd) - 1
+    return 99.0 if mse == 0 else 10.0 * math.log10(peak * peak / mse)
+
+
+def bd_rate(r_a, p_a, r_b, p_b):
+    """Bjontegaard delta rate (percent) of curve b against curve a: cubic fits of log10(rate) over PSNR, averaged over
+    the PSNR range both curves cover (positive: b needs more bits for the same quality)."""
+    la, lb = np.log10(np.asarray(r_a, float)), np.log10(np.asarray(r_b, float))
+    pa, pb = np.asarray(p_a, float), np.asarray(p_b, float)
+    fa, fb = np.polyfit(pa, la, 3), np.polyfit(pb, lb, 3)
+    lo, hi = max(pa.min(), pb.min()), min(pa.max(), pb.max())
+    ia, ib = np.polyint(fa), np.polyint(fb)
+    avg = ((np.polyval(ib, hi) - np.polyval(ib, lo)) - (np.polyval(ia, hi) - np.polyval(ia, lo))) / (hi - lo)
+    return (10.0 ** avg - 1.0) * 100.0

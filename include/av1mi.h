@@ -112,6 +112,24 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
                  int want_recon, char *err, int errlen);
 void av1mi_frame_out_free(av1mi_frame_out *out);
 
+/* Learned candidate pruning (no counterpart in the reference, whose pruning stages are heuristics:
+ * encoder/src/k_leafprep.cu, an_stage.h, gp.h:136-143).  A BF16 CNN per 64x64 superblock predicts the partition nodes
+ * whose children the analysis needs and a top-8 luma-mode shortlist; below the confidence the search stays exhaustive
+ * (per node, and for the shortlist per superblock).  params: av1mi_ln_param_count() floats (NULL disables). */
+int av1mi_set_shortlist(av1mi_encoder *enc, const float *params, float confidence, char *err, int errlen);
+size_t av1mi_ln_param_count(void);
+void av1mi_ln_init_params(float *params, uint64_t seed);
+/* Training on the device with this library's own forward (MFMA) and backward kernels. */
+typedef struct av1mi_learner av1mi_learner;
+int av1mi_ln_create(const float *params, av1mi_learner **out);
+void av1mi_ln_destroy(av1mi_learner *l);
+int av1mi_ln_get_params(av1mi_learner *l, float *params);
+int av1mi_ln_forward(av1mi_learner *l, const uint8_t *tiles, const uint8_t *q, uint32_t n, float *logits);
+int av1mi_ln_activations(av1mi_learner *l, int layer, uint32_t n, uint16_t *out);
+int av1mi_ln_set_data(av1mi_learner *l, const uint8_t *tiles, const uint8_t *q, const float *label, const float *lmask, uint32_t n);
+int av1mi_ln_gradient(av1mi_learner *l, uint32_t first, uint32_t n, float *loss, float *grads);
+int av1mi_ln_train_step(av1mi_learner *l, uint32_t first, uint32_t n, float lr, float weight_decay, float *loss);
+
 /* The bitstream of a record (the reference's host writer, av1core/src/encoder.cpp:1072-1659 with avif.cpp):
  * av1mi_write_obus gives the sequence header and frame OBUs of the still picture (default CDFs, no post-filters),
  * av1mi_write_avif the same wrapped in an AVIF file.  edge_filter: enable_intra_edge_filter of the encode.  *data is

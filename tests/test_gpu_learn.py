@@ -1,0 +1,154 @@
+"""The pruning network's kernels against the float64 NumPy restatement (av1gpu_amd/learn.py), layer by layer: the MFMA
+convolutions and the FC head (forward), the backward kernels (gradients), Adam training lowering the loss, and the
+encoder integration (a network that expands every node and keeps every mode gives the exhaustive records bit for bit;
+one that prunes gives a record the reference's backend decodes to the GPU reconstruction)."""
+import numpy as np
+import pytest
+
+import av1gpu_amd as A
+from av1gpu_amd import learn as Ln
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(n_img=2, seed=3000):
+    tiles, qs = [], []
+    for i in range(n_img):
+        W, H = 256, 192
+        y = A.synth_image(seed + i, W, H)[0]
+        t = Ln.sb_tiles(y, W, H)
+        tiles.append(t)
+        qs.append(np.full(len(t), (80, 160)[i % 2], np.uint8))
+    return np.concatenate(tiles), np.concatenate(qs)
+
+
+def test_forward_matches_reference():
+    p = Ln.init_params(11)
+    tiles, q = _data()
+    L = Ln.Learner(p)
+    logits = L.forward(tiles, q)
+    n = len(tiles)
+    acts = [L.activations(k, n) for k in range(5)]
+    x0 = Ln.input_tensor(tiles, q)
+    assert np.array_equal(acts[0].reshape(x0.shape), x0)
+    convs, w5, b5 = Ln.unpack(p)
+    shapes = [(n, 64, 64, 2), (n, 32, 32, 32), (n, 16, 16, 32), (n, 8, 8, 32), (n, 4, 4, 32)]
+    for l in range(4):
+        inp = acts[l].reshape(shapes[l])
+        ref = Ln.ref_conv(inp, convs[l][0], convs[l][1], l)
+        got = acts[l + 1].reshape(shapes[l + 1])
+        # FP32 accumulation in another order than float64: outputs agree to BF16 rounding (one step at most)
+        assert np.allclose(got, ref, rtol=2 ** -7, atol=1e-4), "layer %d" % (l + 1)
+        assert np.mean(got != ref) < 0.02
+    zr = Ln.ref_fc(acts[4].reshape(n, -1), w5, b5)
+    assert np.allclose(logits, zr, rtol=1e-4, atol=1e-4)
+    L.close()
+
+
+def test_gradient_matches_reference():
+    p = Ln.init_params(12)
+    tiles, q = _data()
+    rng = np.random.default_rng(5)
+    lab = (rng.uniform(size=(len(tiles), Ln.N_OUT)) < 0.4).astype(np.float32)
+    msk = (rng.uniform(size=(len(tiles), Ln.N_OUT)) < 0.9).astype(np.float32)
+    L = Ln.Learner(p)
+    L.set_data(tiles, q, lab, msk)
+    loss, g = L.gradient(0, len(tiles))
+    acts = [L.activations(k, len(tiles)) for k in range(5)]
+    shapes = [(64, 64, 2), (32, 32, 32), (16, 16, 32), (8, 8, 32), (4, 4, 32)]
+    acts = [a.reshape((len(tiles),) + s) for a, s in zip(acts, shapes)]
+    lr, gr = Ln.ref_grads(p, acts, lab, msk)
+    assert abs(loss - lr) < 1e-4 * max(1.0, abs(lr))
+    gc, gw5, gb5 = Ln.unpack(g.astype(np.float64))
+    rc, rw5, rb5 = Ln.unpack(gr)
+    for a, b, name in [(gw5, rw5, "fc W"), (gb5, rb5, "fc b")] + [(gc[l][0], rc[l][0], "conv%d W" % l) for l in range(4)] + \
+            [(gc[l][1], rc[l][1], "conv%d b" % l) for l in range(4)]:
+        assert np.max(np.abs(a - b)) <= 1e-3 * np.max(np.abs(b)) + 1e-7, name
+    L.close()
+
+
+def test_training_lowers_the_loss():
+    tiles, q = _data(4)
+    rng = np.random.default_rng(6)
+    # a learnable target: the node outputs follow the tile's variance, the modes a fixed pattern
+    var = tiles.reshape(len(tiles), -1).astype(np.float64).std(1)
+    lab = np.zeros((len(tiles), Ln.N_OUT), np.float32)
+    lab[:, :Ln.N_SPLIT] = (var > np.median(var))[:, None]
+    lab[:, Ln.N_SPLIT:] = rng.dirichlet(np.ones(13))
+    L = Ln.Learner(Ln.init_params(13))
+    L.set_data(tiles, q, lab, np.ones_like(lab))
+    first = L.step(0, len(tiles), 3e-3)
+    for _ in range(60):
+        last = L.step(0, len(tiles), 3e-3)
+    assert last < 0.8 * first
+    L.close()
+
+
+def _params_bias(split_bias, mode_bias):
+    p = Ln.init_params(1)
+    convs, w5, b5 = Ln.unpack(p)
+    w5[:] = 0.0
+    b5[:Ln.N_SPLIT] = split_bias
+    b5[Ln.N_SPLIT:] = mode_bias
+    return p
+
+
+def test_shortlist_that_keeps_everything_is_exhaustive():
+    W, H = 264, 200
+    f = A.DeviceFrame(A.synth_image(1006, W, H), W, H, qindex=120)
+    enc = A.GpEncoder().init(A.GpConfig(qindex=120))
+    base = enc.encode([f])[0][0]
+    Ln.set_shortlist(enc, _params_bias(20.0, np.zeros(13)), 0.8)   # every node expanded, uniform modes: no shortlist
+    got = enc.encode([f])[0][0]
+    for k in ("blocks", "txb", "coeffs"):
+        assert np.array_equal(getattr(base, k), getattr(got, k)), k
+    for p in range(3):
+        assert np.array_equal(base.recon[p], got.recon[p])
+    enc.close()
+    f.close()
+
+
+def test_pruned_record_decodes(ref):
+    W, H = 264, 200
+    f = A.DeviceFrame(A.synth_image(1007, W, H), W, H, qindex=120)
+    enc = A.GpEncoder().init(A.GpConfig(qindex=120))
+    base = enc.encode([f])[0][0]
+    mb = np.full(13, -10.0)
+    mb[[0, 1, 2, 9, 10, 11, 12, 3]] = 10.0                          # 8 modes kept
+    Ln.set_shortlist(enc, _params_bias(-20.0, mb), 0.8)             # nothing expanded below 64x64 where not forced
+    g = enc.encode([f])[0][0]
+    assert len(g.blocks) < len(base.blocks) and set(np.unique(g.blocks["y_mode"])) <= {0, 1, 2, 3, 9, 10, 11, 12}
+    dec, _, _ = ref.encode(dict(blocks=g.blocks, block_txb=g.block_txb, txb=g.txb, coeffs=g.coeffs), W, H, 120)
+    for p in range(3):
+        assert np.array_equal(g.recon[p][:dec[p].shape[0], :dec[p].shape[1]], dec[p])
+    enc.close()
+    f.close()
+
+
+def test_network_chunks_and_batching_do_not_change_records(monkeypatch):
+    """The masks of a superblock depend on its own pixels only: a batch of mixed-size images run through the network 7
+    superblocks at a time (chunks straddle images) gives each image the record it gets alone in one chunk."""
+    p = Ln.init_params(21)
+    convs, w5, b5 = Ln.unpack(p)
+    w5 *= 4.0                                                       # spread the logits so the masks vary per superblock
+    sizes = [(264, 200), (136, 72), (328, 264), (64, 64)]
+    frames = [A.DeviceFrame(A.synth_image(1100 + i, W, H), W, H, qindex=(80, 120, 160, 120)[i]) for i, (W, H) in enumerate(sizes)]
+    solo = A.GpEncoder().init(A.GpConfig(qindex=120))
+    Ln.set_shortlist(solo, p, 0.8)
+    alone = [solo.encode([f])[0][0] for f in frames]
+    monkeypatch.setenv("AV1MI_LN_CHUNK", "7")
+    batch = A.GpEncoder().init(A.GpConfig(qindex=120))
+    Ln.set_shortlist(batch, p, 0.8)
+    out = batch.encode(frames)[0]
+    full = A.GpEncoder().init(A.GpConfig(qindex=120))
+    exh = full.encode(frames)[0]
+    pruned_any = False
+    for a, b, e in zip(alone, out, exh):
+        for k in ("blocks", "txb", "coeffs"):
+            assert np.array_equal(getattr(a, k), getattr(b, k)), k
+        pruned_any |= len(a.blocks) != len(e.blocks) or not np.array_equal(a.blocks, e.blocks)
+    assert pruned_any
+    for e in (solo, batch, full):
+        e.close()
+    for f in frames:
+        f.close()
