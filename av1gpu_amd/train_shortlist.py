@@ -73,7 +73,7 @@ def train(params, data, steps, batch, lr, log=print, seed=7):
     for s in range(steps):
         cur = lr * 0.5 * (1.0 + math.cos(math.pi * s / steps))
         losses.append(L.step((s % nb) * batch, batch, cur))
-        if s % 500 == 0 or s == steps - 1:
+        if s % 100 == 0 or s == steps - 1:
             log("step %d loss %.4f (%.0f s)" % (s, float(np.mean(losses[-100:])), time.time() - t0))
     p = L.params()
     L.close()
@@ -106,6 +106,7 @@ def evaluate(params, n_img=12, seed=30000, qs=(60, 100, 140, 180), conf=0.8, log
                 f.close()
             rates.append(bits)
             psnrs.append(float(np.mean(mse_w)))
+            log("eval %s q%d: %d bits, PSNR-Y %.3f dB, %.3f s" % (name, q, bits, psnrs[-1], secs))
         enc.close()
         res[name] = dict(bits=rates, psnr=psnrs, seconds=secs, blocks=blocks)
     mp = sum(W * H for W, H, _, _ in spec) * len(qs) / 1e6
