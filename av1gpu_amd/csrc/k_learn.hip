@@ -7,7 +7,7 @@
 // Network (per superblock):  x0 = [64 x 64 luma / max - 0.5, qindex / 255]  (NHWC, 2 channels, BF16)
 //   L1..L4: 3x3 convolution, stride 2, zero padding 1, 32 output channels, bias, ReLU   (64 -> 32 -> 16 -> 8 -> 4)
 //   L5: fully connected 512 -> 98 logits: 85 "expand" logits (node 0: the 64x64 node, 1-4: 32x32 nodes, 5-20: 16x16,
-//       21-84: 8x8, raster order per level) and 13 luma-mode logits (each mode's share of the superblock's area).
+//       21-84: 8x8, raster order per level) and 13 luma-mode logits (the mode is used by some block of the superblock).
 // Convolutions are implicit GEMMs on v_mfma_f32_16x16x16_bf16 (positions x input taps times taps x channels), weights
 // and activations BF16, accumulation and biases FP32.  Training runs the same forward kernels, then FP32 backward
 // kernels (convolution weight gradients accumulated with float atomics) and Adam on FP32 master weights.
@@ -118,7 +118,8 @@ __global__ void __launch_bounds__(128) k_ln_fc(const bf16_t *a4, const float *W,
 // ---- decisions: per superblock the nodes whose children are analysed and the luma modes screened ----
 // A node is expanded unless the network is confident (p <= 1 - conf) that the search would not use its children, and
 // always where it reaches past the frame (the partition syntax may force a split there).  The modes are the 8 most
-// likely when they carry at least conf of the total probability, else all 13.  mask: 3 words of node bits, 1 of modes.
+// likely when the network is confident that none of the other 5 is used (product of their 1 - p at least conf), else
+// all 13.  mask: 3 words of node bits, 1 of modes.
 // logits of slots 0 .. n - 1 belong to superblocks s0 .. s0 + n - 1 of the batch; mask is indexed by superblock
 __global__ void __launch_bounds__(64) k_ln_decide(const float *logits, const Img *imgs, const uint32_t *sbImg, uint32_t s0, uint32_t n,
                                                   float conf, uint32_t *mask) {
@@ -140,17 +141,17 @@ __global__ void __launch_bounds__(64) k_ln_decide(const float *logits, const Img
                 if (edge || p > 1.f - conf) m[node >> 5] |= 1u << (node & 31);
             }
     }
-    float p[13], tot = 0.f;
-    for (int k = 0; k < 13; k++) { p[k] = 1.f / (1.f + expf(-z[LN_SPLIT + k])); tot += p[k]; }
+    float p[13];
+    for (int k = 0; k < 13; k++) p[k] = 1.f / (1.f + expf(-z[LN_SPLIT + k]));
     uint32_t top = 0;
-    float mass = 0.f;
     for (int r = 0; r < 8; r++) {
         int b = -1;
         for (int k = 0; k < 13; k++) if (!((top >> k) & 1) && (b < 0 || p[k] > p[b])) b = k;
         top |= 1u << b;
-        mass += p[b];
     }
-    m[3] = mass >= conf * tot ? top : 0x1FFFu;
+    float none = 1.f;   // probability that no mode outside the shortlist is used
+    for (int k = 0; k < 13; k++) if (!((top >> k) & 1)) none *= 1.f - p[k];
+    m[3] = none >= conf ? top : 0x1FFFu;
     for (int k = 0; k < 4; k++) mask[(size_t)sb * 4 + k] = m[k];
 }
 

@@ -4,9 +4,10 @@ The network (av1gpu_amd/csrc/k_learn.hip) sees the 64x64 luma of a superblock an
   * 85 "expand" outputs, one per square partition node (the 64x64 node, four 32x32, sixteen 16x16, sixty-four 8x8, raster
     order per level): 1 when the exhaustive search's leaves inside the node are at most half its size in both dimensions
     (its children are used);
-  * 13 luma-mode outputs: the share of the superblock's area each luma mode covers in the exhaustive result.
-The encoder then analyses a node's children only where the network is not confident (1 - conf) that they are unused,
-and screens the 8 most likely modes only where they carry at least conf of the probability mass.
+  * 13 luma-mode outputs: 1 when some block of the superblock uses that luma mode in the exhaustive result.
+The encoder then analyses a node's children unless the network is confident (probability of use at most 1 - conf) that
+they are unused, and screens only the 8 most likely modes where it is confident (the product of 1 - p over the 5 others
+at least conf) that none of the others is used.
 
 Training data are the exhaustive search's own decisions on seeded synthetic images; training runs on the GPU with the
 library's forward (MFMA) and backward kernels.  ``ref_conv`` / ``ref_fc`` / ``ref_grads`` restate the same network in
@@ -15,6 +16,7 @@ them.
 """
 import ctypes as C
 import math
+import os
 
 import numpy as np
 
@@ -105,11 +107,8 @@ def sb_labels(blocks, W, H):
                 sxs = (np.arange(sbr * sbc) % sbc) * 64 + nx * n
                 sys = (np.arange(sbr * sbc) // sbc) * 64 + ny * n
                 msk[(sxs + n > FW) | (sys + n > FH), nd] = 0.0
-    # modes: the share of the superblock's (visible) area each luma mode covers
-    area = (np.minimum(x + bw, FW) - x) * (np.minimum(y + bh, FH) - y)
-    np.add.at(lab, (sb, N_SPLIT + blocks["y_mode"].astype(np.int64)), area.astype(np.float32))
-    tot = lab[:, N_SPLIT:].sum(1, keepdims=True)
-    lab[:, N_SPLIT:] /= np.maximum(tot, 1.0)
+    # modes: used by some block of the superblock
+    lab[sb, N_SPLIT + blocks["y_mode"].astype(np.int64)] = 1.0
     return lab, msk
 
 
@@ -179,6 +178,45 @@ class Learner:
             self.close()
         except Exception:
             pass
+
+
+def decide(logits, W, H, conf):
+    """The decisions the encoder takes from the logits of an image's superblocks (raster order): expand [nSb, 85] bool
+    (a node's children are analysed) and the luma-mode mask [nSb] (bit m: mode m screened)."""
+    z = np.asarray(logits, np.float64)
+    MiCols, MiRows = 2 * ((W + 7) >> 3), 2 * ((H + 7) >> 3)
+    FW, FH = 4 * MiCols, 4 * MiRows
+    sbc = (MiCols + 15) >> 4
+    idx = np.arange(len(z))
+    sx, sy = (idx % sbc) * 64, (idx // sbc) * 64
+    p = 1.0 / (1.0 + np.exp(-z))
+    expand = np.zeros((len(z), N_SPLIT), bool)
+    for lv in range(4):
+        n, per = 64 >> lv, 1 << lv
+        for ny in range(per):
+            for nx in range(per):
+                nd = LEVEL_BASE[lv] + ny * per + nx
+                edge = (sx + nx * n + n > FW) | (sy + ny * n + n > FH)
+                expand[:, nd] = edge | (p[:, nd] > 1.0 - conf)
+    pm = p[:, N_SPLIT:]
+    order = np.argsort(-pm, axis=1, kind="stable")      # ties: the lower mode first
+    top, rest = order[:, :8], order[:, 8:]
+    none = np.prod(1.0 - np.take_along_axis(pm, rest, 1), axis=1)
+    mask = np.full(len(z), 0x1FFF, np.int64)
+    bits = np.sum(1 << top, axis=1)
+    mask[none >= conf] = bits[none >= conf]
+    return expand, mask
+
+
+PARAMS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "shortlist_params.npy")
+
+
+def load_params(path=PARAMS_FILE):
+    """The trained network parameters shipped with the package (written by av1gpu_amd.train_shortlist)."""
+    p = np.load(path, allow_pickle=False).astype(np.float32)
+    if p.shape != (param_count(),):
+        raise ValueError("%s holds %s parameters, the network has %d" % (path, p.shape, param_count()))
+    return p
 
 
 def set_shortlist(enc, params, confidence=0.8):
@@ -273,6 +311,26 @@ def psnr_y(src, rec, W, H, bd=8):
     mse = float(np.mean(d * d))
     peak = (1 << bd) - 1
     return 99.0 if mse == 0 else 10.0 * math.log10(peak * peak / mse)
+
+
+def rd_curve(enc, images, qs):
+    """Rate-distortion points of an encoder over images [(planes, W, H)] at each qindex of qs: total bits of this
+    build's own bitstream writer (bare OBUs), mean PSNR-Y, and the encode seconds (one call per qindex, the planes
+    uploaded beforehand, one untimed warm-up call on the first image)."""
+    import time
+    bits, psnr, secs = [], [], 0.0
+    for q in qs:
+        frames = [A.DeviceFrame(p, W, H, qindex=q) for p, W, H in images]
+        enc.encode(frames[:1])
+        A.lib().av1mi_device_sync()
+        t0 = time.perf_counter()
+        out, _ = enc.encode(frames)
+        secs += time.perf_counter() - t0
+        bits.append(sum(8 * len(A.write_bitstream(o, avif=False)) for o in out))
+        psnr.append(float(np.mean([psnr_y(p[0], o.recon[0], W, H) for (p, W, H), o in zip(images, out)])))
+        for f in frames:
+            f.close()
+    return bits, psnr, secs
 
 
 def bd_rate(r_a, p_a, r_b, p_b):

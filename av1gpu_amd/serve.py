@@ -35,7 +35,9 @@ class Request:
 
 
 class Server:
-    def __init__(self, workers=4, hbm_budget=64 << 30, max_batch=32, qindex=120, params=None):
+    def __init__(self, workers=4, hbm_budget=64 << 30, max_batch=32, qindex=120, params=None, shortlist=None):
+        """params: GpParams of the search; shortlist: (network parameters, confidence) of the learned pruning, or None
+        for the exhaustive search."""
         self.workers, self.budget, self.max_batch = workers, hbm_budget, max_batch
         self._pending = deque()
         self._lock = threading.Condition()
@@ -56,6 +58,9 @@ class Server:
             if params is not None:
                 cfg.P = params
             enc = A.GpEncoder().init(cfg)
+            if shortlist is not None:
+                from av1gpu_amd import learn
+                learn.set_shortlist(enc, shortlist[0], shortlist[1])
             self._encoders.append((s.value, enc))
             t = threading.Thread(target=self._run, args=(w,), daemon=True)
             self._threads.append(t)

@@ -1,6 +1,6 @@
 """Train the learned pruning network on the GPU and evaluate it against the exhaustive search.
 
-  python -m av1gpu_amd.train_shortlist [--sbs 1000000] [--steps 6000] [--out av1gpu_amd/shortlist_params.npy]
+  python -m av1gpu_amd.train_shortlist [--sbs 2000000] [--steps 60000] [--train-budget SECONDS] [--out av1gpu_amd/shortlist_params.npy]
 
 1. Labels: seeded synthetic images (sizes log-uniform over 0.2-4 MP, aspect 0.5-2, qindex 80/120/160, seeds from
    20000) are encoded with the exhaustive search; every superblock's leaves give its 85 node labels and 13 mode shares
@@ -61,7 +61,8 @@ def labelled_data(n_sbs, seed=20000, log=print):
     return np.concatenate(tiles), np.concatenate(qs), np.concatenate(labs), np.concatenate(msks)
 
 
-def train(params, data, steps, batch, lr, log=print, seed=7):
+def train(params, data, steps, batch, lr, log=print, seed=7, budget=None):
+    """budget (seconds): after 20 steps the step count is cut to what fits the budget at the measured step time."""
     tiles, q, lab, msk = data
     rng = np.random.default_rng(seed)
     perm = rng.permutation(len(tiles))   # one shuffle; batches are contiguous slices of the shuffled set
@@ -70,68 +71,60 @@ def train(params, data, steps, batch, lr, log=print, seed=7):
     nb = len(tiles) // batch
     losses = []
     t0 = time.time()
-    for s in range(steps):
+    s = 0
+    while s < steps:
+        if budget and s == 20:
+            steps = max(21, min(steps, int(budget * 20 / (time.time() - t0))))
+            log("%d steps fit the budget of %.0f s" % (steps, budget))
         cur = lr * 0.5 * (1.0 + math.cos(math.pi * s / steps))
         losses.append(L.step((s % nb) * batch, batch, cur))
         if s % 100 == 0 or s == steps - 1:
             log("step %d loss %.4f (%.0f s)" % (s, float(np.mean(losses[-100:])), time.time() - t0))
+        s += 1
     p = L.params()
     L.close()
     return p, losses
 
 
-def evaluate(params, n_img=12, seed=30000, qs=(60, 100, 140, 180), conf=0.8, log=print):
-    """BD-rate (PSNR-Y) of the pruned search against the exhaustive one, per-image curves summed, and timings."""
+def evaluate(params, n_img=12, seed=30000, qs=(60, 100, 140, 180), confs=(0.8, 0.9, 0.95), log=print):
+    """BD-rate (PSNR-Y) of the pruned search at each confidence against the exhaustive one (rates summed over the
+    images, PSNR averaged) and encode times (the network included)."""
     spec = image_spec(n_img, seed, 0.5, 2.0, qs=(120,))
-    planes = [A.synth_image(s, W, H) for W, H, _, s in spec]
+    images = [(A.synth_image(s, W, H), W, H) for W, H, _, s in spec]
     res = {}
-    for name, pr in (("exhaustive", None), ("pruned", params)):
+    for name, conf in [("exhaustive", None)] + [("pruned@%g" % c, c) for c in confs]:
         enc = A.GpEncoder().init(A.GpConfig(qindex=120))
-        if pr is not None:
-            Ln.set_shortlist(enc, pr, conf)
-        rates, psnrs, secs, blocks = [], [], 0.0, 0
-        for q in qs:
-            frames = [A.DeviceFrame(p, W, H, qindex=q) for p, (W, H, _, _) in zip(planes, spec)]
-            enc.encode(frames[:2])
-            A.lib().av1mi_device_sync()
-            t0 = time.perf_counter()
-            out, t = enc.encode(frames)
-            secs += time.perf_counter() - t0
-            bits, mse_w = 0, []
-            for p, (W, H, _, _), o in zip(planes, spec, out):
-                bits += 8 * len(A.write_bitstream(o, avif=False))
-                mse_w.append(Ln.psnr_y(p[0], o.recon[0], W, H))
-                blocks += len(o.blocks)
-            for f in frames:
-                f.close()
-            rates.append(bits)
-            psnrs.append(float(np.mean(mse_w)))
-            log("eval %s q%d: %d bits, PSNR-Y %.3f dB, %.3f s" % (name, q, bits, psnrs[-1], secs))
+        if conf is not None:
+            Ln.set_shortlist(enc, params, conf)
+        bits, psnr, secs = Ln.rd_curve(enc, images, qs)
         enc.close()
-        res[name] = dict(bits=rates, psnr=psnrs, seconds=secs, blocks=blocks)
+        res[name] = dict(bits=bits, psnr=psnr, seconds=round(secs, 4))
+        log("eval %s: bits %s, PSNR-Y %s, %.3f s" % (name, bits, ["%.3f" % v for v in psnr], secs))
+    ex = res["exhaustive"]
+    for name, r in res.items():
+        if name != "exhaustive":
+            r["bd_rate_psnr_y"] = round(float(Ln.bd_rate(ex["bits"], ex["psnr"], r["bits"], r["psnr"])), 3)
+            r["speedup"] = round(ex["seconds"] / r["seconds"], 3)
+            log("%s: BD-rate %.3f %%, speed-up %.2fx" % (name, r["bd_rate_psnr_y"], r["speedup"]))
     mp = sum(W * H for W, H, _, _ in spec) * len(qs) / 1e6
-    bd = Ln.bd_rate(res["exhaustive"]["bits"], res["exhaustive"]["psnr"], res["pruned"]["bits"], res["pruned"]["psnr"])
-    out = dict(bd_rate_psnr_y=round(float(bd), 3), qindex=list(qs), images=n_img, megapixels=round(mp, 2),
-               exhaustive=res["exhaustive"], pruned=res["pruned"],
-               speedup=round(res["exhaustive"]["seconds"] / res["pruned"]["seconds"], 3), confidence=conf)
-    log("BD-rate %.3f %%, speed-up %.2fx" % (bd, out["speedup"]))
-    return out
+    return dict(qindex=list(qs), images=n_img, megapixels=round(mp, 2), curves=res)
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--sbs", type=int, default=1000000)
-    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--sbs", type=int, default=2000000)
+    ap.add_argument("--steps", type=int, default=60000)
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--train-budget", type=float, default=None, help="seconds of training at most")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "shortlist_params.npy"))
     a = ap.parse_args()
     log = lambda m: print(m, flush=True)
     data = labelled_data(a.sbs, log=log)
-    params, losses = train(Ln.init_params(1), data, a.steps, a.batch, a.lr, log=log)
+    params, losses = train(Ln.init_params(1), data, a.steps, a.batch, a.lr, log=log, budget=a.train_budget)
     np.save(a.out, params)
     ev = evaluate(params, log=log)
-    summary = dict(superblocks=int(len(data[0])), steps=a.steps, batch=a.batch, lr=a.lr,
+    summary = dict(superblocks=int(len(data[0])), steps=len(losses), batch=a.batch, lr=a.lr,
                    loss_first=float(np.mean(losses[:50])), loss_last=float(np.mean(losses[-50:])), eval=ev)
     with open(os.path.splitext(a.out)[0] + ".json", "w") as f:
         json.dump(summary, f, indent=1)

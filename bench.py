@@ -1,15 +1,22 @@
 """Benchmark of the intra RD core on one MI355X.
 
-A step is one encode (analysis -> partition DP -> closed-loop reconstruction -> pack -> records and reconstruction on
-the host) of the workload with its input planes already resident in device memory.  Default workload: the plain
-exhaustive search on one 4000 x 3000 8-bit 4:2:0 synthetic image (seed 1000) at qindex 120.  Prints one JSON line.
+Headline (value): the batched+learned path.  A step is one encode call (analysis with the learned pruning network ->
+partition DP -> closed-loop reconstruction -> pack -> records on the host) over the mixed-size batch: 256 synthetic
+8-bit 4:2:0 images of 0.18-24 MP, aspect 0.5-2, qindex 80/120/160 (av1gpu_amd.batch_spec, seeds 1000..1255), planes
+resident in device memory, the shipped network (av1gpu_amd/shortlist_params.npy) at confidence 0.8.  Beside it:
+  plain_single_image   the plain exhaustive search on one 4000 x 3000 image (seed 1000) at qindex 120, the path the
+                       ratio vs_plain_single_image divides by;
+  batched_exhaustive   the same batch without the network;
+  bd_rate              BD-rate (PSNR-Y, bits of this build's own writer) of the learned search against the exhaustive
+                       one on the batch's images of at most 2 MP (first 16), qindex 60/100/140/180;
+  serving              Poisson arrivals into the continuous-batching server on the batched+learned path, p50 / p99;
+  cpu_baseline         the CPU restatement (oracle/liborc.so) on a bounded crop of the 4000 x 3000 image.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--batch 256] [--serve 64] [--conf 0.8]
 
-roofline: the dominant kernel's (the closed loop's) algorithmic HBM bytes (source planes read once + reconstruction
-written once + 2 bytes per coefficient + 16 bytes per block and tx-block record) per launch over its average launch
-duration, timed with HIP events on the encode stream.  cpu_baseline: the CPU restatement (oracle/liborc.so) on a bounded crop of the
-same image with all host threads.
+roofline: the dominant kernel of the headline step, the closed loop k_recon_queue (one persistent launch per encode):
+its algorithmic HBM bytes (source planes read once + reconstruction written once + 2 bytes per coefficient + 16 bytes
+per block and tx-block record) over its launch duration, timed with HIP events on the encode stream.
 """
 import argparse
 import json
@@ -24,17 +31,27 @@ METRIC = "MP/s at qindex 120, batched+learned vs plain, at <=+1% BD-rate; p99 la
 HBM_PEAK_GBS = 8000.0
 
 
+def alg_bytes(recs, bps=1):
+    """Algorithmic HBM bytes of the closed loop over records: planes read + reconstruction written (4:2:0), 2 bytes
+    per coefficient, 16 per block and tx-block entry."""
+    return sum(r.width * r.height * bps * 1.5 * 2 + 2 * len(r.coeffs) + 16 * (len(r.blocks) + len(r.txb)) for r in recs)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=256, help="images of the mixed-size batch")
+    ap.add_argument("--conf", type=float, default=0.8, help="confidence of the learned pruning")
     ap.add_argument("--width", type=int, default=4000)
     ap.add_argument("--height", type=int, default=3000)
-    ap.add_argument("--qindex", type=int, default=120)
+    ap.add_argument("--plain-steps", type=int, default=3, help="timed encodes of the plain leg (0 = skip it)")
+    ap.add_argument("--no-exhaustive", action="store_true", help="skip the batched exhaustive leg")
+    ap.add_argument("--plain-only", action="store_true", help="run the plain leg alone (profiling)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-bd", action="store_true", help="skip the BD-rate leg")
     ap.add_argument("--schedule", type=int, default=0, help="closed loop: 0 persistent queue, 1 one launch per wavefront step")
-    ap.add_argument("--batch", type=int, default=256, help="images of the mixed-size batch leg (0 = skip it)")
     ap.add_argument("--serve", type=int, default=64, help="requests of the serving leg (0 = skip it)")
     ap.add_argument("--rate", type=float, default=50.0, help="serving leg: Poisson arrival rate, requests per second")
     a = ap.parse_args()
@@ -43,36 +60,42 @@ def main():
               "--gpus %d is not supported" % a.gpus, file=sys.stderr)
         sys.exit(2)
     import av1gpu_amd as A
+    from av1gpu_amd import learn
 
-    W, H = a.width, a.height
-    planes = A.synth_image(1000, W, H)
-    frame = A.DeviceFrame(planes, W, H, qindex=a.qindex)
-    enc = A.GpEncoder().init(A.GpConfig(qindex=a.qindex, schedule=a.schedule))
+    plain, planes = plain_leg(A, a.width, a.height, 120, a.schedule, a.plain_steps) if a.plain_steps > 0 else (None, None)
+    if a.plain_only:
+        print(json.dumps({"plain_single_image": plain}), flush=True)
+        return
+    params = learn.load_params()
+
+    from concurrent.futures import ThreadPoolExecutor
+    spec = A.batch_spec(a.batch)
+    with ThreadPoolExecutor(16) as ex:
+        bplanes = list(ex.map(lambda s: A.synth_image(s[3], s[0], s[1]), spec))
+    frames = [A.DeviceFrame(p, W, H, qindex=q) for p, (W, H, q, _) in zip(bplanes, spec)]
+    mp = sum(W * H for W, H, _, _ in spec) / 1e6
+
+    enc = A.GpEncoder().init(A.GpConfig(qindex=120, schedule=a.schedule))
+    learn.set_shortlist(enc, params, a.conf)
     for _ in range(a.warmup):
-        enc.encode([frame], want_recon=True)
+        enc.encode(frames, want_recon=False)
     A.lib().av1mi_device_sync()
-    times, stage = [], []
+    stage = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        s0 = time.perf_counter()
-        out, t = enc.encode([frame], want_recon=True)
-        times.append(time.perf_counter() - s0)
+        out, t = enc.encode(frames, want_recon=False)
         stage.append(t)
     A.lib().av1mi_device_sync()
     elapsed = time.perf_counter() - t0
-    mp = W * H / 1e6
+    enc.close()
     value = mp * a.steps / elapsed
-    rec = out[0]
-    n_coeffs, n_blocks, n_txb = len(rec.coeffs), len(rec.blocks), len(rec.txb)
-    alg_bytes = W * H * 1 * 1.5 * 2 + 2 * n_coeffs + 16 * (n_blocks + n_txb)
-    recon_ms = sum(s.recon for s in stage) / len(stage)
-    analysis_ms = sum(s.analysis for s in stage) / len(stage)
-    steps = stage[-1].steps
-    # dominant kernel: the closed loop, one persistent launch per encode (queue schedule) or one per wavefront step
-    launches = 1 if stage[-1].schedule == 0 else max(steps, 1)
-    kernel = "k_recon_queue" if stage[-1].schedule == 0 else "k_recon"
-    per_launch_ms = recon_ms / launches
-    achieved = (alg_bytes / launches) / (per_launch_ms * 1e-3) / 1e9
+    avg = lambda k: round(sum(getattr(s, k) for s in stage) / len(stage), 3)
+    last = stage[-1]
+    launches = 1 if last.schedule == 0 else max(last.steps, 1)
+    kernel = "k_recon_queue" if last.schedule == 0 else "k_recon"
+    per_launch_ms = avg("recon") / launches
+    ab = alg_bytes(out)
+    achieved = (ab / launches) / (per_launch_ms * 1e-3) / 1e9
     line = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -86,64 +109,111 @@ def main():
         "vs_baseline": None,
         "dtype": "u8",
         "data": "synthetic",
-        "config": {"workload": "plain exhaustive intra search, one %dx%d 8-bit 4:2:0 synthetic image (seed 1000), "
-                               "qindex %d" % (W, H, a.qindex), "width": W, "height": H, "qindex": a.qindex,
-                   "parallelism": "single GPU"},
-        "stages_ms": {"analysis": round(analysis_ms, 3), "dp": round(sum(s.dp for s in stage) / len(stage), 3),
-                      "recon": round(recon_ms, 3), "pack": round(sum(s.pack for s in stage) / len(stage), 3),
-                      "download": round(sum(s.download for s in stage) / len(stage), 3),
-                      "total_events": round(sum(s.total for s in stage) / len(stage), 3)},
-        "record": {"blocks": n_blocks, "txb": n_txb, "coeffs": n_coeffs, "wavefront_steps": steps, "sbs": stage[-1].sbs},
-        "schedule": {"closed_loop": "persistent queue" if stage[-1].schedule == 0 else "launch per wavefront step",
-                     "workgroups": stage[-1].queueBlocks},
+        "config": {"workload": "batched+learned: %d synthetic 8-bit 4:2:0 images, 0.18-24 MP log-uniform, aspect 0.5-2, "
+                               "qindex 80/120/160 (seeds 1000..%d), one encode call per step, learned pruning at "
+                               "confidence %g" % (a.batch, 1000 + a.batch - 1, a.conf),
+                   "images": a.batch, "megapixels": round(mp, 3), "confidence": a.conf, "parallelism": "single GPU"},
+        "vs_plain_single_image": round(value / plain["value"], 3) if plain else None,
+        "stages_ms": {k: avg(k) for k in ("analysis", "dp", "recon", "pack", "download", "total")},
+        "record": {"sbs": last.sbs, "blocks": last.blocks, "device_bytes": last.deviceBytes},
+        "schedule": {"closed_loop": "persistent queue" if last.schedule == 0 else "launch per wavefront step",
+                     "workgroups": last.queueBlocks},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 4), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "per_launch_ms": round(per_launch_ms, 4), "alg_bytes_per_launch": round(alg_bytes / launches)},
+                     "per_launch_ms": round(per_launch_ms, 4), "alg_bytes_per_launch": round(ab / launches)},
+        "plain_single_image": plain,
     }
-    frame.close()
-    if a.batch > 0:
-        line["batched"] = batch_leg(A, a.batch, a.schedule, value)
-    if a.serve > 0:
-        line["serving"] = serving_leg(A, a.serve, a.rate)
-    if not a.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(planes, W, H, a.qindex)
-    print(json.dumps(line), flush=True)
-    enc.close()
-
-
-def batch_leg(A, n, schedule, plain_value):
-    """The mixed-size batch (A.batch_spec: n images, 0.18-24 MP, qindex 80/120/160) in one encode call, inputs
-    resident in device memory; one warm-up call on the first 8 images, then one timed call on all n."""
-    from concurrent.futures import ThreadPoolExecutor
-    spec = A.batch_spec(n)
-    with ThreadPoolExecutor(16) as ex:
-        planes = list(ex.map(lambda s: A.synth_image(s[3], s[0], s[1]), spec))
-    frames = [A.DeviceFrame(p, W, H, qindex=q) for p, (W, H, q, _) in zip(planes, spec)]
-    del planes
-    enc = A.GpEncoder().init(A.GpConfig(qindex=120, schedule=schedule))
-    enc.encode(frames[:8], want_recon=True)
-    A.lib().av1mi_device_sync()
-    t0 = time.perf_counter()
-    out, t = enc.encode(frames, want_recon=True)
-    A.lib().av1mi_device_sync()
-    dt = time.perf_counter() - t0
-    mp = sum(W * H for W, H, _, _ in spec) / 1e6
-    res = {"workload": "%d synthetic 8-bit 4:2:0 images, 0.18-24 MP log-uniform, aspect 0.5-2, qindex 80/120/160, "
-                       "one encode call (seeds 1000..%d)" % (n, 1000 + n - 1),
-           "images": n, "megapixels": round(mp, 3), "value": round(mp / dt, 4), "unit": "MP/s", "seconds": round(dt, 3),
-           "vs_plain_single_image": round(mp / dt / plain_value, 3),
-           "stages_ms": {"analysis": round(t.analysis, 3), "dp": round(t.dp, 3), "recon": round(t.recon, 3),
-                         "pack": round(t.pack, 3), "download": round(t.download, 3)},
-           "sbs": t.sbs, "blocks": t.blocks, "device_bytes": t.deviceBytes}
+    print("headline done: %.1f MP/s" % value, file=sys.stderr, flush=True)
+    if not a.no_exhaustive:
+        line["batched_exhaustive"] = exhaustive_leg(A, frames, mp, a.schedule, value)
     for f in frames:
         f.close()
+    if not a.no_bd:
+        line["bd_rate"] = bd_leg(A, learn, params, a.conf, spec, bplanes)
+    del bplanes
+    if a.serve > 0:
+        line["serving"] = serving_leg(A, a.serve, a.rate, (params, a.conf))
+    if not a.no_cpu and planes is not None:
+        line["cpu_baseline"] = cpu_baseline(planes, a.width, a.height, 120)
+    print(json.dumps(line), flush=True)
+
+
+def plain_leg(A, W, H, qindex, schedule, steps):
+    """The plain exhaustive search on one W x H image, steps timed encodes after one warm-up (records to the host, the
+    reconstruction kept on the device, as in the headline)."""
+    planes = A.synth_image(1000, W, H)
+    frame = A.DeviceFrame(planes, W, H, qindex=qindex)
+    enc = A.GpEncoder().init(A.GpConfig(qindex=qindex, schedule=schedule))
+    enc.encode([frame], want_recon=False)
+    A.lib().av1mi_device_sync()
+    stage = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out, t = enc.encode([frame], want_recon=False)
+        stage.append(t)
+    A.lib().av1mi_device_sync()
+    elapsed = time.perf_counter() - t0
     enc.close()
+    frame.close()
+    avg = lambda k: round(sum(getattr(s, k) for s in stage) / len(stage), 3)
+    rec, last = out[0], stage[-1]
+    launches = 1 if last.schedule == 0 else max(last.steps, 1)
+    ab = alg_bytes([rec])
+    achieved = (ab / launches) / (avg("recon") / launches * 1e-3) / 1e9
+    res = {"workload": "plain exhaustive intra search, one %dx%d 8-bit 4:2:0 synthetic image (seed 1000), qindex %d"
+                       % (W, H, qindex), "value": round(W * H / 1e6 * steps / elapsed, 4), "unit": "MP/s", "steps": steps,
+           "ms_per_step": round(1e3 * elapsed / steps, 3),
+           "stages_ms": {k: avg(k) for k in ("analysis", "dp", "recon", "pack", "download", "total")},
+           "record": {"blocks": len(rec.blocks), "txb": len(rec.txb), "coeffs": len(rec.coeffs), "wavefront_steps": last.steps,
+                      "sbs": last.sbs},
+           "roofline": {"bound": "hbm", "kernel": "k_recon_queue" if last.schedule == 0 else "k_recon",
+                        "achieved": round(achieved, 4), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "per_launch_ms": round(avg("recon") / launches, 4)}}
+    print("plain leg done: %.1f MP/s" % res["value"], file=sys.stderr, flush=True)
+    return res, planes
+
+
+def exhaustive_leg(A, frames, mp, schedule, learned_value):
+    """The same batch with the exhaustive search: one untimed pass, one timed."""
+    enc = A.GpEncoder().init(A.GpConfig(qindex=120, schedule=schedule))
+    enc.encode(frames[:8], want_recon=False)
+    A.lib().av1mi_device_sync()
+    t0 = time.perf_counter()
+    _, t = enc.encode(frames, want_recon=False)
+    A.lib().av1mi_device_sync()
+    dt = time.perf_counter() - t0
+    enc.close()
+    res = {"value": round(mp / dt, 4), "unit": "MP/s", "seconds": round(dt, 3),
+           "learned_speedup": round(learned_value / (mp / dt), 3),
+           "stages_ms": {"analysis": round(t.analysis, 3), "dp": round(t.dp, 3), "recon": round(t.recon, 3),
+                         "pack": round(t.pack, 3), "download": round(t.download, 3)}, "blocks": t.blocks}
+    print("exhaustive batch done: %.1f MP/s" % res["value"], file=sys.stderr, flush=True)
     return res
 
 
-def serving_leg(A, n, rate):
+def bd_leg(A, learn, params, conf, spec, bplanes, n=16, max_mp=2.0, qs=(60, 100, 140, 180)):
+    """BD-rate of the learned search against the exhaustive one on the first n batch images of at most max_mp."""
+    idx = [i for i, (W, H, _, _) in enumerate(spec) if W * H <= max_mp * 1e6][:n]
+    images = [(bplanes[i], spec[i][0], spec[i][1]) for i in idx]
+    curves = {}
+    for name, c in (("exhaustive", None), ("learned", conf)):
+        enc = A.GpEncoder().init(A.GpConfig(qindex=120))
+        if c is not None:
+            learn.set_shortlist(enc, params, c)
+        bits, psnr, secs = learn.rd_curve(enc, images, qs)
+        enc.close()
+        curves[name] = {"bits": bits, "psnr_y": [round(v, 4) for v in psnr], "seconds": round(secs, 3)}
+    bd = learn.bd_rate(curves["exhaustive"]["bits"], curves["exhaustive"]["psnr_y"], curves["learned"]["bits"],
+                       curves["learned"]["psnr_y"])
+    print("bd leg done: %.3f %%" % bd, file=sys.stderr, flush=True)
+    return {"bd_rate_psnr_y_pct": round(float(bd), 3), "images": [spec[i][3] for i in idx],
+            "megapixels": round(sum(W * H for _, W, H in images) / 1e6, 3), "qindex": list(qs), "curves": curves}
+
+
+def serving_leg(A, n, rate, shortlist):
     """n single-image requests (serve.serving_spec: 0.2-24 MP, 1 % at 24 MP, qindex 120) arriving as a Poisson
-    process at `rate` per second into the continuous-batching server; latency = submission to result on the host."""
+    process at `rate` per second into the continuous-batching server on the batched+learned path; latency =
+    submission to result on the host."""
     from concurrent.futures import ThreadPoolExecutor
     import numpy as np
     from av1gpu_amd import serve
@@ -151,7 +221,7 @@ def serving_leg(A, n, rate):
     with ThreadPoolExecutor(16) as ex:
         planes = list(ex.map(lambda s: A.synth_image(s[3], s[0], s[1]), spec))
     images = [(p, W, H, q) for p, (W, H, q, _) in zip(planes, spec)]
-    srv = serve.Server(workers=4, hbm_budget=96 << 30, max_batch=16)
+    srv = serve.Server(workers=4, hbm_budget=96 << 30, max_batch=16, shortlist=shortlist)
     serve.poisson_run(srv, images[:4], rate)          # warm-up
     nb0 = len(srv.batches)
     reqs, wall = serve.poisson_run(srv, images, rate)
@@ -161,7 +231,7 @@ def serving_leg(A, n, rate):
     return {"workload": "%d requests, 8-bit 4:2:0, 0.2-24 MP log-uniform with 1%% at 24 MP, qindex 120, Poisson %.0f/s"
                         % (n, rate), "requests": n, "megapixels": round(mp, 3), "value": round(mp / wall, 4), "unit": "MP/s",
             "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
-            "workers": 4, "batches": len(srv.batches) - nb0}
+            "workers": 4, "batches": len(srv.batches) - nb0, "learned_confidence": shortlist[1]}
 
 
 def cpu_baseline(planes, W, H, qindex):

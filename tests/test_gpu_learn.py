@@ -152,3 +152,41 @@ def test_network_chunks_and_batching_do_not_change_records(monkeypatch):
         e.close()
     for f in frames:
         f.close()
+
+
+def test_records_obey_the_restated_decisions():
+    """Every block of a pruned record lies in a node the decisions (learn.decide over the network's logits) expand and
+    uses a luma mode of its superblock's mask; a confidence where the masks are mixed is used (some nodes pruned, some
+    superblocks shortlisted)."""
+    p = Ln.init_params(22)
+    convs, w5, b5 = Ln.unpack(p)
+    w5 *= 4.0
+    b5[Ln.N_SPLIT:] = np.linspace(6.0, -6.0, 13)            # a few modes likely, the rest not: shortlists appear
+    W, H = 328, 264
+    planes = A.synth_image(1200, W, H)
+    L = Ln.Learner(p)
+    tiles = Ln.sb_tiles(planes[0], W, H)
+    logits = L.forward(tiles, np.full(len(tiles), 120, np.uint8))
+    L.close()
+    f = A.DeviceFrame(planes, W, H, qindex=120)
+    enc = A.GpEncoder().init(A.GpConfig(qindex=120))
+    conf = 0.8
+    Ln.set_shortlist(enc, p, conf)
+    g = enc.encode([f])[0][0]
+    expand, mmask = Ln.decide(logits, W, H, conf)
+    assert 0 < expand[:, 1:].mean() < 1 and (mmask != 0x1FFF).any()
+    sbc = (2 * ((W + 7) >> 3) + 15) >> 4
+    w4 = np.array([1 << v for v in Ln._MI_W_LOG2])
+    h4 = np.array([1 << v for v in Ln._MI_H_LOG2])
+    for b in g.blocks:
+        x, y, s = 4 * int(b["mi_col"]), 4 * int(b["mi_row"]), int(b["bsize"])
+        sb = (y // 64) * sbc + x // 64
+        assert (mmask[sb] >> int(b["y_mode"])) & 1
+        M = 4 * max(w4[s], h4[s])
+        if M < 64:
+            n = 2 * M
+            lv = {64: 0, 32: 1, 16: 2, 8: 3}[n]
+            node = Ln.LEVEL_BASE[lv] + ((y % 64) // n) * (64 // n) + (x % 64) // n
+            assert expand[sb, node], (x, y, s)
+    enc.close()
+    f.close()

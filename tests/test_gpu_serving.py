@@ -7,13 +7,15 @@ import numpy as np
 import pytest
 
 import av1gpu_amd as A
-from av1gpu_amd import serve
+from av1gpu_amd import learn, serve
 
 pytestmark = pytest.mark.gpu
 
 
-def _alone(specs):
+def _alone(specs, shortlist=None):
     enc = A.GpEncoder().init(A.GpConfig(qindex=120))
+    if shortlist is not None:
+        learn.set_shortlist(enc, *shortlist)
     out = []
     for W, H, q, s in specs:
         f = A.DeviceFrame(A.synth_image(s, W, H), W, H, qindex=q)
@@ -51,6 +53,20 @@ def test_serving_results_equal_alone(workers, budget):
     for a, b in zip(ref, outs):
         _same(a, b)
     assert all(r.latency > 0 for r in reqs)
+
+
+def test_serving_with_learned_pruning_equals_alone():
+    """The serving loop on the batched+learned path: each request equals its image encoded alone with the same network."""
+    sl = (learn.load_params(), 0.8)
+    ref = _alone(SPECS, sl)
+    srv = serve.Server(workers=2, hbm_budget=8 << 30, max_batch=4, shortlist=sl)
+    reqs = [serve.Request(A.synth_image(s, W, H), W, H, q) for W, H, q, s in SPECS]
+    for r in reqs:
+        srv.submit(r)
+    outs = [r.future.result(timeout=120) for r in reqs]
+    srv.close()
+    for a, b in zip(ref, outs):
+        _same(a, b)
 
 
 def test_serving_rejects_oversized_request():

@@ -1,8 +1,8 @@
 """Training labels of the pruning network (av1gpu_amd/learn.py sb_labels / sb_tiles) against a raster restatement
 over oracle records: every 4x4 unit of the coded area is painted with the size and luma mode of the block that covers
 it (which also checks that the blocks tile the area exactly once, pinning the block-size tables), and a node's expand
-label is then "some unit inside it belongs to a block at most half the node's size", its mode shares the histogram of
-the painted modes over the visible area."""
+label is then "some unit inside it belongs to a block at most half the node's size", its mode labels the modes that
+appear among the painted units."""
 import numpy as np
 import pytest
 
@@ -53,9 +53,8 @@ def test_labels_match_raster_restatement(oracle, W, H, q, seed):
                         parent = Ln.LEVEL_BASE[lv - 1] + (ny // 2) * (1 << (lv - 1)) + nx // 2
                         assert lab[s, parent] == 1.0
         m = mode[r0:r0 + 16, c0:c0 + 16].ravel()
-        share = np.bincount(m, minlength=13)[:13] / len(m)
-        assert np.allclose(lab[s, Ln.N_SPLIT:], share, atol=1e-6)
-        assert abs(lab[s, Ln.N_SPLIT:].sum() - 1.0) < 1e-5
+        used = (np.bincount(m, minlength=13)[:13] > 0).astype(np.float32)
+        assert np.array_equal(lab[s, Ln.N_SPLIT:], used)
 
 
 def test_tiles_are_the_padded_superblocks():
