@@ -312,6 +312,30 @@ class DeviceFrame:
             pass
 
 
+class _OutOwner:
+    """Owns the host buffers of one av1mi_frame_out (the library's allocations) until garbage collection."""
+
+    def __init__(self, o):
+        self.o = _FrameOut()
+        C.memmove(C.byref(self.o), C.byref(o), C.sizeof(_FrameOut))
+
+    def __del__(self):
+        try:
+            lib().av1mi_frame_out_free(C.byref(self.o))
+        except Exception:
+            pass
+
+
+def _owned(ptr, n, dtype, owner):
+    """A NumPy array over n items at ptr (no copy) that keeps owner alive."""
+    dt = np.dtype(dtype)
+    if not n or not ptr:
+        return np.zeros(0, dt)
+    raw = (C.c_char * (n * dt.itemsize)).from_address(ptr)
+    raw._owner = owner
+    return np.frombuffer(raw, dt)
+
+
 class GpEncoder:
     """GpEncoder (encoder/src/pipeline.h:191-211): init(config) once, encode(frames) per batch."""
 
@@ -359,19 +383,21 @@ class GpEncoder:
         try:
             for i in range(n):
                 o = outs[i]
+                own = _OutOwner(o)          # frees the host buffers when the last array over them is gone
+                o = own.o
+                outs[i] = _FrameOut()       # ownership moved to `own`
                 nb, nt, nc = o.n_blocks, o.n_txb, o.n_coeffs
-                blocks = np.frombuffer(C.string_at(o.blocks, nb * 16), BLOCK_DTYPE).copy() if nb else np.zeros(0, BLOCK_DTYPE)
-                bt = np.frombuffer(C.string_at(o.block_txb, (nb + 1) * 4), np.uint32).copy()
-                txb = np.frombuffer(C.string_at(o.txb, nt * 16), TXB_DTYPE).copy() if nt else np.zeros(0, TXB_DTYPE)
-                co = np.frombuffer(C.string_at(o.coeffs, nc * 2), np.int16).copy() if nc else np.zeros(0, np.int16)
+                blocks = _owned(o.blocks, nb, BLOCK_DTYPE, own)
+                bt = _owned(o.block_txb, nb + 1, np.uint32, own)
+                txb = _owned(o.txb, nt, TXB_DTYPE, own)
+                co = _owned(o.coeffs, nc, np.int16, own)
                 rec = []
                 if want_recon:
                     bps = 2 if o.bit_depth > 8 else 1
                     for p in range(3):
                         h = o.ah >> (1 if p else 0)
                         w = o.recon_pitch[p]
-                        buf = C.string_at(o.recon[p], w * h * bps)
-                        rec.append(np.frombuffer(buf, np.uint16 if bps == 2 else np.uint8).reshape(h, w).copy())
+                        rec.append(_owned(o.recon[p], w * h, np.uint16 if bps == 2 else np.uint8, own).reshape(h, w))
                 res.append(GpFrameOut(o.width, o.height, o.base_q_idx, o.bit_depth, blocks, bt, txb, co, rec, o.tile_cols,
                                       o.tile_rows))
         finally:

@@ -568,7 +568,8 @@ struct Batch {
     RecTxb *oTxb;                // [nSb * SB_TXB_CAP]
     int16_t *oCoeffs;            // [nSb * SB_COEF_CAP]
     uint32_t *stats;             // [8]: blocks, skip blocks, CfL blocks, range repairs
-    const uint32_t *lnMask;      // [nSb][4] learned pruning: expanded nodes (3 words), screened luma modes; null = exhaustive
+    const uint32_t *lnMask;      // [nSb][8] learned pruning: expanded nodes (words 0-2), screened luma modes (3), nodes
+                                 // keeping their own shapes (4-6); null = exhaustive
 };
 
 // the device work queue of the persistent closed-loop schedule

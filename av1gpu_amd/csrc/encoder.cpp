@@ -24,7 +24,9 @@ int recon_queue_slots(int bd);
 struct LnNet;
 LnNet *ln_new(const float *params);
 void ln_delete(LnNet *N);
-int ln_masks(LnNet &N, const Img *dImg, const uint32_t *dSbImg, uint32_t nSb, float conf, uint32_t chunk, uint32_t *dMask, hipStream_t st);
+int ln_masks(const LnNet &N, void *scratch, const Img *dImg, const uint32_t *dSbImg, uint32_t nSb, float conf, uint32_t chunk,
+             uint32_t *dMask, hipStream_t st);
+size_t ln_scratch_bytes(uint32_t n);
 int recon_prepare(int bd);
 size_t recon_smem_bytes();
 void launch_pack(const Batch &B, const PackOff *dOff, RecBlock *oB, uint32_t *oBT, RecTxb *oT, int16_t *oC, hipStream_t st);
@@ -336,7 +338,9 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     uint32_t *dSteps, *dDone, *dQctl, *dLnMask = nullptr;
     PackOff *dPO;
     A.add(dSteps, stepFlat.size()); A.add(dPO, nSb); A.add(dDone, nSb); A.add(dQctl, 2);
-    if (enc->ln) A.add(dLnMask, (size_t)nSb * 4);
+    uint8_t *dLnAct = nullptr;
+    const uint32_t lnN = std::min(nSb, enc->lnChunk);
+    if (enc->ln) { A.add(dLnMask, (size_t)nSb * 8); A.add(dLnAct, ln_scratch_bytes(lnN)); }
     HCK(A.alloc(st));
     tm.deviceBytes = A.total;
     for (int i = 0; i < n; i++) {
@@ -368,7 +372,7 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     B.oTxb = dOT; B.oCoeffs = dOC; B.stats = dStats;
     (void)hipEventRecord(enc->ev[1], st);
     if (enc->ln) {   // the pruning network runs first (its time counts as analysis)
-        if (ln_masks(*enc->ln, dImg, dSbImg, nSb, enc->lnConf, enc->lnChunk, dLnMask, st)) { snprintf(err, errlen, "pruning network failed"); A.release(st); return -1; }
+        if (ln_masks(*enc->ln, dLnAct, dImg, dSbImg, nSb, enc->lnConf, lnN, dLnMask, st)) { snprintf(err, errlen, "pruning network failed"); A.release(st); return -1; }
         B.lnMask = dLnMask;
     }
     launch_analysis(B, P, shTotal, bd, st);
@@ -519,7 +523,8 @@ size_t av1mi_frame_bytes(int W, int H, int bitDepth) {
                          nSb * (4 + 2 + MAX_LEAVES * sizeof(Leaf) + 4 + sizeof(SbOut) + MAX_LEAVES * (sizeof(RecBlock) + 2) +
                                 SB_TXB_CAP * sizeof(RecTxb) + SB_COEF_CAP * 2 + 4 + sizeof(PackOff) + 4);
     const size_t records = nSb * (MAX_LEAVES * (sizeof(RecBlock) + 4) + SB_TXB_CAP * sizeof(RecTxb) + SB_COEF_CAP * 2);
-    return planes + arena + records + 64 * 1024;
+    const size_t network = nSb * 32 + ln_scratch_bytes((uint32_t)std::min<size_t>(nSb, 65536));   // masks, activations
+    return planes + arena + records + network + 64 * 1024;
 }
 int av1mi_dev_free(void *ptr) { return hipFree(ptr) == hipSuccess ? 0 : -1; }
 int av1mi_copy_h2d(void *dst, const void *src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : -1; }

@@ -77,19 +77,28 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
             if (lane == 0) { out.J = INF; out.ncand = 0; }
             continue;
         }
-        // learned pruning: a block belongs to the node of its larger side; it is analysed when that node's parent was
-        // expanded (the 64x64 node's own blocks always are); the screened luma modes are the superblock's shortlist
+        // learned pruning: a block belongs to the node of its larger side (one of that node's own shapes); it is
+        // analysed when that node's parent was expanded (the 64x64 node has no parent) and the node keeps its own
+        // shapes (4x4 blocks: the 8x8 node's children, no own bit); the screened luma modes are the superblock's shortlist
         uint32_t modeMask = 0x1FFFu;
         if (B.lnMask) {
-            const uint32_t *mk = B.lnMask + (size_t)(I.sbBase + (uint32_t)(y >> 6) * I.sbCols + (uint32_t)(x >> 6)) * 4;
-            constexpr int M = W > H ? W : H, PN = 2 * M;
+            const uint32_t *mk = B.lnMask + (size_t)(I.sbBase + (uint32_t)(y >> 6) * I.sbCols + (uint32_t)(x >> 6)) * 8;
+            constexpr int M = W > H ? W : H;
+            bool keep = true;
             if constexpr (M < 64) {
-                constexpr int LV = PN == 64 ? 0 : PN == 32 ? 1 : PN == 16 ? 2 : 3, BASE = LV == 0 ? 0 : LV == 1 ? 1 : LV == 2 ? 5 : 21;
+                constexpr int PN = 2 * M, LV = PN == 64 ? 0 : PN == 32 ? 1 : PN == 16 ? 2 : 3;
+                constexpr int BASE = LV == 0 ? 0 : LV == 1 ? 1 : LV == 2 ? 5 : 21;
                 const int node = BASE + ((y & 63) / PN) * (64 / PN) + (x & 63) / PN;
-                if (!((mk[node >> 5] >> (node & 31)) & 1)) {
-                    if (lane == 0) { out.J = INF; out.ncand = 0; }
-                    continue;
-                }
+                keep = (mk[node >> 5] >> (node & 31)) & 1;
+            }
+            if constexpr (M > 4) {
+                constexpr int LV = M == 64 ? 0 : M == 32 ? 1 : M == 16 ? 2 : 3, BASE = LV == 0 ? 0 : LV == 1 ? 1 : LV == 2 ? 5 : 21;
+                const int node = BASE + ((y & 63) / M) * (64 / M) + (x & 63) / M;
+                keep = keep && ((mk[4 + (node >> 5)] >> (node & 31)) & 1);
+            }
+            if (!keep) {
+                if (lane == 0) { out.J = INF; out.ncand = 0; }
+                continue;
             }
             modeMask = mk[3] & 0x1FFFu;
         }

@@ -1,13 +1,15 @@
 // Learned candidate pruning: a small BF16 CNN per 64x64 superblock, run on MFMA with FP32 accumulation, that predicts
-// which square partition nodes need their children analysed and which luma modes to screen.  No counterpart in the
+// for each square partition node whether its children and its own shapes are needed, and which luma modes to screen.  No counterpart in the
 // reference (its pruning stages are hand-written heuristics: encoder/src/k_leafprep.cu:1-14, an_stage.h:1-15, the
 // screener hook gp.h:136-143).  The network only removes candidates from the exhaustive search, so every decision it
 // leads to is still an ordinary, decodable AV1 decision.
 //
 // Network (per superblock):  x0 = [64 x 64 luma / max - 0.5, qindex / 255]  (NHWC, 2 channels, BF16)
 //   L1..L4: 3x3 convolution, stride 2, zero padding 1, 32 output channels, bias, ReLU   (64 -> 32 -> 16 -> 8 -> 4)
-//   L5: fully connected 512 -> 98 logits: 85 "expand" logits (node 0: the 64x64 node, 1-4: 32x32 nodes, 5-20: 16x16,
-//       21-84: 8x8, raster order per level) and 13 luma-mode logits (the mode is used by some block of the superblock).
+//   L5: fully connected 512 -> 183 logits: 85 "expand" logits (node 0: the 64x64 node, 1-4: 32x32 nodes, 5-20: 16x16,
+//       21-84: 8x8, raster order per level: the search uses a block inside the node at most half its size), 85 "own"
+//       logits (same nodes: it uses a block whose larger side is the node's size, i.e. one of the node's own partition
+//       shapes) and 13 luma-mode logits (the mode is used by some block of the superblock).
 // Convolutions are implicit GEMMs on v_mfma_f32_16x16x16_bf16 (positions x input taps times taps x channels), weights
 // and activations BF16, accumulation and biases FP32.  Training runs the same forward kernels, then FP32 backward
 // kernels (convolution weight gradients accumulated with float atomics) and Adam on FP32 master weights.
@@ -22,7 +24,7 @@ typedef uint16_t bf16_t;
 typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
-constexpr int LN_OUT = 98, LN_SPLIT = 85, LN_FEAT = 512, LN_C = 32;
+constexpr int LN_NODES = 85, LN_OWN0 = 85, LN_MODE0 = 170, LN_OUT = 183, LN_FEAT = 512, LN_C = 32;
 
 __device__ __forceinline__ bf16_t to_bf16(float f) {   // round to nearest even
     const uint32_t u = __float_as_uint(f);
@@ -115,11 +117,13 @@ __global__ void __launch_bounds__(128) k_ln_fc(const bf16_t *a4, const float *W,
     }
 }
 
-// ---- decisions: per superblock the nodes whose children are analysed and the luma modes screened ----
-// A node is expanded unless the network is confident (p <= 1 - conf) that the search would not use its children, and
-// always where it reaches past the frame (the partition syntax may force a split there).  The modes are the 8 most
-// likely when the network is confident that none of the other 5 is used (product of their 1 - p at least conf), else
-// all 13.  mask: 3 words of node bits, 1 of modes.
+// ---- decisions: per superblock the nodes whose children / own shapes are analysed and the luma modes screened ----
+// A node is expanded (its children analysed) unless the network is confident (p <= 1 - conf) that the search would not
+// use its children, and keeps its own shapes unless it is as confident that those go unused; a node where both would
+// be dropped keeps its own shapes (every node reached keeps a partition), and a node reaching past the frame keeps
+// both (the partition syntax may force a split there).  The modes are the 8 most likely when the network is confident
+// that none of the other 5 is used (product of their 1 - p at least conf), else all 13.
+// mask (8 words per superblock): 0-2 expand bits, 3 screened modes, 4-6 own bits, 7 zero.
 // logits of slots 0 .. n - 1 belong to superblocks s0 .. s0 + n - 1 of the batch; mask is indexed by superblock
 __global__ void __launch_bounds__(64) k_ln_decide(const float *logits, const Img *imgs, const uint32_t *sbImg, uint32_t s0, uint32_t n,
                                                   float conf, uint32_t *mask) {
@@ -130,19 +134,22 @@ __global__ void __launch_bounds__(64) k_ln_decide(const float *logits, const Img
     const int sx = (int)(local % I.sbCols) * 64, sy = (int)(local / I.sbCols) * 64;
     const int FW = I.MiCols * 4, FH = I.MiRows * 4;
     const float *z = logits + (size_t)slot * LN_OUT;
-    uint32_t m[4] = {0, 0, 0, 0};
+    uint32_t m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int node = 0;
     for (int lv = 0; lv < 4; lv++) {
         const int n = 64 >> lv, per = 1 << lv;
         for (int ny = 0; ny < per; ny++)
             for (int nx = 0; nx < per; nx++, node++) {
                 const bool edge = sx + nx * n + n > FW || sy + ny * n + n > FH;
-                const float p = 1.f / (1.f + expf(-z[node]));
-                if (edge || p > 1.f - conf) m[node >> 5] |= 1u << (node & 31);
+                const float pe = 1.f / (1.f + expf(-z[node])), po = 1.f / (1.f + expf(-z[LN_OWN0 + node]));
+                const bool ex = edge || pe > 1.f - conf;
+                const bool ow = edge || po > 1.f - conf || !ex;
+                if (ex) m[node >> 5] |= 1u << (node & 31);
+                if (ow) m[4 + (node >> 5)] |= 1u << (node & 31);
             }
     }
     float p[13];
-    for (int k = 0; k < 13; k++) p[k] = 1.f / (1.f + expf(-z[LN_SPLIT + k]));
+    for (int k = 0; k < 13; k++) p[k] = 1.f / (1.f + expf(-z[LN_MODE0 + k]));
     uint32_t top = 0;
     for (int r = 0; r < 8; r++) {
         int b = -1;
@@ -152,7 +159,7 @@ __global__ void __launch_bounds__(64) k_ln_decide(const float *logits, const Img
     float none = 1.f;   // probability that no mode outside the shortlist is used
     for (int k = 0; k < 13; k++) if (!((top >> k) & 1)) none *= 1.f - p[k];
     m[3] = none >= conf ? top : 0x1FFFu;
-    for (int k = 0; k < 4; k++) mask[(size_t)sb * 4 + k] = m[k];
+    for (int k = 0; k < 8; k++) mask[(size_t)sb * 8 + k] = m[k];
 }
 
 // ================================ training ================================
@@ -281,13 +288,34 @@ constexpr size_t LN_NBF = 32 * LN_KP1 + 3 * 32 * LN_KP;   // the BF16 copies of 
 inline size_t ln_kp(int l) { return l == 0 ? LN_KP1 : LN_KP; }
 inline size_t ln_bf_off(int l) { return l == 0 ? 0 : 32 * LN_KP1 + (size_t)(l - 1) * 32 * LN_KP; }
 
+// activations of n items: the input and the four convolution outputs (BF16, NHWC) and the logits
+struct LnActs {
+    bf16_t *x0 = nullptr, *a1 = nullptr, *a2 = nullptr, *a3 = nullptr, *a4 = nullptr;
+    float *logits = nullptr;
+};
+constexpr size_t LN_ACT_SZ[6] = {64 * 64 * 2 * 2, 32 * 32 * 32 * 2, 16 * 16 * 32 * 2, 8 * 8 * 32 * 2, LN_FEAT * 2, LN_OUT * 4};
+inline size_t ln_al(size_t b) { return (b + 255) & ~(size_t)255; }
+// device bytes of the activations of n items (one buffer, carved by ln_carve)
+size_t ln_scratch_bytes(uint32_t n) {
+    size_t t = 0;
+    for (size_t b : LN_ACT_SZ) t += ln_al(b * n);
+    return t;
+}
+static LnActs ln_carve(void *base, uint32_t n) {
+    char *p = (char *)base;
+    LnActs a;
+    void **dst[6] = {(void **)&a.x0, (void **)&a.a1, (void **)&a.a2, (void **)&a.a3, (void **)&a.a4, (void **)&a.logits};
+    for (int k = 0; k < 6; k++) { *dst[k] = p; p += ln_al(LN_ACT_SZ[k] * n); }
+    return a;
+}
+
 struct LnNet {
     float *w = nullptr;            // FP32 parameters (LN_NPARAM)
     bf16_t *wb = nullptr;          // BF16 convolution weights (LN_NBF)
-    // activations for up to cap items
+    // activations of the training and check entry points, for up to cap items (one allocation)
     uint32_t cap = 0;
-    bf16_t *x0 = nullptr, *a1 = nullptr, *a2 = nullptr, *a3 = nullptr, *a4 = nullptr;
-    float *logits = nullptr;
+    void *actMem = nullptr;
+    LnActs act;
     // training state (allocated on first use)
     float *g = nullptr, *m = nullptr, *v = nullptr, *dlog = nullptr, *d4 = nullptr, *d3 = nullptr, *d2 = nullptr, *d1 = nullptr, *loss = nullptr;
     float *label = nullptr, *lmask = nullptr;
@@ -299,27 +327,23 @@ struct LnNet {
 
 static hipError_t ln_alloc_acts(LnNet &N, uint32_t n) {
     if (n <= N.cap) return hipSuccess;
-    (void)hipFree(N.x0); (void)hipFree(N.a1); (void)hipFree(N.a2); (void)hipFree(N.a3); (void)hipFree(N.a4); (void)hipFree(N.logits);
-    N.x0 = N.a1 = N.a2 = N.a3 = N.a4 = nullptr; N.logits = nullptr; N.cap = 0;
-    hipError_t e;
-    if ((e = hipMalloc(&N.x0, (size_t)n * 64 * 64 * 2 * 2)) != hipSuccess) return e;
-    if ((e = hipMalloc(&N.a1, (size_t)n * 32 * 32 * 32 * 2)) != hipSuccess) return e;
-    if ((e = hipMalloc(&N.a2, (size_t)n * 16 * 16 * 32 * 2)) != hipSuccess) return e;
-    if ((e = hipMalloc(&N.a3, (size_t)n * 8 * 8 * 32 * 2)) != hipSuccess) return e;
-    if ((e = hipMalloc(&N.a4, (size_t)n * LN_FEAT * 2)) != hipSuccess) return e;
-    if ((e = hipMalloc(&N.logits, (size_t)n * LN_OUT * 4)) != hipSuccess) return e;
+    (void)hipFree(N.actMem);
+    N.actMem = nullptr; N.act = LnActs(); N.cap = 0;
+    const hipError_t e = hipMalloc(&N.actMem, ln_scratch_bytes(n));
+    if (e != hipSuccess) return e;
+    N.act = ln_carve(N.actMem, n);
     N.cap = n;
     return hipSuccess;
 }
 
-// forward of n items whose inputs are in N.x0
-static void ln_forward(LnNet &N, uint32_t n, hipStream_t st) {
+// forward of n items whose inputs are in a.x0
+static void ln_forward(const LnNet &N, const LnActs &a, uint32_t n, hipStream_t st) {
     const float *w = N.w;
-    hipLaunchKernelGGL((k_ln_conv<2, 64>), dim3(n * 64), dim3(64), 0, st, N.x0, N.wb + ln_bf_off(0), w + LN_W_OFF[0] + 32 * LN_KP1, N.a1, n);
-    hipLaunchKernelGGL((k_ln_conv<32, 32>), dim3(n * 16), dim3(64), 0, st, N.a1, N.wb + ln_bf_off(1), w + LN_W_OFF[1] + 32 * LN_KP, N.a2, n);
-    hipLaunchKernelGGL((k_ln_conv<32, 16>), dim3(n * 4), dim3(64), 0, st, N.a2, N.wb + ln_bf_off(2), w + LN_W_OFF[2] + 32 * LN_KP, N.a3, n);
-    hipLaunchKernelGGL((k_ln_conv<32, 8>), dim3(n), dim3(64), 0, st, N.a3, N.wb + ln_bf_off(3), w + LN_W_OFF[3] + 32 * LN_KP, N.a4, n);
-    hipLaunchKernelGGL(k_ln_fc, dim3(n), dim3(128), 0, st, N.a4, w + LN_FC_OFF, w + LN_FC_OFF + (size_t)LN_OUT * LN_FEAT, N.logits, n);
+    hipLaunchKernelGGL((k_ln_conv<2, 64>), dim3(n * 64), dim3(64), 0, st, a.x0, N.wb + ln_bf_off(0), w + LN_W_OFF[0] + 32 * LN_KP1, a.a1, n);
+    hipLaunchKernelGGL((k_ln_conv<32, 32>), dim3(n * 16), dim3(64), 0, st, a.a1, N.wb + ln_bf_off(1), w + LN_W_OFF[1] + 32 * LN_KP, a.a2, n);
+    hipLaunchKernelGGL((k_ln_conv<32, 16>), dim3(n * 4), dim3(64), 0, st, a.a2, N.wb + ln_bf_off(2), w + LN_W_OFF[2] + 32 * LN_KP, a.a3, n);
+    hipLaunchKernelGGL((k_ln_conv<32, 8>), dim3(n), dim3(64), 0, st, a.a3, N.wb + ln_bf_off(3), w + LN_W_OFF[3] + 32 * LN_KP, a.a4, n);
+    hipLaunchKernelGGL(k_ln_fc, dim3(n), dim3(128), 0, st, a.a4, w + LN_FC_OFF, w + LN_FC_OFF + (size_t)LN_OUT * LN_FEAT, a.logits, n);
 }
 
 static void ln_refresh_bf16(LnNet &N, hipStream_t st) {
@@ -338,7 +362,7 @@ static hipError_t ln_upload(LnNet &N, const float *params) {
 }
 
 static void ln_free(LnNet &N) {
-    void *p[] = {N.w, N.wb, N.x0, N.a1, N.a2, N.a3, N.a4, N.logits, N.g, N.m, N.v, N.dlog, N.d4, N.d3, N.d2, N.d1, N.loss, N.label, N.lmask,
+    void *p[] = {N.w, N.wb, N.actMem, N.g, N.m, N.v, N.dlog, N.d4, N.d3, N.d2, N.d1, N.loss, N.label, N.lmask,
                  N.tiles, N.q};
     for (void *x : p) if (x) (void)hipFree(x);
     N = LnNet();
@@ -355,14 +379,16 @@ void ln_delete(LnNet *N) {
     delete N;
 }
 
-// learned masks of a batch (encoder integration): mask[nSb][4]
-int ln_masks(LnNet &N, const Img *dImg, const uint32_t *dSbImg, uint32_t nSb, float conf, uint32_t chunk, uint32_t *dMask, hipStream_t st) {
+// learned masks of a batch (encoder integration): mask[nSb][8].  scratch: ln_scratch_bytes(chunk) device bytes
+// (stream-ordered memory of the encode call, so the network allocates and frees nothing itself).
+int ln_masks(const LnNet &N, void *scratch, const Img *dImg, const uint32_t *dSbImg, uint32_t nSb, float conf, uint32_t chunk,
+             uint32_t *dMask, hipStream_t st) {
+    const LnActs a = ln_carve(scratch, chunk);
     for (uint32_t s0 = 0; s0 < nSb; s0 += chunk) {
         const uint32_t n = nSb - s0 < chunk ? nSb - s0 : chunk;
-        if (ln_alloc_acts(N, n) != hipSuccess) return -1;
-        hipLaunchKernelGGL(k_ln_gather, dim3(n), dim3(256), 0, st, dImg, dSbImg, s0, n, N.x0);
-        ln_forward(N, n, st);
-        hipLaunchKernelGGL(k_ln_decide, dim3((n + 63) / 64), dim3(64), 0, st, N.logits, dImg, dSbImg, s0, n, conf, dMask);
+        hipLaunchKernelGGL(k_ln_gather, dim3(n), dim3(256), 0, st, dImg, dSbImg, s0, n, a.x0);
+        ln_forward(N, a, n, st);
+        hipLaunchKernelGGL(k_ln_decide, dim3((n + 63) / 64), dim3(64), 0, st, a.logits, dImg, dSbImg, s0, n, conf, dMask);
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -423,9 +449,9 @@ int av1mi_ln_forward(av1mi_learner *L, const uint8_t *tiles, const uint8_t *q, u
     if (hipMalloc(&dt, (size_t)n * 4096) != hipSuccess || hipMalloc(&dq, n) != hipSuccess) { (void)hipFree(dt); return -1; }
     (void)hipMemcpy(dt, tiles, (size_t)n * 4096, hipMemcpyHostToDevice);
     (void)hipMemcpy(dq, q, n, hipMemcpyHostToDevice);
-    hipLaunchKernelGGL(k_ln_tiles, dim3(n), dim3(256), 0, 0, dt, dq, n, N.x0);
-    ln_forward(N, n, 0);
-    const hipError_t e = hipMemcpy(logits, N.logits, (size_t)n * LN_OUT * 4, hipMemcpyDeviceToHost);
+    hipLaunchKernelGGL(k_ln_tiles, dim3(n), dim3(256), 0, 0, dt, dq, n, N.act.x0);
+    ln_forward(N, N.act, n, 0);
+    const hipError_t e = hipMemcpy(logits, N.act.logits, (size_t)n * LN_OUT * 4, hipMemcpyDeviceToHost);
     (void)hipFree(dt); (void)hipFree(dq);
     return e == hipSuccess ? 0 : -1;
 }
@@ -434,7 +460,7 @@ int av1mi_ln_forward(av1mi_learner *L, const uint8_t *tiles, const uint8_t *q, u
 int av1mi_ln_activations(av1mi_learner *L, int layer, uint32_t n, uint16_t *out) {
     LnNet &N = L->net;
     const size_t per[5] = {64 * 64 * 2, 32 * 32 * 32, 16 * 16 * 32, 8 * 8 * 32, LN_FEAT};
-    const bf16_t *src[5] = {N.x0, N.a1, N.a2, N.a3, N.a4};
+    const bf16_t *src[5] = {N.act.x0, N.act.a1, N.act.a2, N.act.a3, N.act.a4};
     if (layer < 0 || layer > 4 || n > N.cap) return -1;
     return hipMemcpy(out, src[layer], n * per[layer] * 2, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
 }
@@ -478,24 +504,24 @@ static int ln_grad(LnNet &N, uint32_t first, uint32_t n, float *loss, hipStream_
     }
     (void)hipMemsetAsync(N.g, 0, LN_NPARAM * 4, st);
     (void)hipMemsetAsync(N.loss, 0, 4, st);
-    hipLaunchKernelGGL(k_ln_tiles, dim3(n), dim3(256), 0, st, N.tiles + (size_t)first * 4096, N.q + first, n, N.x0);
-    ln_forward(N, n, st);
+    hipLaunchKernelGGL(k_ln_tiles, dim3(n), dim3(256), 0, st, N.tiles + (size_t)first * 4096, N.q + first, n, N.act.x0);
+    ln_forward(N, N.act, n, st);
     const float scale = 1.0f / (float)n;
-    hipLaunchKernelGGL(k_ln_loss, dim3((n * LN_OUT + 127) / 128), dim3(128), 0, st, N.logits, N.label + (size_t)first * LN_OUT,
+    hipLaunchKernelGGL(k_ln_loss, dim3((n * LN_OUT + 127) / 128), dim3(128), 0, st, N.act.logits, N.label + (size_t)first * LN_OUT,
                        N.lmask + (size_t)first * LN_OUT, n, scale, N.dlog, N.loss);
     float *g = N.g;
     const float *w = N.w;
-    hipLaunchKernelGGL(k_ln_fc_bwd_w, dim3((LN_OUT * (LN_FEAT + 1) + 255) / 256), dim3(256), 0, st, N.a4, N.dlog, n, g + LN_FC_OFF,
+    hipLaunchKernelGGL(k_ln_fc_bwd_w, dim3((LN_OUT * (LN_FEAT + 1) + 255) / 256), dim3(256), 0, st, N.act.a4, N.dlog, n, g + LN_FC_OFF,
                        g + LN_FC_OFF + (size_t)LN_OUT * LN_FEAT);
-    hipLaunchKernelGGL(k_ln_fc_bwd_in, dim3(n), dim3(256), 0, st, N.a4, w + LN_FC_OFF, N.dlog, n, N.d4);
+    hipLaunchKernelGGL(k_ln_fc_bwd_in, dim3(n), dim3(256), 0, st, N.act.a4, w + LN_FC_OFF, N.dlog, n, N.d4);
     // layer 4 (input a3, 8x8): weights, then the gradient at a3 (masked by its ReLU)
-    hipLaunchKernelGGL((k_ln_conv_bwd_w<32, 8>), dim3(n), dim3(256), 0, st, N.a3, N.d4, n, g + LN_W_OFF[3], g + LN_W_OFF[3] + 32 * LN_KP);
-    hipLaunchKernelGGL((k_ln_conv_bwd_in<32, 8>), dim3(n), dim3(256), 0, st, N.a3, w + LN_W_OFF[3], N.d4, n, N.d3);
-    hipLaunchKernelGGL((k_ln_conv_bwd_w<32, 16>), dim3(n), dim3(256), 0, st, N.a2, N.d3, n, g + LN_W_OFF[2], g + LN_W_OFF[2] + 32 * LN_KP);
-    hipLaunchKernelGGL((k_ln_conv_bwd_in<32, 16>), dim3(n), dim3(256), 0, st, N.a2, w + LN_W_OFF[2], N.d3, n, N.d2);
-    hipLaunchKernelGGL((k_ln_conv_bwd_w<32, 32>), dim3(n), dim3(256), 0, st, N.a1, N.d2, n, g + LN_W_OFF[1], g + LN_W_OFF[1] + 32 * LN_KP);
-    hipLaunchKernelGGL((k_ln_conv_bwd_in<32, 32>), dim3(n), dim3(256), 0, st, N.a1, w + LN_W_OFF[1], N.d2, n, N.d1);
-    hipLaunchKernelGGL((k_ln_conv_bwd_w<2, 64>), dim3(n), dim3(256), 0, st, N.x0, N.d1, n, g + LN_W_OFF[0], g + LN_W_OFF[0] + 32 * LN_KP1);
+    hipLaunchKernelGGL((k_ln_conv_bwd_w<32, 8>), dim3(n), dim3(256), 0, st, N.act.a3, N.d4, n, g + LN_W_OFF[3], g + LN_W_OFF[3] + 32 * LN_KP);
+    hipLaunchKernelGGL((k_ln_conv_bwd_in<32, 8>), dim3(n), dim3(256), 0, st, N.act.a3, w + LN_W_OFF[3], N.d4, n, N.d3);
+    hipLaunchKernelGGL((k_ln_conv_bwd_w<32, 16>), dim3(n), dim3(256), 0, st, N.act.a2, N.d3, n, g + LN_W_OFF[2], g + LN_W_OFF[2] + 32 * LN_KP);
+    hipLaunchKernelGGL((k_ln_conv_bwd_in<32, 16>), dim3(n), dim3(256), 0, st, N.act.a2, w + LN_W_OFF[2], N.d3, n, N.d2);
+    hipLaunchKernelGGL((k_ln_conv_bwd_w<32, 32>), dim3(n), dim3(256), 0, st, N.act.a1, N.d2, n, g + LN_W_OFF[1], g + LN_W_OFF[1] + 32 * LN_KP);
+    hipLaunchKernelGGL((k_ln_conv_bwd_in<32, 32>), dim3(n), dim3(256), 0, st, N.act.a1, w + LN_W_OFF[1], N.d2, n, N.d1);
+    hipLaunchKernelGGL((k_ln_conv_bwd_w<2, 64>), dim3(n), dim3(256), 0, st, N.act.x0, N.d1, n, g + LN_W_OFF[0], g + LN_W_OFF[0] + 32 * LN_KP1);
     if (hipGetLastError() != hipSuccess) return -1;
     if (loss && hipMemcpyAsync(loss, N.loss, 4, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
     return hipStreamSynchronize(st) == hipSuccess ? 0 : -1;

@@ -2,12 +2,14 @@
 
 The network (av1gpu_amd/csrc/k_learn.hip) sees the 64x64 luma of a superblock and the qindex and predicts
   * 85 "expand" outputs, one per square partition node (the 64x64 node, four 32x32, sixteen 16x16, sixty-four 8x8, raster
-    order per level): 1 when the exhaustive search's leaves inside the node are at most half its size in both dimensions
-    (its children are used);
+    order per level): 1 when some leaf of the exhaustive search inside the node is at most half its size in both
+    dimensions (its children are used);
+  * 85 "own" outputs, same nodes: 1 when some leaf inside the node has the node's size as its larger side (one of the
+    node's own partition shapes: N x N, N x N/2, N/2 x N, N x N/4, N/4 x N);
   * 13 luma-mode outputs: 1 when some block of the superblock uses that luma mode in the exhaustive result.
-The encoder then analyses a node's children unless the network is confident (probability of use at most 1 - conf) that
-they are unused, and screens only the 8 most likely modes where it is confident (the product of 1 - p over the 5 others
-at least conf) that none of the others is used.
+The encoder then analyses a node's children / own shapes unless the network is confident (probability at most
+1 - conf) that they are unused (a node would lose both: it keeps its own shapes), and screens only the 8 most likely
+modes where it is confident (the product of 1 - p over the 5 others at least conf) that none of the others is used.
 
 Training data are the exhaustive search's own decisions on seeded synthetic images; training runs on the GPU with the
 library's forward (MFMA) and backward kernels.  ``ref_conv`` / ``ref_fc`` / ``ref_grads`` restate the same network in
@@ -22,7 +24,7 @@ import numpy as np
 
 import av1gpu_amd as A
 
-N_OUT, N_SPLIT, N_FEAT = 98, 85, 512
+N_SPLIT, OWN0, MODE0, N_OUT, N_FEAT = 85, 85, 170, 183, 512   # node count / first own / first mode output
 LEVEL_BASE = (0, 1, 5, 21)
 KP = (32, 288, 288, 288)
 CIN = (2, 32, 32, 32)
@@ -59,7 +61,7 @@ def init_params(seed=1):
 
 
 def unpack(p):
-    """Views of the parameter vector: [(W_l [32][KP_l], b_l [32]) for l in 0..3], W5 [98][512], b5 [98]."""
+    """Views of the parameter vector: [(W_l [32][KP_l], b_l [32]) for l in 0..3], W5 [183][512], b5 [183]."""
     out, off = [], 0
     for l in range(4):
         w = p[off:off + 32 * KP[l]].reshape(32, KP[l])
@@ -82,8 +84,8 @@ def sb_tiles(y, W, H):
 
 
 def sb_labels(blocks, W, H):
-    """Labels and label masks [nSb, 98] of a record's leaves (see the module docstring); nodes reaching past the frame
-    are masked out (the encoder expands them regardless)."""
+    """Labels and label masks [nSb, 183] of a record's leaves (see the module docstring); nodes reaching past the
+    frame are masked out (the encoder keeps their children and own shapes regardless)."""
     MiCols, MiRows = 2 * ((W + 7) >> 3), 2 * ((H + 7) >> 3)
     sbc, sbr = (MiCols + 15) >> 4, (MiRows + 15) >> 4
     lab = np.zeros((sbr * sbc, N_OUT), np.float32)
@@ -97,18 +99,22 @@ def sb_labels(blocks, W, H):
     m = np.maximum(bw, bh)
     for lv in range(4):
         n = 64 >> lv
-        sel = n >= 2 * m
         node = LEVEL_BASE[lv] + (ly // n) * (64 // n) + lx // n
+        sel = n >= 2 * m
         lab[sb[sel], node[sel]] = 1.0
+        sel = n == m
+        lab[sb[sel], OWN0 + node[sel]] = 1.0
         per = 1 << lv
         for ny in range(per):
             for nx in range(per):
                 nd = LEVEL_BASE[lv] + ny * per + nx
                 sxs = (np.arange(sbr * sbc) % sbc) * 64 + nx * n
                 sys = (np.arange(sbr * sbc) // sbc) * 64 + ny * n
-                msk[(sxs + n > FW) | (sys + n > FH), nd] = 0.0
+                out = (sxs + n > FW) | (sys + n > FH)
+                msk[out, nd] = 0.0
+                msk[out, OWN0 + nd] = 0.0
     # modes: used by some block of the superblock
-    lab[sb, N_SPLIT + blocks["y_mode"].astype(np.int64)] = 1.0
+    lab[sb, MODE0 + blocks["y_mode"].astype(np.int64)] = 1.0
     return lab, msk
 
 
@@ -182,7 +188,8 @@ class Learner:
 
 def decide(logits, W, H, conf):
     """The decisions the encoder takes from the logits of an image's superblocks (raster order): expand [nSb, 85] bool
-    (a node's children are analysed) and the luma-mode mask [nSb] (bit m: mode m screened)."""
+    (a node's children are analysed), own [nSb, 85] bool (its own shapes are) and the luma-mode mask [nSb] (bit m:
+    mode m screened)."""
     z = np.asarray(logits, np.float64)
     MiCols, MiRows = 2 * ((W + 7) >> 3), 2 * ((H + 7) >> 3)
     FW, FH = 4 * MiCols, 4 * MiRows
@@ -191,6 +198,7 @@ def decide(logits, W, H, conf):
     sx, sy = (idx % sbc) * 64, (idx // sbc) * 64
     p = 1.0 / (1.0 + np.exp(-z))
     expand = np.zeros((len(z), N_SPLIT), bool)
+    own = np.zeros((len(z), N_SPLIT), bool)
     for lv in range(4):
         n, per = 64 >> lv, 1 << lv
         for ny in range(per):
@@ -198,14 +206,15 @@ def decide(logits, W, H, conf):
                 nd = LEVEL_BASE[lv] + ny * per + nx
                 edge = (sx + nx * n + n > FW) | (sy + ny * n + n > FH)
                 expand[:, nd] = edge | (p[:, nd] > 1.0 - conf)
-    pm = p[:, N_SPLIT:]
+                own[:, nd] = edge | (p[:, OWN0 + nd] > 1.0 - conf) | ~expand[:, nd]
+    pm = p[:, MODE0:]
     order = np.argsort(-pm, axis=1, kind="stable")      # ties: the lower mode first
     top, rest = order[:, :8], order[:, 8:]
     none = np.prod(1.0 - np.take_along_axis(pm, rest, 1), axis=1)
     mask = np.full(len(z), 0x1FFF, np.int64)
     bits = np.sum(1 << top, axis=1)
     mask[none >= conf] = bits[none >= conf]
-    return expand, mask
+    return expand, own, mask
 
 
 PARAMS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "shortlist_params.npy")

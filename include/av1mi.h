@@ -152,7 +152,8 @@ int av1mi_stream_create(void **stream);
 int av1mi_stream_destroy(void *stream);
 int av1mi_stream_sync(void *stream);
 /* Upper bound of the device bytes one frame of W x H occupies during av1mi_encode (its padded input planes, the
- * encode's arena share and its compacted records): the admission unit of a serving loop. */
+ * encode's arena share including the pruning network's activations and masks, and its compacted records): the
+ * admission unit of a serving loop. */
 size_t av1mi_frame_bytes(int W, int H, int bitDepth);
 /* compute units of the current device (0 if there is none) */
 int av1mi_compute_units(void);

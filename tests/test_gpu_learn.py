@@ -73,8 +73,8 @@ def test_training_lowers_the_loss():
     # a learnable target: the node outputs follow the tile's variance, the modes a fixed pattern
     var = tiles.reshape(len(tiles), -1).astype(np.float64).std(1)
     lab = np.zeros((len(tiles), Ln.N_OUT), np.float32)
-    lab[:, :Ln.N_SPLIT] = (var > np.median(var))[:, None]
-    lab[:, Ln.N_SPLIT:] = rng.dirichlet(np.ones(13))
+    lab[:, :Ln.MODE0] = (var > np.median(var))[:, None]
+    lab[:, Ln.MODE0:] = rng.dirichlet(np.ones(13))
     L = Ln.Learner(Ln.init_params(13))
     L.set_data(tiles, q, lab, np.ones_like(lab))
     first = L.step(0, len(tiles), 3e-3)
@@ -84,12 +84,13 @@ def test_training_lowers_the_loss():
     L.close()
 
 
-def _params_bias(split_bias, mode_bias):
+def _params_bias(split_bias, mode_bias, own_bias=20.0):
     p = Ln.init_params(1)
     convs, w5, b5 = Ln.unpack(p)
     w5[:] = 0.0
     b5[:Ln.N_SPLIT] = split_bias
-    b5[Ln.N_SPLIT:] = mode_bias
+    b5[Ln.OWN0:Ln.MODE0] = own_bias
+    b5[Ln.MODE0:] = mode_bias
     return p
 
 
@@ -155,13 +156,13 @@ def test_network_chunks_and_batching_do_not_change_records(monkeypatch):
 
 
 def test_records_obey_the_restated_decisions():
-    """Every block of a pruned record lies in a node the decisions (learn.decide over the network's logits) expand and
-    uses a luma mode of its superblock's mask; a confidence where the masks are mixed is used (some nodes pruned, some
-    superblocks shortlisted)."""
+    """Every block of a pruned record lies in a node the decisions (learn.decide over the network's logits) expand, is
+    one of the own shapes of a node that keeps them, and uses a luma mode of its superblock's mask; a confidence where
+    the masks are mixed is used (some nodes pruned, some superblocks shortlisted)."""
     p = Ln.init_params(22)
     convs, w5, b5 = Ln.unpack(p)
     w5 *= 4.0
-    b5[Ln.N_SPLIT:] = np.linspace(6.0, -6.0, 13)            # a few modes likely, the rest not: shortlists appear
+    b5[Ln.MODE0:] = np.linspace(6.0, -6.0, 13)              # a few modes likely, the rest not: shortlists appear
     W, H = 328, 264
     planes = A.synth_image(1200, W, H)
     L = Ln.Learner(p)
@@ -173,8 +174,8 @@ def test_records_obey_the_restated_decisions():
     conf = 0.8
     Ln.set_shortlist(enc, p, conf)
     g = enc.encode([f])[0][0]
-    expand, mmask = Ln.decide(logits, W, H, conf)
-    assert 0 < expand[:, 1:].mean() < 1 and (mmask != 0x1FFF).any()
+    expand, own, mmask = Ln.decide(logits, W, H, conf)
+    assert 0 < expand[:, 1:].mean() < 1 and 0 < own.mean() < 1 and (mmask != 0x1FFF).any()
     sbc = (2 * ((W + 7) >> 3) + 15) >> 4
     w4 = np.array([1 << v for v in Ln._MI_W_LOG2])
     h4 = np.array([1 << v for v in Ln._MI_H_LOG2])
@@ -183,10 +184,13 @@ def test_records_obey_the_restated_decisions():
         sb = (y // 64) * sbc + x // 64
         assert (mmask[sb] >> int(b["y_mode"])) & 1
         M = 4 * max(w4[s], h4[s])
+        lvl = {64: 0, 32: 1, 16: 2, 8: 3}
         if M < 64:
             n = 2 * M
-            lv = {64: 0, 32: 1, 16: 2, 8: 3}[n]
-            node = Ln.LEVEL_BASE[lv] + ((y % 64) // n) * (64 // n) + (x % 64) // n
+            node = Ln.LEVEL_BASE[lvl[n]] + ((y % 64) // n) * (64 // n) + (x % 64) // n
             assert expand[sb, node], (x, y, s)
+        if M > 4:
+            node = Ln.LEVEL_BASE[lvl[M]] + ((y % 64) // M) * (64 // M) + (x % 64) // M
+            assert own[sb, node], (x, y, s)
     enc.close()
     f.close()

@@ -1,8 +1,9 @@
 """Training labels of the pruning network (av1gpu_amd/learn.py sb_labels / sb_tiles) against a raster restatement
 over oracle records: every 4x4 unit of the coded area is painted with the size and luma mode of the block that covers
 it (which also checks that the blocks tile the area exactly once, pinning the block-size tables), and a node's expand
-label is then "some unit inside it belongs to a block at most half the node's size", its mode labels the modes that
-appear among the painted units."""
+label is then "some unit inside it belongs to a block at most half the node's size", its own label "some unit inside it
+belongs to a block whose larger side is the node's size", its mode labels the modes that appear among the painted
+units."""
 import numpy as np
 import pytest
 
@@ -45,16 +46,19 @@ def test_labels_match_raster_restatement(oracle, W, H, q, seed):
                     node = Ln.LEVEL_BASE[lv] + ny * (1 << lv) + nx
                     y0, x0 = r0 + ny * n4, c0 + nx * n4
                     inside = y0 + n4 <= mir and x0 + n4 <= mic
-                    assert msk[s, node] == (1.0 if inside else 0.0)
+                    assert msk[s, node] == (1.0 if inside else 0.0) and msk[s, Ln.OWN0 + node] == msk[s, node]
                     sub = size[y0:y0 + n4, x0:x0 + n4]
                     if sub.size:
                         assert lab[s, node] == (1.0 if sub.min() * 2 <= 4 * n4 else 0.0), (s, lv, ny, nx)
+                        assert lab[s, Ln.OWN0 + node] == (1.0 if (sub == 4 * n4).any() else 0.0), (s, lv, ny, nx)
+                        if sub.max() <= 4 * n4:        # not inside a larger block: its children or own shapes used
+                            assert lab[s, node] + lab[s, Ln.OWN0 + node] >= 1.0
                     if lv and lab[s, node]:            # children used implies the parent's children used
                         parent = Ln.LEVEL_BASE[lv - 1] + (ny // 2) * (1 << (lv - 1)) + nx // 2
                         assert lab[s, parent] == 1.0
         m = mode[r0:r0 + 16, c0:c0 + 16].ravel()
         used = (np.bincount(m, minlength=13)[:13] > 0).astype(np.float32)
-        assert np.array_equal(lab[s, Ln.N_SPLIT:], used)
+        assert np.array_equal(lab[s, Ln.MODE0:], used)
 
 
 def test_tiles_are_the_padded_superblocks():
