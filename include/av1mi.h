@@ -113,9 +113,11 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
 void av1mi_frame_out_free(av1mi_frame_out *out);
 
 /* Learned candidate pruning (no counterpart in the reference, whose pruning stages are heuristics:
- * encoder/src/k_leafprep.cu, an_stage.h, gp.h:136-143).  A BF16 CNN per 64x64 superblock predicts the partition nodes
- * whose children the analysis needs and a top-8 luma-mode shortlist; below the confidence the search stays exhaustive
- * (per node, and for the shortlist per superblock).  params: av1mi_ln_param_count() floats (NULL disables). */
+ * encoder/src/k_leafprep.cu, an_stage.h, gp.h:136-143).  A BF16 CNN per 64x64 superblock predicts, per square
+ * partition node, whether the analysis needs its children and its own shapes, and a top-8 luma-mode shortlist; below
+ * the confidence the search stays exhaustive (per node, and for the shortlist per superblock).  params:
+ * av1mi_ln_param_count() floats (NULL disables).  The network runs in the encode call's stream-ordered memory,
+ * AV1MI_LN_CHUNK superblocks at a time (environment, read here; default 65536, ~100 KB of activations each). */
 int av1mi_set_shortlist(av1mi_encoder *enc, const float *params, float confidence, char *err, int errlen);
 size_t av1mi_ln_param_count(void);
 void av1mi_ln_init_params(float *params, uint64_t seed);
@@ -131,7 +133,8 @@ int av1mi_ln_gradient(av1mi_learner *l, uint32_t first, uint32_t n, float *loss,
 int av1mi_ln_train_step(av1mi_learner *l, uint32_t first, uint32_t n, float lr, float weight_decay, float *loss);
 
 /* The bitstream of a record (the reference's host writer, av1core/src/encoder.cpp:1072-1659 with avif.cpp):
- * av1mi_write_obus gives the sequence header and frame OBUs of the still picture (default CDFs, no post-filters),
+ * av1mi_write_obus gives the sequence header and frame OBUs of the still picture (default initial CDFs adapted per
+ * symbol, no post-filters),
  * av1mi_write_avif the same wrapped in an AVIF file.  edge_filter: enable_intra_edge_filter of the encode.  *data is
  * malloc'd (release with av1mi_buffer_free).  0, or -1 with a message when the record is not codable. */
 int av1mi_write_obus(const av1mi_frame_out *rec, int edge_filter, void **data, size_t *size, char *err, int errlen);
