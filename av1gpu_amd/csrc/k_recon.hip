@@ -304,7 +304,7 @@ __device__ EvalOut grp_task_big(lds<RcShared> &S, lds<RcTeam> &X, const Img &I, 
 
 // ---- exact inverse transform + reconstruction by the workgroup (spec 7.13.3, 7.12.3 step 3) ----
 template <int LW>
-__device__ uint32_t inv_row(int i, int W, int th, int tw, const lds<int16_t> *q, int dcQ, int acQ, int dsh, int bd, bool rect, int rk,
+__device__ __forceinline__ uint32_t inv_row(int i, int W, int th, int tw, const lds<int16_t> *q, int dcQ, int acQ, int dsh, int bd, bool rect, int rk,
                             int rowShift, lds<int32_t> *res) {
     constexpr int N = 1 << LW;
     int32_t Tv[N];
@@ -322,7 +322,7 @@ __device__ uint32_t inv_row(int i, int W, int th, int tw, const lds<int16_t> *q,
     return err;
 }
 template <int LH>
-__device__ uint32_t inv_col(int j, int W, int ck, int bd, lds<int32_t> *res) {
+__device__ __forceinline__ uint32_t inv_col(int j, int W, int ck, int bd, lds<int32_t> *res) {
     constexpr int N = 1 << LH;
     int32_t Tv[N];
 #pragma unroll
@@ -336,7 +336,7 @@ __device__ uint32_t inv_col(int j, int W, int ck, int bd, lds<int32_t> *res) {
 
 // dequantize + inverse + add the levels q (LDS, modified by the repair) onto dst; returns the number of repairs
 template <typename Px>
-__device__ int wg_inverse_add(lds<RcShared> &S, lds<int16_t> *q, int txSz, int txType, int bd, int dcQ, int acQ, Px *dst, int pitch) {
+__device__ __forceinline__ int wg_inverse_add(lds<RcShared> &S, lds<int16_t> *q, int txSz, int txType, int bd, int dcQ, int acQ, Px *dst, int pitch) {
     const int tid = threadIdx.x, lw = av1t::Tx_Width_Log2[txSz], lh = av1t::Tx_Height_Log2[txSz], W = 1 << lw, H = 1 << lh;
     const int tw = W > 32 ? 32 : W, th = H > 32 ? 32 : H, n = tw * th, den = dq_denom(txSz), dsh = den == 4 ? 2 : den == 2 ? 1 : 0;
     const bool rect = lw - lh == 1 || lh - lw == 1;

@@ -31,6 +31,20 @@ METRIC = "MP/s at qindex 120, batched+learned vs plain, at <=+1% BD-rate; p99 la
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(kernel, name="counters_batched_learned_256.md"):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE of the same
+    workload, calibrated against a copy kernel of known size), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 3 and w[0] == "traffic_bytes_per_launch" and w[1] == kernel:
+                    return int(w[2])
+    except OSError:
+        pass
+    return None
+
+
 def alg_bytes(recs, bps=1):
     """Algorithmic HBM bytes of the closed loop over records: planes read + reconstruction written (4:2:0), 2 bytes
     per coefficient, 16 per block and tx-block entry."""
@@ -120,7 +134,9 @@ def main():
         "schedule": {"closed_loop": "persistent queue" if last.schedule == 0 else "launch per wavefront step",
                      "workgroups": last.queueBlocks},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 4), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel) if launches == 1 else None,
+                     "traffic_source": "profiles/counters_batched_learned_256.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, "
+                                       "factors from a 1 GiB copy)",
                      "per_launch_ms": round(per_launch_ms, 4), "alg_bytes_per_launch": round(ab / launches)},
         "plain_single_image": plain,
     }
