@@ -170,6 +170,7 @@ class GpFrameOut:
     recon: list
     tile_cols: int
     tile_rows: int
+    chroma_format: int = 0       # 0 = 4:2:0 (or monochrome), 1 = 4:4:4
 
 
 @dataclass
@@ -226,15 +227,17 @@ def synth_image(seed, W, H, ssx=1, bd=8):
 
 def write_bitstream(rec, edge_filter=1, avif=True, monochrome=0):
     """The AV1 bitstream of a record (GpFrameOut, or a dict with width, height, base_q_idx, bit_depth, blocks,
-    block_txb, txb, coeffs, tile_cols, tile_rows): an AVIF file (avif=True) or the bare OBUs.  Host only."""
+    block_txb, txb, coeffs, tile_cols, tile_rows, and chroma_format 0 = 4:2:0 / 1 = 4:4:4, default 0): an AVIF file
+    (avif=True) or the bare OBUs.  Host only."""
     g = (lambda k: rec[k]) if isinstance(rec, dict) else (lambda k: getattr(rec, k))
+    cf = rec.get("chroma_format", 0) if isinstance(rec, dict) else getattr(rec, "chroma_format", 0)
     blocks = np.ascontiguousarray(g("blocks"))
     bt = np.ascontiguousarray(g("block_txb"), dtype=np.uint32)
     txb = np.ascontiguousarray(g("txb"))
     co = np.ascontiguousarray(g("coeffs"), dtype=np.int16)
     o = _FrameOut()
     o.width, o.height, o.base_q_idx, o.bit_depth = g("width"), g("height"), g("base_q_idx"), g("bit_depth")
-    o.chroma_format, o.monochrome = 0, monochrome
+    o.chroma_format, o.monochrome = cf, monochrome
     o.n_blocks, o.n_txb, o.n_coeffs = len(blocks), len(txb), len(co)
     o.blocks, o.block_txb, o.txb, o.coeffs = blocks.ctypes.data, bt.ctypes.data, txb.ctypes.data, co.ctypes.data
     o.tile_cols, o.tile_rows = g("tile_cols"), g("tile_rows")
@@ -399,7 +402,7 @@ class GpEncoder:
                         w = o.recon_pitch[p]
                         rec.append(_owned(o.recon[p], w * h, np.uint16 if bps == 2 else np.uint8, own).reshape(h, w))
                 res.append(GpFrameOut(o.width, o.height, o.base_q_idx, o.bit_depth, blocks, bt, txb, co, rec, o.tile_cols,
-                                      o.tile_rows))
+                                      o.tile_rows, o.chroma_format))
         finally:
             for i in range(n):
                 lib().av1mi_frame_out_free(C.byref(outs[i]))

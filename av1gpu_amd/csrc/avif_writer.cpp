@@ -178,7 +178,8 @@ public:
         W = (int)R.width; H = (int)R.height;
         MiCols = 2 * ((W + 7) >> 3); MiRows = 2 * ((H + 7) >> 3);
         mono = R.monochrome; bd = R.bit_depth == 10 ? 10 : 8;
-        if (R.chroma_format != 0 && !mono) throw WriteError("only 4:2:0 and monochrome records are written");
+        if (R.chroma_format > 1) throw WriteError("chroma format must be 4:2:0 (0) or 4:4:4 (1)");
+        ss = (R.chroma_format == 1 && !mono) ? 0 : 1;   // 4:4:4: profile 1, chroma at full resolution
         if (R.base_q_idx < 1 || R.base_q_idx > 255) throw WriteError("base_q_idx outside 1..255");
         qctx = R.base_q_idx <= 20 ? 0 : R.base_q_idx <= 60 ? 1 : R.base_q_idx <= 120 ? 2 : 3;
         tiles = tile_info(MiCols, MiRows);
@@ -218,12 +219,13 @@ public:
         return out;
     }
     std::vector<uint8_t> seqObu;   // the sequence header OBU (for the av1C box)
-    int profile() const { return 0; }
+    int profile() const { return ss ? 0 : 1; }
+    int subsampling() const { return ss; }
     int level() const { return 31; }
 
 private:
     const av1mi_frame_out &R;
-    int edge, W, H, MiCols, MiRows, mono, bd, qctx;
+    int edge, W, H, MiCols, MiRows, mono, bd, qctx, ss = 1;
     size_t seqLen = 0;
     Tiles tiles;
     std::vector<uint8_t> mSize, mSkip, mYMode;
@@ -246,7 +248,7 @@ private:
     // sequence_header_obu, reduced still picture header (5.5)
     Bits sequence_header() {
         Bits s;
-        s.put(0, 3);                 // seq_profile: Main (8/10-bit 4:2:0, monochrome)
+        s.put(profile(), 3);         // seq_profile: 0 Main (8/10-bit 4:2:0, monochrome), 1 High (8/10-bit 4:4:4)
         s.put(1, 1);                 // still_picture
         s.put(1, 1);                 // reduced_still_picture_header
         s.put(level(), 5);           // seq_level_idx[0]
@@ -262,14 +264,14 @@ private:
         s.put(0, 1);                 // enable_cdef
         s.put(0, 1);                 // enable_restoration
         s.put(bd == 10, 1);          // high_bitdepth
-        s.put(mono, 1);              // mono_chrome
+        if (profile() != 1) s.put(mono, 1);   // mono_chrome (absent in profile 1)
         s.put(1, 1);                 // color_description_present_flag
         s.put(1, 8);                 // color_primaries BT.709
         s.put(13, 8);                // transfer_characteristics sRGB
         s.put(6, 8);                 // matrix_coefficients BT.601
         s.put(1, 1);                 // color_range: full
         if (!mono) {
-            s.put(0, 2);             // chroma_sample_position: unknown
+            if (ss) s.put(0, 2);     // chroma_sample_position: unknown (4:2:0 only; profile 1 implies 4:4:4)
             s.put(0, 1);             // separate_uv_delta_q
         }
         s.put(0, 1);                 // film_grain_params_present
@@ -433,7 +435,7 @@ private:
         const uint32_t t0 = R.block_txb[blk], t1 = R.block_txb[blk + 1];
         blk++;
         const int availU = inside(r - 1, c), availL = inside(r, c - 1);
-        const int hasC = has_chroma(bsize, r, c, mono, 1);
+        const int hasC = has_chroma(bsize, r, c, mono, ss);
         // intra_frame_mode_info: skip, y mode, angle delta, uv mode (+ CfL), uv angle delta
         const int sctx = (availU ? at(mSkip, r - 1, c) : 0) + (availL ? at(mSkip, r, c - 1) : 0);
         ec.adapt(b.skip ? 1 : 0, cdf.skip[sctx], 2);
@@ -462,9 +464,9 @@ private:
         const int bh = bh4(bsize), bw = bw4(bsize);
         if (b.skip) {   // reset_block_context
             for (int p = 0; p < 1 + 2 * hasC; p++) {
-                const int ss = p ? 1 : 0;
-                for (int i = c >> ss; i < ((c + bw) >> ss); i++) aLev[p][i] = aDc[p][i] = 0;
-                for (int i = (r & 15) >> ss; i < (((r & 15) + bh) >> ss); i++) lLev[p][i] = lDc[p][i] = 0;
+                const int sp = p ? ss : 0;
+                for (int i = c >> sp; i < ((c + bw) >> sp); i++) aLev[p][i] = aDc[p][i] = 0;
+                for (int i = (r & 15) >> sp; i < (((r & 15) + bh) >> sp); i++) lLev[p][i] = lDc[p][i] = 0;
             }
         }
         const int rEnd = std::min(r + bh, MiRows), cEnd = std::min(c + bw, MiCols);
@@ -482,14 +484,14 @@ private:
 
     // coeffs() (5.11.39) of one tx block, and the level / dc-sign contexts it leaves
     void coeffs(RangeEnc &ec, const RecBlock &b, const RecTxb &t, int bsize) {
-        const int plane = t.plane, txSz = t.tx_size, pt = plane > 0, ss = plane ? 1 : 0;
+        const int plane = t.plane, txSz = t.tx_size, pt = plane > 0, sp = plane ? ss : 0;
         const int x4 = t.x4, y4 = t.y4, w4 = tx_w(txSz) >> 2, h4 = tx_h(txSz) >> 2;
-        const int maxX4 = MiCols >> ss, maxY4 = MiRows >> ss;
+        const int maxX4 = MiCols >> sp, maxY4 = MiRows >> sp;
         const int ctxSz = txsz_ctx_of(txSz);
         const int16_t *q = R.coeffs + t.coeff_off;
         std::vector<uint8_t> &aL = aLev[plane], &aD = aDc[plane], &lL = lLev[plane], &lD = lDc[plane];
         // the left arrays hold the current superblock row: plane 4x4 row modulo the superblock height
-        auto lidx = [&](int k) { return (y4 + k) & ((16 >> ss) - 1); };
+        auto lidx = [&](int k) { return (y4 + k) & ((16 >> sp) - 1); };
         int ctx;
         if (plane == 0) {
             int top = 0, left = 0;
@@ -506,7 +508,7 @@ private:
             int above = 0, left = 0;
             for (int k = 0; k < w4; k++) if (x4 + k < maxX4) above |= aL[x4 + k] | aD[x4 + k];
             for (int k = 0; k < h4; k++) if (y4 + k < maxY4) left |= lL[lidx(k)] | lD[lidx(k)];
-            const int ub = uv_bsize(bsize, 1);
+            const int ub = uv_bsize(bsize, ss);
             ctx = (above != 0) + (left != 0) + 7 + (16 * bw4(ub) * bh4(ub) > tx_w(txSz) * tx_h(txSz) ? 3 : 0);
         }
         const int eob = t.eob;
@@ -643,7 +645,7 @@ std::vector<uint8_t> box(const char *type, const std::vector<uint8_t> &payload, 
 }
 
 std::vector<uint8_t> avif(const std::vector<uint8_t> &obus, const std::vector<uint8_t> &seqObu, int W, int H, int bd, int mono,
-                          int level) {
+                          int level, int profile, int ss) {
     Box ftyp;
     ftyp.str("avif"); ftyp.u32(0); ftyp.str("avif"); ftyp.str("mif1"); ftyp.str("miaf");
     Box hdlr;
@@ -658,8 +660,8 @@ std::vector<uint8_t> avif(const std::vector<uint8_t> &obus, const std::vector<ui
     ispe.u32((uint32_t)W); ispe.u32((uint32_t)H);
     Box av1c;
     av1c.u8(0x81);                                   // marker, version 1
-    av1c.u8((0u << 5) | (uint32_t)level);            // seq_profile 0, seq_level_idx_0
-    av1c.u8(((bd == 10) << 6) | (mono << 4) | (1 << 3) | (1 << 2));   // tier 0, high_bitdepth, monochrome, 4:2:0 subsampling
+    av1c.u8(((uint32_t)profile << 5) | (uint32_t)level);   // seq_profile, seq_level_idx_0
+    av1c.u8(((bd == 10) << 6) | (mono << 4) | (ss << 3) | (ss << 2));   // tier 0, high_bitdepth, monochrome, subsampling x / y
     av1c.u8(0);                                      // no initial presentation delay
     av1c.add(seqObu);
     Box pixi;
@@ -719,7 +721,7 @@ int av1mi_write_avif(const av1mi_frame_out *rec, int edgeFilter, void **data, si
         Writer w(*rec, edgeFilter);
         const std::vector<uint8_t> o = w.obus();
         const std::vector<uint8_t> f = avif(o, w.seqObu, (int)rec->width, (int)rec->height, rec->bit_depth == 10 ? 10 : 8,
-                                            rec->monochrome ? 1 : 0, w.level());
+                                            rec->monochrome ? 1 : 0, w.level(), w.profile(), w.subsampling());
         *data = malloc(f.size());
         memcpy(*data, f.data(), f.size());
         *size = f.size();

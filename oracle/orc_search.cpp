@@ -838,15 +838,18 @@ struct PlaneEdges {
     const uint16_t *L() const { return el + EDGE_OFF; }
 };
 // edges of a tx block from the reconstruction (k_recon.cuh:750-775)
+// fA / fL: the block's AvailU / AvailL for the edge filter type when they differ from the tx block's edges (a further
+// 4:4:4 chroma tx block inside its block, k_recon.cuh:1654-1656); -1: the same
 static void rc_edges(const Frame &F, PlaneEdges &E, int plane, int px, int py, int lw, int lh, int haveL, int haveA, int haveAR,
-                     int haveBL, int mi_row, int mi_col) {
+                     int haveBL, int mi_row, int mi_col, int fA = -1, int fL = -1) {
     const int sx = plane ? F.ssx : 0, sy = plane ? F.ssy : 0;
     E.maxX = ((F.MiCols * 4) >> sx) - 1; E.maxY = ((F.MiRows * 4) >> sy) - 1;
     build_edges(F.rec[plane].data(), F.stride[plane], px, py, lw, lh, haveL, haveA, haveAR, haveBL, E.maxX, E.maxY,
                 E.ea + EDGE_OFF, E.el + EDGE_OFF, F.bd);
     E.dcv = dc_value(E.A(), E.L(), haveL, haveA, lw, lh, F.bd);
     E.haveL = haveL; E.haveA = haveA; E.x = px; E.y = py; E.lw = lw; E.lh = lh;
-    E.ft = F.edgeFilter ? filter_type(plane, mi_row, mi_col, haveA, haveL, F.mY.data(), F.mUV.data(), F.MiCols, F.ssx, F.ssy) : 0;
+    E.ft = F.edgeFilter ? filter_type(plane, mi_row, mi_col, fA < 0 ? haveA : fA, fL < 0 ? haveL : fL, F.mY.data(), F.mUV.data(), F.MiCols,
+                                      F.ssx, F.ssy) : 0;
 }
 static void pred_of(const Frame &F, const PlaneEdges &E, int c, int *pred) {
     predict_block(cand_mode(c), cand_delta(c), F.edgeFilter, E.ft, E.haveL, E.haveA, E.lw, E.lh, E.x, E.y, E.maxX, E.maxY,
@@ -1100,14 +1103,41 @@ static ChromaOut rc_chroma(Frame &F, const Params &P, SbState &S, int sb, int bs
     const bool cfl = bestUv == UV_CFL_PRED;
     const int uvTxType = chroma_tx_type(txSz, bestUv);
     int eobs[2] = {0, 0};
-    for (int pl = 1; pl <= 2; pl++) {
-        Task &t = tasks[bestT + pl - 1];
-        chroma_pred(pl, cfl ? 255 : bestUvC, pred);
-        uint16_t *dst = F.rec[pl].data() + (size_t)cy * F.stride[pl] + cx;
-        for (int i = 0; i < H; i++) for (int j = 0; j < W; j++) dst[i * F.stride[pl] + j] = (uint16_t)pred[i * W + j];
-        if (t.eob > 0) F.stats[3] += inverse_add(F, t.q.data(), txSz, uvTxType, dcQ[pl], acQ[pl], dst, F.stride[pl]) ? 1 : 0;
-        eobs[pl - 1] = record_txb(F, S, sb, pl, txSz, uvTxType, x4, y4, t.q.data());
-    }
+    // 4:4:4 blocks with a 64-sample side hold several chroma tx blocks: the decision above is the first one's; each
+    // further one (raster order, per plane) gets its edges from the reconstruction, its own contexts and a closed-loop
+    // final quantization with the chosen mode, and is not coded outside the coded area (k_recon.cuh:1636-1670)
+    const int nTxX = ssx ? 1 : (4 * bw4(bsize)) / W, nTxY = ssy ? 1 : (4 * bh4(bsize)) / H;
+    for (int pl = 1; pl <= 2; pl++)
+        for (int ty = 0; ty < nTxY; ty++)
+            for (int tx = 0; tx < nTxX; tx++) {
+                const int ox = tx * W, oy = ty * H;
+                if (x4 + (ox >> 2) >= F.MiCols || y4 + (oy >> 2) >= F.MiRows) continue;
+                uint16_t *dst = F.rec[pl].data() + (size_t)(cy + oy) * F.stride[pl] + cx + ox;
+                if (tx == 0 && ty == 0) {
+                    Task &t = tasks[bestT + pl - 1];
+                    chroma_pred(pl, cfl ? 255 : bestUvC, pred);
+                    for (int i = 0; i < H; i++) for (int j = 0; j < W; j++) dst[i * F.stride[pl] + j] = (uint16_t)pred[i * W + j];
+                    if (t.eob > 0) F.stats[3] += inverse_add(F, t.q.data(), txSz, uvTxType, dcQ[pl], acQ[pl], dst, F.stride[pl]) ? 1 : 0;
+                    eobs[pl - 1] |= record_txb(F, S, sb, pl, txSz, uvTxType, x4, y4, t.q.data());
+                    continue;
+                }
+                const int r4 = sbr4 + (oy >> 2), c4 = sbc4 + (ox >> 2);
+                PlaneEdges Et;
+                rc_edges(F, Et, pl, cx + ox, cy + oy, lw, lh, availLC || tx > 0, availUC || ty > 0, S.bdm[pl][r4 - 1 + 1][c4 + stepX + 1],
+                         S.bdm[pl][r4 + stepY + 1][c4 - 1 + 1], mi_row, mi_col, availUC, availLC);
+                int sc, dc;
+                txb_ctx(F, pl, txSz, x4 + (ox >> 2), y4 + (oy >> 2), bsize, tb, &sc, &dc);
+                pred_of(F, Et, bestUvC, pred);
+                const uint16_t *so = src[pl] + (size_t)oy * sp + ox;
+                for (int i = 0; i < H; i++) for (int j = 0; j < W; j++) res[i * W + j] = (int16_t)((int)so[i * sp + j] - pred[i * W + j]);
+                EvalIn e;
+                e.txSz = txSz; e.txType = uvTxType; e.plane = pl; e.dcQ = dcQ[pl]; e.acQ = acQ[pl]; e.rnd = P.rndC; e.zbin = P.zbinC;
+                e.skipCtx = sc; e.dcCtx = dc; e.intraDir = 0; e.bd = bd; e.C = &C; e.resEnergy = 0.f;
+                const EvalOut o = eval_txb(res, e, q);
+                for (int i = 0; i < H; i++) for (int j = 0; j < W; j++) dst[i * F.stride[pl] + j] = (uint16_t)pred[i * W + j];
+                if (o.eob > 0) F.stats[3] += inverse_add(F, q, txSz, uvTxType, dcQ[pl], acQ[pl], dst, F.stride[pl]) ? 1 : 0;
+                eobs[pl - 1] |= record_txb(F, S, sb, pl, txSz, uvTxType, x4 + (ox >> 2), y4 + (oy >> 2), q);
+            }
     return ChromaOut{bestUv, cfl ? 0 : cand_delta(bestUvC), cfl ? alpha[0] : 0, cfl ? alpha[1] : 0, eobs[0], eobs[1]};
 }
 

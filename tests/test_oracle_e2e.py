@@ -34,6 +34,25 @@ def test_oracle_10bit_record_decodes(oracle, ref):
         assert np.array_equal(r["recon"][p][:dec[p].shape[0], :dec[p].shape[1]], dec[p])
 
 
+@pytest.mark.parametrize("W,H,seed,q,bd", [(200, 136, 1, 120, 8), (256, 256, 11, 255, 8), (200, 264, 13, 240, 10), (130, 18, 4, 200, 8)])
+def test_oracle_444_record_decodes(oracle, ref, W, H, seed, q, bd):
+    """4:4:4 (BASELINE config 4's layout): chroma at full resolution, blocks with a 64-sample side carrying 2 or 4
+    chroma tx blocks (the high-qindex cases hold such blocks); the reference's backend reconstructs the record to the
+    oracle's reconstruction and dav1d decodes the reference writer's AVIF of it to the same."""
+    planes = A.synth_image(seed, W, H, ssx=0, bd=bd)
+    r = oracle.encode(planes, W, H, ssx=0, bd=bd, qindex=q, threads=4)
+    if q >= 240:
+        assert np.isin(r["blocks"]["bsize"], [10, 11, 12, 20, 21]).any()
+    dec, avif, info = ref.encode(r, W, H, q, ssx=0, bd=bd, avif=True)
+    for p in range(3):
+        assert np.array_equal(r["recon"][p][:dec[p].shape[0], :dec[p].shape[1]], dec[p]), p
+    assert np.array_equal(info["eob"], r["txb"]["eob"])
+    if refcheck.dav1d_available():
+        d = refcheck.dav1d_decode_yuv(avif)
+        for p in range(3):
+            assert np.array_equal(d[p], dec[p]), p
+
+
 def test_oracle_threads_do_not_change_the_result(oracle):
     W, H = 160, 128
     planes = A.synth_image(1020, W, H)

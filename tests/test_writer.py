@@ -12,16 +12,18 @@ import refcheck
 
 CASES = [(64, 64, 120, 1, 8, 1), (128, 96, 80, 2, 8, 1), (200, 136, 160, 3, 8, 1), (130, 18, 30, 8, 8, 1),
          (72, 40, 255, 5, 8, 1), (136, 72, 120, 50, 10, 1), (96, 64, 120, 60, 8, 0), (4160, 64, 120, 9, 8, 1)]
+# 4:4:4 (seq_profile 1): ssx = 0; the qindex-255 cases hold 64-sided blocks with several chroma tx blocks
+CASES444 = [(200, 136, 120, 11, 8, 1), (256, 256, 255, 12, 8, 1), (200, 264, 240, 13, 10, 1), (130, 18, 200, 14, 8, 0)]
 
 
-def _record(oracle, W, H, q, seed, bd, edge):
-    planes = A.synth_image(1000 + seed, W, H, bd=bd)
-    r = oracle.encode(planes, W, H, qindex=q, bd=bd, edge_filter=edge, threads=8)
+def _record(oracle, W, H, q, seed, bd, edge, ssx=1):
+    planes = A.synth_image(1000 + seed, W, H, ssx=ssx, bd=bd)
+    r = oracle.encode(planes, W, H, ssx=ssx, qindex=q, bd=bd, edge_filter=edge, threads=8)
     cols, rows = C.c_int(), C.c_int()
     cs, rs = (C.c_int * 65)(), (C.c_int * 65)()
     oracle.lib.orc_tile_layout(W, H, C.byref(cols), C.byref(rows), cs, rs)
     rec = dict(width=W, height=H, base_q_idx=q, bit_depth=bd, blocks=r["blocks"], block_txb=r["block_txb"], txb=r["txb"],
-               coeffs=r["coeffs"], tile_cols=cols.value, tile_rows=rows.value)
+               coeffs=r["coeffs"], tile_cols=cols.value, tile_rows=rows.value, chroma_format=1 - ssx)
     return rec, r["recon"]
 
 
@@ -43,6 +45,21 @@ def test_avif_decodes_to_the_reconstruction(oracle, W, H, q, seed, bd, edge):
     d = refcheck.dav1d_decode_yuv(A.write_bitstream(rec, edge_filter=edge))
     for p in range(3):
         assert np.array_equal(d[p].astype(np.uint16), recon[p][:d[p].shape[0], :d[p].shape[1]]), "plane %d" % p
+
+
+@pytest.mark.parametrize("W,H,q,seed,bd,edge", CASES444)
+def test_444_stream_parses_back_and_decodes(oracle, ref, W, H, q, seed, bd, edge):
+    rec, recon = _record(oracle, W, H, q, seed, bd, edge, ssx=0)
+    obus = A.write_bitstream(rec, edge_filter=edge, avif=False)
+    assert obus[2] >> 5 == 1                                      # seq_profile 1 (High) in the sequence header
+    p = ref.parse(obus)
+    for k in ("blocks", "block_txb", "txb", "coeffs"):
+        assert np.array_equal(p[k], rec[k]), k
+    if refcheck.dav1d_available():
+        d = refcheck.dav1d_decode_yuv(A.write_bitstream(rec, edge_filter=edge))
+        for pl in range(3):
+            assert d[pl].shape == recon[pl][:H, :W].shape or d[pl].shape[1] == W   # full-resolution chroma
+            assert np.array_equal(d[pl].astype(np.uint16), recon[pl][:d[pl].shape[0], :d[pl].shape[1]]), "plane %d" % pl
 
 
 def _boxes(b, off=0, end=None):
