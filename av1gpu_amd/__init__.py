@@ -82,7 +82,7 @@ def lib():
         L.av1mi_stream_create.argtypes = [C.POINTER(C.c_void_p)]
         L.av1mi_stream_destroy.argtypes = [C.c_void_p]
         L.av1mi_stream_sync.argtypes = [C.c_void_p]
-        L.av1mi_frame_bytes.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.av1mi_frame_bytes.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
         L.av1mi_frame_bytes.restype = C.c_size_t
         L.av1mi_compute_units.argtypes = []
         for fn in (L.av1mi_write_obus, L.av1mi_write_avif):
@@ -276,10 +276,12 @@ def batch_spec(n=256, seed=1000):
 class DeviceFrame:
     """Padded planes of one frame resident in device memory (owned; freed on close / garbage collection)."""
 
-    def __init__(self, planes, W, H, bd=8, qindex=0, stream=None):
-        """stream: upload with stream-ordered allocation and copies on that stream (freed on it too)."""
-        padded, aw, ah = pad_planes(planes, W, H)
+    def __init__(self, planes, W, H, bd=8, qindex=0, stream=None, yuv444=False):
+        """stream: upload with stream-ordered allocation and copies on that stream (freed on it too); yuv444: the chroma
+        planes are at full resolution."""
+        padded, aw, ah = pad_planes(planes, W, H, ssx=0 if yuv444 else 1)
         self.W, self.H, self.aw, self.ah, self.bd, self.qindex = W, H, aw, ah, bd, qindex
+        self.yuv444 = 1 if yuv444 else 0
         self.ptrs, self.pitch, self.stream = [], [], stream
         self._host = padded if stream else None   # pageable sources must outlive the asynchronous copies
         for a in padded:
@@ -297,7 +299,7 @@ class DeviceFrame:
 
     def frame_in(self):
         return GpFrameIn(W=self.W, H=self.H, planes=tuple(self.ptrs), spitch=tuple(self.pitch), aw=self.aw, ah=self.ah,
-                         bitDepth=self.bd, qindex=self.qindex)
+                         bitDepth=self.bd, qindex=self.qindex, yuv444=self.yuv444)
 
     def close(self):
         for p in self.ptrs:
@@ -398,7 +400,7 @@ class GpEncoder:
                 if want_recon:
                     bps = 2 if o.bit_depth > 8 else 1
                     for p in range(3):
-                        h = o.ah >> (1 if p else 0)
+                        h = o.ah >> (1 if p and o.chroma_format != 1 else 0)
                         w = o.recon_pitch[p]
                         rec.append(_owned(o.recon[p], w * h, np.uint16 if bps == 2 else np.uint8, own).reshape(h, w))
                 res.append(GpFrameOut(o.width, o.height, o.base_q_idx, o.bit_depth, blocks, bt, txb, co, rec, o.tile_cols,

@@ -35,6 +35,8 @@ constexpr int MAXK = 6;
 constexpr int MAX_LEAVES = 256;          // blocks per 64x64 superblock at most
 constexpr int SB_TXB_CAP = 384;          // tx blocks per SB (4:2:0): 256 luma + 2 x 64 chroma
 constexpr int SB_COEF_CAP = 6144;        // coefficients per SB (4:2:0): 64 x 64 + 2 x 32 x 32
+constexpr int SB_TXB_CAP_444 = 768;      // 4:4:4: 3 x 256
+constexpr int SB_COEF_CAP_444 = 12288;   // 4:4:4: 3 x 64 x 64
 constexpr int FWD_FLOATS = (16 + 64 + 256 + 1024 + 4096) + (16 + 64 + 256) + (16 + 64 + 256 + 1024);
 
 AV1M_HD int clip3(int lo, int hi, int x) { return x < lo ? lo : x > hi ? hi : x; }
@@ -565,8 +567,9 @@ struct Batch {
     SbOut *sbOut;                // [nSb]
     RecBlock *oBlocks;           // [nSb * MAX_LEAVES]
     uint16_t *oBlockTxb;         // [nSb * MAX_LEAVES]
-    RecTxb *oTxb;                // [nSb * SB_TXB_CAP]
-    int16_t *oCoeffs;            // [nSb * SB_COEF_CAP]
+    RecTxb *oTxb;                // [nSb * txbCap]
+    int16_t *oCoeffs;            // [nSb * coefCap]
+    uint32_t txbCap, coefCap;    // per-SB record slots: SB_TXB_CAP / SB_COEF_CAP, or the 4:4:4 ones
     uint32_t *stats;             // [8]: blocks, skip blocks, CfL blocks, range repairs
     const uint32_t *lnMask;      // [nSb][8] learned pruning: expanded nodes (words 0-2), screened luma modes (3), nodes
                                  // keeping their own shapes (4-6); null = exhaustive

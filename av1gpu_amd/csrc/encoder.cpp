@@ -258,10 +258,17 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     uint32_t shTotal[NSHAPES] = {0};
     std::vector<uint32_t> shStart((size_t)NSHAPES * (n + 1), 0), sbImg;
     uint32_t nSb = 0;
+    bool any444 = false;
+    int batchSs = -1;
     for (int i = 0; i < n; i++) {
         const av1mi_frame_in &f = in[i];
         if ((f.bitDepth == 10 ? 10 : 8) != bd) { snprintf(err, errlen, "frames of different bit depths in one batch"); return -1; }
-        if (f.yuv444 && !f.mono) { snprintf(err, errlen, "4:4:4 frames are not supported by this build"); return -1; }
+        const int ss = (f.yuv444 && !f.mono) ? 0 : 1;   // chroma subsampling: 0 = 4:4:4, 1 = 4:2:0 (monochrome: no chroma)
+        if (!f.mono) {
+            if (batchSs < 0) batchSs = ss;
+            else if (batchSs != ss) { snprintf(err, errlen, "frames of different chroma formats in one batch"); return -1; }
+        }
+        if (!ss) any444 = true;
         if (f.W <= 0 || f.H <= 0 || f.W > 65536 || f.H > 65536) { snprintf(err, errlen, "frame %d: bad size %dx%d", i, f.W, f.H); return -1; }
         Img &I = imgs[i];
         memset(&I, 0, sizeof I);
@@ -270,7 +277,7 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
         I.sbCols = (I.MiCols + 15) >> 4; I.sbRows = (I.MiRows + 15) >> 4;
         if (f.aw < I.sbCols * 64 || f.ah < I.sbRows * 64) { snprintf(err, errlen, "frame %d: input planes not padded to 64", i); return -1; }
         I.aw = f.aw; I.ah = f.ah;
-        I.ss = 1; I.mono = f.mono; I.bd = bd;
+        I.ss = ss; I.mono = f.mono; I.bd = bd;
         I.qindex = f.qindex > 0 ? f.qindex : enc->cfg.qindex;
         if (I.qindex < 1 || I.qindex > 255) { snprintf(err, errlen, "frame %d: qindex %d outside 1..255", i, I.qindex); return -1; }
         I.qctx = I.qindex <= 20 ? 0 : I.qindex <= 60 ? 1 : I.qindex <= 120 ? 2 : 3;
@@ -305,8 +312,8 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     size_t mapTotal = 0;
     for (int i = 0; i < n; i++) {
         const Img &I = imgs[i];
-        A.add(recMem[i], ((size_t)I.aw * I.ah + 2 * (size_t)(I.aw >> 1) * (I.ah >> 1)) * bps);
-        const size_t mi = (size_t)I.MiCols * I.MiRows, c1 = (size_t)(I.MiCols >> 1) * (I.MiRows >> 1);
+        A.add(recMem[i], ((size_t)I.aw * I.ah + 2 * (size_t)(I.aw >> I.ss) * (I.ah >> I.ss)) * bps);
+        const size_t mi = (size_t)I.MiCols * I.MiRows, c1 = (size_t)(I.MiCols >> I.ss) * (I.MiRows >> I.ss);
         A.add(mapMem[i], 5 * mi + 2 * (mi + 2 * c1));
         mapTotal += 5 * mi + 2 * (mi + 2 * c1);
     }
@@ -314,7 +321,8 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     RecBlock *dOB; uint16_t *dOBT; RecTxb *dOT; int16_t *dOC;
     A.add(dImg, n); A.add(dBlk, blkTotal); A.add(dShStart, shStart.size()); A.add(dSbImg, nSb); A.add(dLeafCount, nSb);
     A.add(dLeaves, (size_t)nSb * MAX_LEAVES); A.add(dSbJ, nSb); A.add(dSbOut, nSb); A.add(dOB, (size_t)nSb * MAX_LEAVES);
-    A.add(dOBT, (size_t)nSb * MAX_LEAVES); A.add(dOT, (size_t)nSb * SB_TXB_CAP); A.add(dOC, (size_t)nSb * SB_COEF_CAP); A.add(dStats, 8);
+    const uint32_t txbCap = any444 ? SB_TXB_CAP_444 : SB_TXB_CAP, coefCap = any444 ? SB_COEF_CAP_444 : SB_COEF_CAP;
+    A.add(dOBT, (size_t)nSb * MAX_LEAVES); A.add(dOT, (size_t)nSb * txbCap); A.add(dOC, (size_t)nSb * coefCap); A.add(dStats, 8);
     // wavefront step lists: per tile, step = (c - c0) + 2 (r - r0)
     std::vector<std::vector<uint32_t>> steps;
     for (int i = 0; i < n; i++) {
@@ -345,10 +353,10 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     tm.deviceBytes = A.total;
     for (int i = 0; i < n; i++) {
         Img &I = imgs[i];
-        const size_t ysz = (size_t)I.aw * I.ah * bps, csz = (size_t)(I.aw >> 1) * (I.ah >> 1) * bps;
+        const size_t ysz = (size_t)I.aw * I.ah * bps, csz = (size_t)(I.aw >> I.ss) * (I.ah >> I.ss) * bps;
         I.rec[0] = recMem[i]; I.rec[1] = recMem[i] + ysz; I.rec[2] = recMem[i] + ysz + csz;
-        I.pitch[0] = I.aw; I.pitch[1] = I.pitch[2] = I.aw >> 1;
-        const size_t mi = (size_t)I.MiCols * I.MiRows, c1 = (size_t)(I.MiCols >> 1) * (I.MiRows >> 1);
+        I.pitch[0] = I.aw; I.pitch[1] = I.pitch[2] = I.aw >> I.ss;
+        const size_t mi = (size_t)I.MiCols * I.MiRows, c1 = (size_t)(I.MiCols >> I.ss) * (I.MiRows >> I.ss);
         uint8_t *m = mapMem[i];
         I.mY = m; I.mUV = m + mi; I.mSkip = m + 2 * mi; I.mMiSize = m + 3 * mi; I.mTx = m + 4 * mi;
         uint8_t *c = m + 5 * mi;
@@ -369,7 +377,7 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     memset(&B, 0, sizeof B);
     B.nimg = n; B.img = dImg; B.T = enc->dT; B.blk = dBlk; B.shStart = dShStart; B.nSb = nSb; B.sbImg = dSbImg;
     B.leafCount = dLeafCount; B.leaves = dLeaves; B.sbJ = dSbJ; B.sbOut = dSbOut; B.oBlocks = dOB; B.oBlockTxb = dOBT;
-    B.oTxb = dOT; B.oCoeffs = dOC; B.stats = dStats;
+    B.oTxb = dOT; B.oCoeffs = dOC; B.stats = dStats; B.txbCap = txbCap; B.coefCap = coefCap;
     (void)hipEventRecord(enc->ev[1], st);
     if (enc->ln) {   // the pruning network runs first (its time counts as analysis)
         if (ln_masks(*enc->ln, dLnAct, dImg, dSbImg, nSb, enc->lnConf, lnN, dLnMask, st)) { snprintf(err, errlen, "pruning network failed"); A.release(st); return -1; }
@@ -440,7 +448,7 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     for (int i = 0; i < n; i++) {
         const Img &I = imgs[i];
         av1mi_frame_out &o = out[i];
-        o.width = I.W; o.height = I.H; o.base_q_idx = I.qindex; o.bit_depth = bd; o.chroma_format = 0; o.monochrome = I.mono;
+        o.width = I.W; o.height = I.H; o.base_q_idx = I.qindex; o.bit_depth = bd; o.chroma_format = I.ss ? 0 : 1; o.monochrome = I.mono;
         o.n_blocks = (uint32_t)(imgBlk[i + 1] - imgBlk[i]);
         o.n_txb = (uint32_t)(imgTxb[i + 1] - imgTxb[i]);
         o.n_coeffs = imgCoef[i + 1] - imgCoef[i];
@@ -455,7 +463,7 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
         HCK(hipMemcpyAsync(o.coeffs, rC + imgCoef[i], (size_t)o.n_coeffs * 2, hipMemcpyDeviceToHost, st));
         if (wantRecon) {
             for (int p = 0; p < 3; p++) {
-                const int pw = p ? I.aw >> 1 : I.aw, ph = p ? I.ah >> 1 : I.ah;
+                const int pw = p ? I.aw >> I.ss : I.aw, ph = p ? I.ah >> I.ss : I.ah;
                 o.recon[p] = malloc((size_t)pw * ph * bps);
                 o.recon_pitch[p] = pw;
                 HCK(hipMemcpyAsync(o.recon[p], I.rec[p], (size_t)pw * ph * bps, hipMemcpyDeviceToHost, st));
@@ -506,23 +514,24 @@ int av1mi_compute_units(void) {
     return v;
 }
 
-size_t av1mi_frame_bytes(int W, int H, int bitDepth) {
+size_t av1mi_frame_bytes(int W, int H, int bitDepth, int yuv444) {
     if (W <= 0 || H <= 0) return 0;
     const size_t bps = bitDepth == 10 ? 2 : 1;
     const size_t MiCols = 2 * (size_t)((W + 7) >> 3), MiRows = 2 * (size_t)((H + 7) >> 3);
     const size_t sbCols = (MiCols + 15) >> 4, sbRows = (MiRows + 15) >> 4, nSb = sbCols * sbRows;
-    const size_t aw = sbCols * 64, ah = sbRows * 64, mi = MiCols * MiRows, c1 = mi / 4;
+    const size_t aw = sbCols * 64, ah = sbRows * 64, mi = MiCols * MiRows, c1 = yuv444 ? mi : mi / 4;
+    const size_t txbCap = yuv444 ? SB_TXB_CAP_444 : SB_TXB_CAP, coefCap = yuv444 ? SB_COEF_CAP_444 : SB_COEF_CAP;
     size_t blocks = 0;
     for (int s = 0; s < BLOCK_INVALID; s++) {
         if (s >= 13 && s <= 15) continue;
         const size_t w = 4 * bw4(s), h = 4 * bh4(s);
         blocks += ((MiCols * 4 + w - 1) / w) * ((MiRows * 4 + h - 1) / h);
     }
-    const size_t planes = aw * ah * bps * 3 / 2;
+    const size_t planes = yuv444 ? aw * ah * bps * 3 : aw * ah * bps * 3 / 2;
     const size_t arena = planes + 5 * mi + 2 * (mi + 2 * c1) + blocks * sizeof(BlockRes) +
                          nSb * (4 + 2 + MAX_LEAVES * sizeof(Leaf) + 4 + sizeof(SbOut) + MAX_LEAVES * (sizeof(RecBlock) + 2) +
-                                SB_TXB_CAP * sizeof(RecTxb) + SB_COEF_CAP * 2 + 4 + sizeof(PackOff) + 4);
-    const size_t records = nSb * (MAX_LEAVES * (sizeof(RecBlock) + 4) + SB_TXB_CAP * sizeof(RecTxb) + SB_COEF_CAP * 2);
+                                txbCap * sizeof(RecTxb) + coefCap * 2 + 4 + sizeof(PackOff) + 4);
+    const size_t records = nSb * (MAX_LEAVES * (sizeof(RecBlock) + 4) + txbCap * sizeof(RecTxb) + coefCap * 2);
     const size_t network = nSb * 32 + ln_scratch_bytes((uint32_t)std::min<size_t>(nSb, 65536));   // masks, activations
     return planes + arena + records + network + 64 * 1024;
 }

@@ -14,11 +14,11 @@ __global__ void __launch_bounds__(256) k_pack(Batch B, const PackOff *off, RecBl
         oBT[o.gBlk + k] = o.rTxb + B.oBlockTxb[(size_t)sb * MAX_LEAVES + k];
     }
     for (uint32_t k = threadIdx.x; k < so.nTxb; k += blockDim.x) {
-        RecTxb t = B.oTxb[(size_t)sb * SB_TXB_CAP + k];
+        RecTxb t = B.oTxb[(size_t)sb * B.txbCap + k];
         t.coeff_off += o.rCoef;
         oT[o.gTxb + k] = t;
     }
-    const int16_t *src = B.oCoeffs + (size_t)sb * SB_COEF_CAP;
+    const int16_t *src = B.oCoeffs + (size_t)sb * B.coefCap;
     for (uint32_t k = threadIdx.x; k < so.nCoeffs; k += blockDim.x) oC[o.gCoef + k] = src[k];
 }
 

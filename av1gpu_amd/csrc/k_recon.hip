@@ -138,8 +138,10 @@ __device__ void grp_txb_ctx(const Img &I, int plane, int txSz, int x4, int y4, i
 
 // base edges of the tx block of plane p at (px, py) from the reconstruction (group g), DC value and filter type
 template <typename Px>
+// fA / fL: the block's AvailU / AvailL for the edge filter type where they differ from the tx block's edges (a further
+// 4:4:4 chroma tx block inside its block); -1: the same
 __device__ void rc_edges(lds<RcShared> &S, const Img &I, int p, int px, int py, int lw, int lh, int haveL, int haveA, int haveAR, int haveBL,
-                         int mi_row, int mi_col) {
+                         int mi_row, int mi_col, int fA = -1, int fL = -1) {
     const int ss = p ? I.ss : 0;
     const int maxX = ((I.MiCols * 4) >> ss) - 1, maxY = ((I.MiRows * 4) >> ss) - 1;
     lds<uint16_t> *A = S.ea[p] + EDGE_OFF, *L = S.el[p] + EDGE_OFF;
@@ -147,7 +149,7 @@ __device__ void rc_edges(lds<RcShared> &S, const Img &I, int p, int px, int py, 
     const int dcv = grp_dc_value(A, L, haveL, haveA, lw, lh, I.bd);
     if (glane() == 0) {
         S.dcv[p] = dcv; S.hvL[p] = haveL; S.hvA[p] = haveA; S.px[p] = px; S.py[p] = py; S.mx[p] = maxX; S.my[p] = maxY;
-        S.ft[p] = I.edgeFilter ? filter_type(I, p, mi_row, mi_col, haveA, haveL) : 0;
+        S.ft[p] = I.edgeFilter ? filter_type(I, p, mi_row, mi_col, fA < 0 ? haveA : fA, fL < 0 ? haveL : fL) : 0;
     }
 }
 
@@ -423,7 +425,7 @@ __device__ int wg_record_txb(lds<RcShared> &S, const Batch &B, uint32_t sb, cons
         cul = gsumi(cul);
         if (tid == 0) { S.eobOut = eob; S.culOut = min(cul, 63); }
     }
-    int16_t *oc = B.oCoeffs + (size_t)sb * SB_COEF_CAP + off;
+    int16_t *oc = B.oCoeffs + (size_t)sb * B.coefCap + off;
     for (int i = tid; i < n; i += RC_NT) oc[i] = q[i];
     __syncthreads();
     const int eob = S.eobOut, cul = S.culOut, dcCat = q[0] < 0 ? 1 : q[0] > 0 ? 2 : 0;
@@ -442,7 +444,7 @@ __device__ int wg_record_txb(lds<RcShared> &S, const Batch &B, uint32_t sb, cons
         RecTxb t;
         t.coeff_off = off; t.x4 = (uint16_t)x4; t.y4 = (uint16_t)y4; t.plane = (uint8_t)p; t.tx_size = (uint8_t)txSz;
         t.tx_type = (uint8_t)(p == 0 && eob == 0 ? DCT_DCT : txType); t.flags = 0; t.eob = (uint16_t)eob; t.reserved = 0;
-        B.oTxb[(size_t)sb * SB_TXB_CAP + S.nTxb] = t;
+        B.oTxb[(size_t)sb * B.txbCap + S.nTxb] = t;
     }
     __syncthreads();
     if (tid == 0) { S.nTxb++; S.nCoef += n; }
@@ -820,7 +822,12 @@ __device__ void rc_chroma(lds<RcShared> &S, const Batch &B, const SearchParams &
         }
     }
     __syncthreads();
-    const int win = S.win, uvTxType = S.taskT[win];
+    const int win = S.win, uvTxType = S.taskT[win], winC = S.taskC[win], nTask0 = S.nTask;
+    // 4:4:4 blocks with a 64-sample side hold several chroma tx blocks (nTx > 1): the decision above is the first
+    // one's; each further one (raster order, U's then V's) gets its edges from the reconstruction, its own contexts
+    // and a closed-loop final quantization with the chosen mode (a one-task run in slot 0 / team 0: the winning pair is
+    // (even, odd), so V's kept task survives U's further blocks), and is not coded outside the coded area
+    const int nTxX = ss ? 1 : (4 * bw4(bsize)) / W, nTxY = ss ? 1 : (4 * bh4(bsize)) / H;
     for (int p = 1; p <= 2; p++) {
         const int t = win + p - 1;
         int team;
@@ -831,7 +838,38 @@ __device__ void rc_chroma(lds<RcShared> &S, const Batch &B, const SearchParams &
             const int fixes = wg_inverse_add<Px>(S, q, txSz, uvTxType, bd, dc_q(I.qindex, bd), ac_q(I.qindex, bd), dst, I.pitch[p]);
             if (fixes && tid == 0) atomicAdd(&B.stats[3], 1u);
         }
-        const int eob = wg_record_txb(S, B, sb, I, p, txSz, uvTxType, x4, y4, q);
+        int eob = wg_record_txb(S, B, sb, I, p, txSz, uvTxType, x4, y4, q);
+        // only a chroma tx block with a 32-sample side belongs to a 64-sided 4:4:4 block
+        if constexpr (W == 32 || H == 32) for (int k = 1; k < nTxX * nTxY; k++) {
+            const int ox = (k % nTxX) * W, oy = (k / nTxX) * H;
+            if (x4 + (ox >> 2) >= I.MiCols || y4 + (oy >> 2) >= I.MiRows) continue;
+            const int r4 = sbr4 + (oy >> 2), c4 = sbc4 + (ox >> 2);
+            if (gid() == 0)
+                rc_edges<Px>(S, I, p, cx + ox, cy + oy, LW, LH, availLC || ox > 0, availUC || oy > 0, S.bdm[p][r4][c4 + (W >> 2) + 1],
+                             S.bdm[p][r4 + (H >> 2) + 1][c4], mi_row, mi_col, availUC, availLC);
+            if (gid() == 1) grp_txb_ctx(I, p, txSz, x4 + (ox >> 2), y4 + (oy >> 2), bsize, tx0, ty0, &S.skipCtx[p], &S.dcCtx[p]);
+            {
+                lds<uint16_t> *sd = S.srcb + (p == 2 ? 2048 : 0);
+                const Px *g = src_plane<Px>(I, p) + (size_t)(cy + oy) * I.spitch[p] + cx + ox;
+                for (int i = tid; i < W * H; i += RC_NT) sd[i] = g[(size_t)(i >> LW) * I.spitch[p] + (i & (W - 1))];
+            }
+            if (tid == 0) { S.taskC[0] = winC; S.taskT[0] = uvTxType; S.taskP[0] = p; S.nTask = 1; }
+            __syncthreads();
+            wg_run_tasks<W, H>(S, I, T, P, lambda, false, false);
+            int tm;
+            lds<int16_t> *qk = wg_final_levels<W, H>(S, I, T, P, 0, tm);
+            Px *dk = rec_plane<Px>(I, p) + (size_t)(cy + oy) * I.pitch[p] + cx + ox;
+            wg_final_pred<W, H, Px>(S, I, p, 0, 0, tm, dk, I.pitch[p]);
+            if (S.taskE[0] > 0) {
+                const int fixes = wg_inverse_add<Px>(S, qk, txSz, uvTxType, bd, dc_q(I.qindex, bd), ac_q(I.qindex, bd), dk, I.pitch[p]);
+                if (fixes && tid == 0) atomicAdd(&B.stats[3], 1u);
+            }
+            eob |= wg_record_txb(S, B, sb, I, p, txSz, uvTxType, x4 + (ox >> 2), y4 + (oy >> 2), qk);
+        }
+        if (nTxX * nTxY > 1) {   // V's first tx block reads the decision's tasks again
+            if (tid == 0) S.nTask = nTask0;
+            __syncthreads();
+        }
         if (tid == 0) S.winEob[p] = eob;
         __syncthreads();
     }

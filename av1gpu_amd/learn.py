@@ -322,21 +322,21 @@ def psnr_y(src, rec, W, H, bd=8):
     return 99.0 if mse == 0 else 10.0 * math.log10(peak * peak / mse)
 
 
-def rd_curve(enc, images, qs):
+def rd_curve(enc, images, qs, bd=8, yuv444=False):
     """Rate-distortion points of an encoder over images [(planes, W, H)] at each qindex of qs: total bits of this
     build's own bitstream writer (bare OBUs), mean PSNR-Y, and the encode seconds (one call per qindex, the planes
     uploaded beforehand, one untimed warm-up call on the first image)."""
     import time
     bits, psnr, secs = [], [], 0.0
     for q in qs:
-        frames = [A.DeviceFrame(p, W, H, qindex=q) for p, W, H in images]
+        frames = [A.DeviceFrame(p, W, H, bd=bd, qindex=q, yuv444=yuv444) for p, W, H in images]
         enc.encode(frames[:1])
         A.lib().av1mi_device_sync()
         t0 = time.perf_counter()
         out, _ = enc.encode(frames)
         secs += time.perf_counter() - t0
         bits.append(sum(8 * len(A.write_bitstream(o, avif=False)) for o in out))
-        psnr.append(float(np.mean([psnr_y(p[0], o.recon[0], W, H) for (p, W, H), o in zip(images, out)])))
+        psnr.append(float(np.mean([psnr_y(p[0], o.recon[0], W, H, bd) for (p, W, H), o in zip(images, out)])))
         for f in frames:
             f.close()
     return bits, psnr, secs

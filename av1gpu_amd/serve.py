@@ -25,7 +25,7 @@ class Request:
 
     def __init__(self, planes, W, H, qindex=120):
         self.planes, self.W, self.H, self.qindex = planes, W, H, qindex
-        self.bytes = int(A.lib().av1mi_frame_bytes(W, H, 8))
+        self.bytes = int(A.lib().av1mi_frame_bytes(W, H, 8, 0))
         self.t_submit = self.t_start = self.t_done = 0.0
         self.future = Future()
 

@@ -9,6 +9,7 @@ resident in device memory, the shipped network (av1gpu_amd/shortlist_params.npy)
   batched_exhaustive   the same batch without the network;
   bd_rate              BD-rate (PSNR-Y, bits of this build's own writer) of the learned search against the exhaustive
                        one on the batch's images of at most 2 MP (first 16), qindex 60/100/140/180;
+  learned_10bit_444    the same batch at 10-bit 4:4:4 (learned and exhaustive, with its BD-rate);
   serving              Poisson arrivals into the continuous-batching server on the batched+learned path, p50 / p99;
   cpu_baseline         the CPU restatement (oracle/liborc.so) on a bounded crop of the 4000 x 3000 image.
 
@@ -66,6 +67,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-bd", action="store_true", help="skip the BD-rate leg")
     ap.add_argument("--schedule", type=int, default=0, help="closed loop: 0 persistent queue, 1 one launch per wavefront step")
+    ap.add_argument("--no-444", action="store_true", help="skip the 10-bit 4:4:4 batched+learned leg")
     ap.add_argument("--serve", type=int, default=64, help="requests of the serving leg (0 = skip it)")
     ap.add_argument("--rate", type=float, default=50.0, help="serving leg: Poisson arrival rate, requests per second")
     a = ap.parse_args()
@@ -148,6 +150,8 @@ def main():
     if not a.no_bd:
         line["bd_rate"] = bd_leg(A, learn, params, a.conf, spec, bplanes)
     del bplanes
+    if not a.no_444:
+        line["learned_10bit_444"] = leg_444(A, learn, params, a.conf, a.batch, a.schedule)
     if a.serve > 0:
         line["serving"] = serving_leg(A, a.serve, a.rate, (params, a.conf))
     if not a.no_cpu and planes is not None:
@@ -225,6 +229,53 @@ def bd_leg(A, learn, params, conf, spec, bplanes, n=16, max_mp=2.0, qs=(60, 100,
     print("bd leg done: %.3f %%" % bd, file=sys.stderr, flush=True)
     return {"bd_rate_psnr_y_pct": round(float(bd), 3), "images": [spec[i][3] for i in idx],
             "megapixels": round(sum(W * H for _, W, H in images) / 1e6, 3), "qindex": list(qs), "curves": curves}
+
+
+def leg_444(A, learn, params, conf, n, schedule, qs=(60, 100, 140, 180)):
+    """The same batch at 10-bit 4:4:4 (generator scaled to [0, 1023], chroma at full resolution): one timed
+    batched+learned encode after a warm-up on 8 images, one exhaustive encode, and the BD-rate (PSNR-Y) of the learned
+    search against the exhaustive one on the first 8 images of at most 2 MP."""
+    from concurrent.futures import ThreadPoolExecutor
+    spec = A.batch_spec(n)
+    with ThreadPoolExecutor(16) as ex:
+        planes = list(ex.map(lambda s: A.synth_image(s[3], s[0], s[1], ssx=0, bd=10), spec))
+    frames = [A.DeviceFrame(p, W, H, bd=10, qindex=q, yuv444=True) for p, (W, H, q, _) in zip(planes, spec)]
+    mp = sum(W * H for W, H, _, _ in spec) / 1e6
+    res = {"workload": "%d synthetic 10-bit 4:4:4 images, same sizes / qindex / seeds as the headline batch" % n,
+           "megapixels": round(mp, 3), "unit": "MP/s"}
+    for name, c in (("learned", conf), ("exhaustive", None)):
+        enc = A.GpEncoder().init(A.GpConfig(qindex=120, schedule=schedule))
+        if c is not None:
+            learn.set_shortlist(enc, params, c)
+        enc.encode(frames[:8], want_recon=False)
+        A.lib().av1mi_device_sync()
+        t0 = time.perf_counter()
+        out, t = enc.encode(frames, want_recon=False)
+        A.lib().av1mi_device_sync()
+        dt = time.perf_counter() - t0
+        out = None
+        enc.close()
+        res[name] = {"value": round(mp / dt, 4), "seconds": round(dt, 3),
+                     "stages_ms": {"analysis": round(t.analysis, 3), "recon": round(t.recon, 3), "download": round(t.download, 3)}}
+    for f in frames:
+        f.close()
+    res["learned_speedup"] = round(res["learned"]["value"] / res["exhaustive"]["value"], 3)
+    idx = [i for i, (W, H, _, _) in enumerate(spec) if W * H <= 2e6][:8]
+    images = [(planes[i], spec[i][0], spec[i][1]) for i in idx]
+    curves = {}
+    for name, c in (("exhaustive", None), ("learned", conf)):
+        enc = A.GpEncoder().init(A.GpConfig(qindex=120))
+        if c is not None:
+            learn.set_shortlist(enc, params, c)
+        bits, psnr, _ = learn.rd_curve(enc, images, qs, bd=10, yuv444=True)
+        enc.close()
+        curves[name] = {"bits": bits, "psnr_y": [round(v, 4) for v in psnr]}
+    res["bd_rate_psnr_y_pct"] = round(float(learn.bd_rate(curves["exhaustive"]["bits"], curves["exhaustive"]["psnr_y"],
+                                                          curves["learned"]["bits"], curves["learned"]["psnr_y"])), 3)
+    res["bd_images"] = [spec[i][3] for i in idx]
+    print("4:4:4 leg done: %.1f MP/s learned, %.1f exhaustive, BD %.3f %%" % (res["learned"]["value"], res["exhaustive"]["value"],
+                                                                             res["bd_rate_psnr_y_pct"]), file=sys.stderr, flush=True)
+    return res
 
 
 def serving_leg(A, n, rate, shortlist):

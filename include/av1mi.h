@@ -157,7 +157,7 @@ int av1mi_stream_sync(void *stream);
 /* Upper bound of the device bytes one frame of W x H occupies during av1mi_encode (its padded input planes, the
  * encode's arena share including the pruning network's activations and masks, and its compacted records): the
  * admission unit of a serving loop. */
-size_t av1mi_frame_bytes(int W, int H, int bitDepth);
+size_t av1mi_frame_bytes(int W, int H, int bitDepth, int yuv444);
 /* compute units of the current device (0 if there is none) */
 int av1mi_compute_units(void);
 

@@ -39,7 +39,7 @@ SPECS = [(200, 136, 80, 7001), (96, 64, 120, 7002), (320, 240, 160, 7003), (64, 
 def test_serving_results_equal_alone(workers, budget):
     ref = _alone(SPECS)
     if budget is None:   # room for the largest single request only: admission serialises
-        budget = max(A.lib().av1mi_frame_bytes(W, H, 8) for W, H, _, _ in SPECS)
+        budget = max(A.lib().av1mi_frame_bytes(W, H, 8, 0) for W, H, _, _ in SPECS)
     srv = serve.Server(workers=workers, hbm_budget=budget, max_batch=3)
     reqs = [serve.Request(A.synth_image(s, W, H), W, H, q) for W, H, q, s in SPECS]
     # submit from two threads at once
