@@ -9,6 +9,9 @@
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <atomic>
+#include <thread>
+#include <tuple>
 #include <string>
 #include <vector>
 #include "../../include/av1mi.h"
@@ -105,7 +108,32 @@ struct av1mi_encoder {
     LnNet *ln = nullptr;              // learned pruning network (null: exhaustive search)
     float lnConf = 0.8f;
     uint32_t lnChunk = 65536;         // superblocks per network pass (bounds the activation memory, ~100 KB each)
+    void *stage = nullptr;            // pinned host staging of the records (grow-only)
+    size_t stageBytes = 0;
 };
+
+// copy jobs (dst, src, bytes) on up to 16 host threads, in chunks (spreads the first-touch faults of fresh outputs)
+static void parallel_copy(const std::vector<std::tuple<void *, const void *, size_t>> &jobs) {
+    constexpr size_t CH = (size_t)8 << 20;
+    std::vector<std::tuple<char *, const char *, size_t>> parts;
+    size_t total = 0;
+    for (const auto &j : jobs) {
+        char *d = (char *)std::get<0>(j);
+        const char *s = (const char *)std::get<1>(j);
+        for (size_t o = 0; o < std::get<2>(j); o += CH) parts.emplace_back(d + o, s + o, std::min(CH, std::get<2>(j) - o));
+        total += std::get<2>(j);
+    }
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nt = (unsigned)std::min<size_t>(std::min(16u, hw), std::max<size_t>(1, total / CH));
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        for (size_t k; (k = next.fetch_add(1)) < parts.size();) memcpy(std::get<0>(parts[k]), std::get<1>(parts[k]), std::get<2>(parts[k]));
+    };
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; t++) th.emplace_back(work);
+    work();
+    for (auto &t : th) t.join();
+}
 
 extern "C" {
 
@@ -237,6 +265,7 @@ int av1mi_set_shortlist(av1mi_encoder *e, const float *params, float confidence,
 void av1mi_destroy(av1mi_encoder *e) {
     if (!e) return;
     ln_delete(e->ln);
+    if (e->stage) (void)hipHostFree(e->stage);
     if (e->dT) (void)hipFree(e->dT);
     for (auto &v : e->ev) if (v) (void)hipEventDestroy(v);
     delete e;
@@ -445,6 +474,25 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     launch_pack(B, dPO, rB, rBT, rT, rC, st);
     HCK(hipGetLastError());
     (void)hipEventRecord(enc->ev[6], st);
+    // records (and reconstructions) go to the host through the encoder's pinned staging buffer in one DMA per region,
+    // then into each frame's own buffers on several host threads
+    size_t reconBytes = 0;
+    if (wantRecon)
+        for (int i = 0; i < n; i++) reconBytes += ((size_t)imgs[i].aw * imgs[i].ah + 2 * (size_t)(imgs[i].aw >> imgs[i].ss) * (imgs[i].ah >> imgs[i].ss)) * bps;
+    const size_t need = R.total + reconBytes;
+    if (need > enc->stageBytes) {
+        HCK(hipStreamSynchronize(st));
+        if (enc->stage) (void)hipHostFree(enc->stage);
+        enc->stage = nullptr; enc->stageBytes = 0;
+        const size_t cap = need + need / 4;
+        HCK(hipHostMalloc(&enc->stage, cap, hipHostMallocDefault));
+        enc->stageBytes = cap;
+    }
+    char *stg = (char *)enc->stage;
+    HCK(hipMemcpyAsync(stg, R.base, R.total, hipMemcpyDeviceToHost, st));
+    auto hostOf = [&](const void *dptr) { return stg + ((const char *)dptr - (const char *)R.base); };
+    std::vector<std::tuple<void *, const void *, size_t>> jobs;
+    size_t reconOff = R.total;
     for (int i = 0; i < n; i++) {
         const Img &I = imgs[i];
         av1mi_frame_out &o = out[i];
@@ -457,21 +505,25 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
         o.txb = (av1mi_txb *)malloc(std::max<size_t>(o.n_txb, 1) * 16);
         o.coeffs = (int16_t *)malloc(std::max<uint64_t>(o.n_coeffs, 1) * 2);
         o.aw = I.aw; o.ah = I.ah; o.tile_cols = I.tileCols; o.tile_rows = I.tileRows;
-        HCK(hipMemcpyAsync(o.blocks, rB + imgBlk[i], (size_t)o.n_blocks * 16, hipMemcpyDeviceToHost, st));
-        HCK(hipMemcpyAsync(o.block_txb, rBT + imgBlk[i], (size_t)o.n_blocks * 4, hipMemcpyDeviceToHost, st));
-        HCK(hipMemcpyAsync(o.txb, rT + imgTxb[i], (size_t)o.n_txb * 16, hipMemcpyDeviceToHost, st));
-        HCK(hipMemcpyAsync(o.coeffs, rC + imgCoef[i], (size_t)o.n_coeffs * 2, hipMemcpyDeviceToHost, st));
+        jobs.emplace_back(o.blocks, hostOf(rB + imgBlk[i]), (size_t)o.n_blocks * 16);
+        jobs.emplace_back(o.block_txb, hostOf(rBT + imgBlk[i]), (size_t)o.n_blocks * 4);
+        jobs.emplace_back(o.txb, hostOf(rT + imgTxb[i]), (size_t)o.n_txb * 16);
+        jobs.emplace_back(o.coeffs, hostOf(rC + imgCoef[i]), (size_t)o.n_coeffs * 2);
         if (wantRecon) {
             for (int p = 0; p < 3; p++) {
                 const int pw = p ? I.aw >> I.ss : I.aw, ph = p ? I.ah >> I.ss : I.ah;
-                o.recon[p] = malloc((size_t)pw * ph * bps);
+                const size_t b = (size_t)pw * ph * bps;
+                o.recon[p] = malloc(b);
                 o.recon_pitch[p] = pw;
-                HCK(hipMemcpyAsync(o.recon[p], I.rec[p], (size_t)pw * ph * bps, hipMemcpyDeviceToHost, st));
+                HCK(hipMemcpyAsync(stg + reconOff, I.rec[p], b, hipMemcpyDeviceToHost, st));
+                jobs.emplace_back(o.recon[p], stg + reconOff, b);
+                reconOff += b;
             }
         }
     }
     (void)hipEventRecord(enc->ev[7], st);
     HCK(hipStreamSynchronize(st));
+    parallel_copy(jobs);
     for (int i = 0; i < n; i++) out[i].block_txb[out[i].n_blocks] = out[i].n_txb;
     R.release(st);
     A.release(st);
