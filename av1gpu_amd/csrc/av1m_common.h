@@ -570,6 +570,8 @@ struct Batch {
     RecTxb *oTxb;                // [nSb * txbCap]
     int16_t *oCoeffs;            // [nSb * coefCap]
     uint32_t txbCap, coefCap;    // per-SB record slots: SB_TXB_CAP / SB_COEF_CAP, or the 4:4:4 ones
+    float *dpRel;                // [nSb][170] or null: per node, the DP's relative cost increase without its children
+                                 // (0..84) and without its own shapes (85..169), for training labels
     uint32_t *stats;             // [8]: blocks, skip blocks, CfL blocks, range repairs
     const uint32_t *lnMask;      // [nSb][8] learned pruning: expanded nodes (words 0-2), screened luma modes (3), nodes
                                  // keeping their own shapes (4-6); null = exhaustive

@@ -110,6 +110,8 @@ struct av1mi_encoder {
     uint32_t lnChunk = 65536;         // superblocks per network pass (bounds the activation memory, ~100 KB each)
     void *stage = nullptr;            // pinned host staging of the records (grow-only)
     size_t stageBytes = 0;
+    bool dpExport = false;            // keep the DP's per-node relative costs of the last call (training labels)
+    std::vector<float> dpRel;
 };
 
 // copy jobs (dst, src, bytes) on up to 16 host threads, in chunks (spreads the first-touch faults of fresh outputs)
@@ -262,6 +264,17 @@ int av1mi_set_shortlist(av1mi_encoder *e, const float *params, float confidence,
     return 0;
 }
 
+int av1mi_set_dp_export(av1mi_encoder *e, int on) {
+    e->dpExport = on != 0;
+    e->dpRel.clear();
+    return 0;
+}
+size_t av1mi_dp_export(const av1mi_encoder *e, float *out, size_t n) {
+    const size_t k = std::min(n, e->dpRel.size());
+    if (out && k) memcpy(out, e->dpRel.data(), k * 4);
+    return e->dpRel.size();
+}
+
 void av1mi_destroy(av1mi_encoder *e) {
     if (!e) return;
     ln_delete(e->ln);
@@ -378,6 +391,8 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     uint8_t *dLnAct = nullptr;
     const uint32_t lnN = std::min(nSb, enc->lnChunk);
     if (enc->ln) { A.add(dLnMask, (size_t)nSb * 8); A.add(dLnAct, ln_scratch_bytes(lnN)); }
+    float *dDpRel = nullptr;
+    if (enc->dpExport) A.add(dDpRel, (size_t)nSb * 170);
     HCK(A.alloc(st));
     tm.deviceBytes = A.total;
     for (int i = 0; i < n; i++) {
@@ -406,7 +421,7 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     memset(&B, 0, sizeof B);
     B.nimg = n; B.img = dImg; B.T = enc->dT; B.blk = dBlk; B.shStart = dShStart; B.nSb = nSb; B.sbImg = dSbImg;
     B.leafCount = dLeafCount; B.leaves = dLeaves; B.sbJ = dSbJ; B.sbOut = dSbOut; B.oBlocks = dOB; B.oBlockTxb = dOBT;
-    B.oTxb = dOT; B.oCoeffs = dOC; B.stats = dStats; B.txbCap = txbCap; B.coefCap = coefCap;
+    B.oTxb = dOT; B.oCoeffs = dOC; B.stats = dStats; B.txbCap = txbCap; B.coefCap = coefCap; B.dpRel = dDpRel;
     (void)hipEventRecord(enc->ev[1], st);
     if (enc->ln) {   // the pruning network runs first (its time counts as analysis)
         if (ln_masks(*enc->ln, dLnAct, dImg, dSbImg, nSb, enc->lnConf, lnN, dLnMask, st)) { snprintf(err, errlen, "pruning network failed"); A.release(st); return -1; }
@@ -440,6 +455,10 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     HCK(hipMemcpyAsync(so.data(), dSbOut, (size_t)nSb * sizeof(SbOut), hipMemcpyDeviceToHost, st));
     HCK(hipMemcpyAsync(stats, dStats, sizeof stats, hipMemcpyDeviceToHost, st));
     HCK(hipMemcpyAsync(qctl, dQctl, sizeof qctl, hipMemcpyDeviceToHost, st));
+    if (enc->dpExport) {
+        enc->dpRel.resize((size_t)nSb * 170);
+        HCK(hipMemcpyAsync(enc->dpRel.data(), dDpRel, enc->dpRel.size() * 4, hipMemcpyDeviceToHost, st));
+    }
     HCK(hipStreamSynchronize(st));
     if (qctl[1]) {
         snprintf(err, errlen, "closed-loop queue: a superblock dependency wait timed out (schedule aborted)");

@@ -311,11 +311,14 @@ __global__ void __launch_bounds__(64) k_dp(Batch B, SearchParams P) {
                     const int cb = lvBase[Lv + 1], cper = per * 2;
                     for (int k = 0; k < 4; k++) splitJ = splitJ + nJ[cb + (ny * 2 + (k >> 1)) * cper + nx * 2 + (k & 1)];
                 }
-                float best = INF;
+                float best = INF, bestOwn = INF, splitCost = INF;   // all options; own shapes only; SPLIT
                 int bp = PARTITION_SPLIT;
                 auto consider = [&](int p, float J, float bits) {
                     const float t = fmaf(lambda, bits, J);
                     if (t < best) { best = t; bp = p; }
+                    if (p == PARTITION_SPLIT) splitCost = t;
+                    else if (p == PARTITION_NONE || p == PARTITION_HORZ || p == PARTITION_VERT || p == PARTITION_HORZ_4 || p == PARTITION_VERT_4)
+                        bestOwn = fminf(bestOwn, t);
                 };
                 const bool big = n > P.maxPart;
                 if (hasRows && hasCols) {
@@ -360,6 +363,11 @@ __global__ void __launch_bounds__(64) k_dp(Batch B, SearchParams P) {
                 }
                 nJ[ni] = best;
                 nP[ni] = (int8_t)bp;
+                if (B.dpRel) {   // relative increase of the node's cost without its children / without its own shapes
+                    const float den = fmaxf(best, 1e-3f);
+                    B.dpRel[(size_t)sb * 170 + ni] = fminf((bestOwn - best) / den, 1e6f);
+                    B.dpRel[(size_t)sb * 170 + 85 + ni] = fminf((splitCost - best) / den, 1e6f);
+                }
             }
     }
     B.sbJ[sb] = nJ[0];

@@ -46,6 +46,9 @@ def _sigs():
     L.av1mi_ln_gradient.argtypes = [P, C.c_uint32, C.c_uint32, C.POINTER(C.c_float), F]
     L.av1mi_ln_train_step.argtypes = [P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.POINTER(C.c_float)]
     L.av1mi_set_shortlist.argtypes = [P, F, C.c_float, C.c_char_p, C.c_int]
+    L.av1mi_set_dp_export.argtypes = [P, C.c_int]
+    L.av1mi_dp_export.argtypes = [P, F, C.c_size_t]
+    L.av1mi_dp_export.restype = C.c_size_t
     L._ln_sigs = True
     return L
 
@@ -116,6 +119,28 @@ def sb_labels(blocks, W, H):
     # modes: used by some block of the superblock
     lab[sb, MODE0 + blocks["y_mode"].astype(np.int64)] = 1.0
     return lab, msk
+
+
+def dp_export(enc, on=True):
+    """Make the encoder keep the partition DP's per-node relative costs (on) and return those of its last call:
+    [nSb, 170] float32, superblocks in batch order (see av1mi_set_dp_export)."""
+    L = _sigs()
+    if on is not None:
+        L.av1mi_set_dp_export(enc._h, 1 if on else 0)
+    n = int(L.av1mi_dp_export(enc._h, None, 0))
+    out = np.zeros(n, np.float32)
+    if n:
+        L.av1mi_dp_export(enc._h, out.ctypes.data, n)
+    return out.reshape(-1, 170)
+
+
+def rd_labels(rel, lab, tau):
+    """Cost-aware node labels: a node needs its children (own shapes) when the DP's best cost without them is more
+    than tau (relative) above its best; the mode labels of lab stay.  rel: [nSb, 170] of dp_export for one image."""
+    out = lab.copy()
+    out[:, :N_SPLIT] = (rel[:, :N_SPLIT] > tau).astype(np.float32)
+    out[:, OWN0:MODE0] = (rel[:, N_SPLIT:] > tau).astype(np.float32)
+    return out
 
 
 _MI_W_LOG2 = (0, 0, 1, 1, 1, 2, 2, 2, 3, 3, 3, 4, 4, 4, 5, 5, 0, 2, 1, 3, 2, 4)

@@ -120,6 +120,12 @@ void av1mi_frame_out_free(av1mi_frame_out *out);
  * AV1MI_LN_CHUNK superblocks at a time (environment, read here; default 65536, ~100 KB of activations each). */
 int av1mi_set_shortlist(av1mi_encoder *enc, const float *params, float confidence, char *err, int errlen);
 size_t av1mi_ln_param_count(void);
+/* Training data for the pruning network (no counterpart in the reference): with on != 0, each later av1mi_encode keeps
+ * the partition DP's per-node relative cost increases, superblocks in batch order, 170 floats each: (best cost without
+ * the node's children - best) / best for the 85 square nodes, then the same without the node's own shapes.
+ * av1mi_dp_export copies up to n floats of the last call and returns how many there are. */
+int av1mi_set_dp_export(av1mi_encoder *enc, int on);
+size_t av1mi_dp_export(const av1mi_encoder *enc, float *out, size_t n);
 void av1mi_ln_init_params(float *params, uint64_t seed);
 /* Training on the device with this library's own forward (MFMA) and backward kernels. */
 typedef struct av1mi_learner av1mi_learner;

@@ -194,3 +194,41 @@ def test_records_obey_the_restated_decisions():
             assert own[sb, node], (x, y, s)
     enc.close()
     f.close()
+
+
+def test_dp_export_is_consistent_with_the_records():
+    """The partition DP's exported relative costs (training labels) change no record, are never negative, and agree
+    with the choices at each superblock's root: one coded as a single 64x64 leaf chose NONE, so dropping the root's
+    children costs 0; one without any leaf of side 64 chose SPLIT, so dropping the root's own shapes costs 0."""
+    W, H = 328, 264
+    f = A.DeviceFrame(A.synth_image(1300, W, H), W, H, qindex=255)
+    e = A.GpEncoder().init(A.GpConfig(qindex=120))
+    base = e.encode([f])[0][0]
+    rel = Ln.dp_export(e, True)
+    assert rel.size == 0
+    g = e.encode([f])[0][0]
+    rel = Ln.dp_export(e, None)
+    for k in ("blocks", "txb", "coeffs"):
+        assert np.array_equal(getattr(base, k), getattr(g, k)), k
+    sbc = (2 * ((W + 7) >> 3) + 15) >> 4
+    assert rel.shape == ((2 * ((H + 7) >> 3) + 15) // 16 * sbc, 170) and (rel > -1e-6).all()
+    w4 = np.array([1 << v for v in Ln._MI_W_LOG2])
+    h4 = np.array([1 << v for v in Ln._MI_H_LOG2])
+    b = g.blocks
+    sb = (b["mi_row"].astype(int) // 16) * sbc + b["mi_col"].astype(int) // 16
+    side = 4 * np.maximum(w4[b["bsize"]], h4[b["bsize"]])
+    none64 = checked = 0
+    for s in range(len(rel)):
+        r0, c0 = (s // sbc) * 64, (s % sbc) * 64
+        if r0 + 64 > 4 * 2 * ((H + 7) >> 3) or c0 + 64 > 4 * 2 * ((W + 7) >> 3):
+            continue                                     # edge superblocks: forced partitions
+        bs = b["bsize"][sb == s]
+        if len(bs) == 1 and bs[0] == 12:                 # PARTITION_NONE at the root
+            assert rel[s, 0] == 0.0
+            none64 += 1
+        if not (side[sb == s] == 64).any():              # SPLIT at the root
+            assert rel[s, 85] == 0.0
+        checked += 1
+    assert checked > 0
+    e.close()
+    f.close()
