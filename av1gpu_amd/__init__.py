@@ -33,7 +33,8 @@ class _Params(C.Structure):
 
 
 class _Config(C.Structure):
-    _fields_ = [("qindex", C.c_int), ("edgeFilter", C.c_int), ("P", _Params), ("schedule", C.c_int), ("queueBlocks", C.c_int)]
+    _fields_ = [("qindex", C.c_int), ("edgeFilter", C.c_int), ("P", _Params), ("schedule", C.c_int), ("queueBlocks", C.c_int),
+                ("anStreams", C.c_int), ("graph", C.c_int), ("tune", C.c_int)]
 
 
 class _FrameIn(C.Structure):
@@ -52,7 +53,8 @@ class _FrameOut(C.Structure):
 class _Times(C.Structure):
     _fields_ = [(n, C.c_float) for n in ("setup", "analysis", "dp", "recon", "pack", "download", "total")] + \
                [(n, C.c_uint32) for n in ("steps", "sbs", "blocks", "skipBlocks", "cflBlocks", "rangeFixes")] + \
-               [("deviceBytes", C.c_size_t), ("schedule", C.c_uint32), ("queueBlocks", C.c_uint32)]
+               [("deviceBytes", C.c_size_t), ("schedule", C.c_uint32), ("queueBlocks", C.c_uint32), ("anStreams", C.c_uint32),
+                ("graph", C.c_uint32), ("tuned", C.c_uint32)]
 
 
 _lib = None
@@ -138,6 +140,9 @@ class GpConfig:
     P: GpParams = field(default_factory=GpParams)
     schedule: int = 0      # closed loop: 0 persistent queue, 1 one launch per wavefront step (same results)
     queueBlocks: int = 0   # persistent workgroups, 0 = every resident slot
+    anStreams: int = 1     # analysis shape launches spread over 1..4 streams
+    graph: int = 0         # capture the device part per batch shape as a hipGraph and replay it
+    tune: int = 0          # the first 4 calls of a batch shape try anStreams 1/4 x schedule 0/1, then keep the fastest
 
 
 @dataclass
@@ -191,6 +196,9 @@ class GpTimes:
     deviceBytes: int = 0
     schedule: int = 0
     queueBlocks: int = 0
+    anStreams: int = 0
+    graph: int = 0
+    tuned: int = 0
 
 
 def check_partitions(P: GpParams):
@@ -352,7 +360,7 @@ class GpEncoder:
         if self._h:
             lib().av1mi_destroy(self._h)
             self._h = None
-        c = _Config(cfg.qindex, cfg.edgeFilter, cfg.P._c(), cfg.schedule, cfg.queueBlocks)
+        c = _Config(cfg.qindex, cfg.edgeFilter, cfg.P._c(), cfg.schedule, cfg.queueBlocks, cfg.anStreams, cfg.graph, cfg.tune)
         h = C.c_void_p()
         err = C.create_string_buffer(512)
         if lib().av1mi_create(C.byref(c), C.byref(h), err, 512) != 0:

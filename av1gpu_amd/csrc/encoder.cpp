@@ -9,6 +9,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <map>
 #include <atomic>
 #include <thread>
 #include <tuple>
@@ -20,6 +21,8 @@
 namespace av1m {
 void build_tables(DevTables &T);
 void launch_analysis(const Batch &B, const SearchParams &P, const uint32_t *shTotal, int bd, hipStream_t st);
+void launch_analysis_on(const Batch &B, const SearchParams &P, const uint32_t *shTotal, int bd, const hipStream_t *sts, int ns);
+void launch_zero(void *p, size_t bytes, hipStream_t st);
 void launch_dp(const Batch &B, const SearchParams &P, hipStream_t st);
 void launch_recon(const Batch &B, const SearchParams &P, const uint32_t *dSbList, uint32_t n, int bd, hipStream_t st);
 void launch_recon_queue(const Batch &B, const SearchParams &P, const RcQueue &Q, uint32_t blocks, int bd, hipStream_t st);
@@ -84,15 +87,23 @@ struct Arena {
         req.push_back({total, (void **)&p});
         total += (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~(size_t)255;
     }
+    bool owned = false;
     hipError_t alloc(hipStream_t st) {
         hipError_t e = hipMallocAsync(&base, total, st);
         if (e != hipSuccess) return e;
+        owned = true;
         for (auto &r : req) *r.second = (char *)base + r.first;
         return hipSuccess;
     }
+    void bind(void *mem) {   // lay the requests out over memory the caller owns (a captured plan's)
+        base = mem;
+        owned = false;
+        for (auto &r : req) *r.second = (char *)base + r.first;
+    }
     void release(hipStream_t st) {
-        if (base) (void)hipFreeAsync(base, st);
+        if (base && owned) (void)hipFreeAsync(base, st);
         base = nullptr;
+        owned = false;
     }
 };
 
@@ -112,7 +123,27 @@ struct av1mi_encoder {
     size_t stageBytes = 0;
     bool dpExport = false;            // keep the DP's per-node relative costs of the last call (training labels)
     std::vector<float> dpRel;
+    hipStream_t aux[3] = {};          // analysis streams besides the caller's
+    hipStream_t cap = nullptr;        // graph capture stream
+    hipEvent_t evFork = nullptr, evJoin[3] = {};
+    uint64_t gen = 0;                 // bumped when the captured state (network, export) changes
+    struct Plan {                     // captured device part of one batch shape, over its own persistent arena
+        std::vector<uint64_t> key;
+        void *mem = nullptr;
+        hipGraphExec_t exec = nullptr;
+    } plan;
+    struct Tune {                     // autotuner state of one batch shape: candidates tried, their device times
+        int next = 0, best = 0;
+        float ms[4] = {0, 0, 0, 0};
+    };
+    std::map<std::vector<uint64_t>, Tune> tunes;
 };
+
+static void drop_plan(av1mi_encoder *e) {
+    if (e->plan.exec) (void)hipGraphExecDestroy(e->plan.exec);
+    if (e->plan.mem) (void)hipFree(e->plan.mem);
+    e->plan = av1mi_encoder::Plan();
+}
 
 // copy jobs (dst, src, bytes) on up to 16 host threads, in chunks (spreads the first-touch faults of fresh outputs)
 static void parallel_copy(const std::vector<std::tuple<void *, const void *, size_t>> &jobs) {
@@ -234,6 +265,11 @@ int av1mi_create(const av1mi_config *cfg, av1mi_encoder **out, char *err, int er
     if (he != hipSuccess) { snprintf(err, errlen, "HIP: %s (table upload)", hipGetErrorString(he)); delete e; return -1; }
     if (recon_prepare(8) || recon_prepare(10)) { snprintf(err, errlen, "closed-loop kernel needs %zu bytes of LDS", recon_smem_bytes()); delete e; return -1; }
     if (c.schedule < 0 || c.schedule > 1 || c.queueBlocks < 0) { snprintf(err, errlen, "bad schedule %d / queueBlocks %d", c.schedule, c.queueBlocks); delete e; return -1; }
+    if (c.anStreams < 0 || c.anStreams > 4 || c.graph < 0 || c.graph > 1 || c.tune < 0 || c.tune > 1) {
+        snprintf(err, errlen, "bad anStreams %d / graph %d / tune %d", c.anStreams, c.graph, c.tune);
+        delete e;
+        return -1;
+    }
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, e->device) == hipSuccess) e->cus = v;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeWallClockRate, e->device) == hipSuccess && v > 0) e->clockHz = (uint64_t)v * 1000;
@@ -241,6 +277,13 @@ int av1mi_create(const av1mi_config *cfg, av1mi_encoder **out, char *err, int er
     e->slots[1] = recon_queue_slots(10);
     for (auto &v : e->ev)
         if ((he = hipEventCreate(&v)) != hipSuccess) { snprintf(err, errlen, "HIP: %s (event)", hipGetErrorString(he)); av1mi_destroy(e); return -1; }
+    // default-priority non-blocking streams for the analysis branches and the capture; timing-free events joining them
+    for (auto &s : e->aux)
+        if ((he = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess) { snprintf(err, errlen, "HIP: %s (stream)", hipGetErrorString(he)); av1mi_destroy(e); return -1; }
+    if ((he = hipStreamCreateWithFlags(&e->cap, hipStreamNonBlocking)) != hipSuccess) { snprintf(err, errlen, "HIP: %s (stream)", hipGetErrorString(he)); av1mi_destroy(e); return -1; }
+    if ((he = hipEventCreateWithFlags(&e->evFork, hipEventDisableTiming)) != hipSuccess) { snprintf(err, errlen, "HIP: %s (event)", hipGetErrorString(he)); av1mi_destroy(e); return -1; }
+    for (auto &v : e->evJoin)
+        if ((he = hipEventCreateWithFlags(&v, hipEventDisableTiming)) != hipSuccess) { snprintf(err, errlen, "HIP: %s (event)", hipGetErrorString(he)); av1mi_destroy(e); return -1; }
     // keep freed stream-ordered allocations cached between batches
     hipMemPool_t pool;
     if (hipDeviceGetDefaultMemPool(&pool, e->device) == hipSuccess) {
@@ -252,6 +295,8 @@ int av1mi_create(const av1mi_config *cfg, av1mi_encoder **out, char *err, int er
 }
 
 int av1mi_set_shortlist(av1mi_encoder *e, const float *params, float confidence, char *err, int errlen) {
+    drop_plan(e);   // a captured graph holds the old network's pointers
+    e->gen++;
     ln_delete(e->ln);
     e->ln = nullptr;
     if (!params) return 0;
@@ -265,6 +310,8 @@ int av1mi_set_shortlist(av1mi_encoder *e, const float *params, float confidence,
 }
 
 int av1mi_set_dp_export(av1mi_encoder *e, int on) {
+    drop_plan(e);
+    e->gen++;
     e->dpExport = on != 0;
     e->dpRel.clear();
     return 0;
@@ -277,8 +324,13 @@ size_t av1mi_dp_export(const av1mi_encoder *e, float *out, size_t n) {
 
 void av1mi_destroy(av1mi_encoder *e) {
     if (!e) return;
+    drop_plan(e);
     ln_delete(e->ln);
     if (e->stage) (void)hipHostFree(e->stage);
+    for (auto &s : e->aux) if (s) (void)hipStreamDestroy(s);
+    if (e->cap) (void)hipStreamDestroy(e->cap);
+    if (e->evFork) (void)hipEventDestroy(e->evFork);
+    for (auto &v : e->evJoin) if (v) (void)hipEventDestroy(v);
     if (e->dT) (void)hipFree(e->dT);
     for (auto &v : e->ev) if (v) (void)hipEventDestroy(v);
     delete e;
@@ -359,12 +411,17 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
         A.add(mapMem[i], 5 * mi + 2 * (mi + 2 * c1));
         mapTotal += 5 * mi + 2 * (mi + 2 * c1);
     }
-    Img *dImg; BlockRes *dBlk; uint32_t *dShStart, *dSbImg, *dStats; uint16_t *dLeafCount; Leaf *dLeaves; float *dSbJ; SbOut *dSbOut;
+    uint32_t *dStats, *dDone, *dQctl;
+    A.add(dStats, 8); A.add(dDone, nSb); A.add(dQctl, 2);
+    // everything so far starts each call at zero (reconstruction, maps, counters, done flags): one fill kernel; the
+    // padding beyond the coded area is never reconstructed, so the returned planes are fully defined
+    const size_t zeroBytes = A.total;
+    Img *dImg; BlockRes *dBlk; uint32_t *dShStart, *dSbImg; uint16_t *dLeafCount; Leaf *dLeaves; float *dSbJ; SbOut *dSbOut;
     RecBlock *dOB; uint16_t *dOBT; RecTxb *dOT; int16_t *dOC;
     A.add(dImg, n); A.add(dBlk, blkTotal); A.add(dShStart, shStart.size()); A.add(dSbImg, nSb); A.add(dLeafCount, nSb);
     A.add(dLeaves, (size_t)nSb * MAX_LEAVES); A.add(dSbJ, nSb); A.add(dSbOut, nSb); A.add(dOB, (size_t)nSb * MAX_LEAVES);
     const uint32_t txbCap = any444 ? SB_TXB_CAP_444 : SB_TXB_CAP, coefCap = any444 ? SB_COEF_CAP_444 : SB_COEF_CAP;
-    A.add(dOBT, (size_t)nSb * MAX_LEAVES); A.add(dOT, (size_t)nSb * txbCap); A.add(dOC, (size_t)nSb * coefCap); A.add(dStats, 8);
+    A.add(dOBT, (size_t)nSb * MAX_LEAVES); A.add(dOT, (size_t)nSb * txbCap); A.add(dOC, (size_t)nSb * coefCap);
     // wavefront step lists: per tile, step = (c - c0) + 2 (r - r0)
     std::vector<std::vector<uint32_t>> steps;
     for (int i = 0; i < n; i++) {
@@ -385,15 +442,44 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     for (auto &v : steps) { stepOff.push_back((uint32_t)stepFlat.size()); stepFlat.insert(stepFlat.end(), v.begin(), v.end()); }
     stepOff.push_back((uint32_t)stepFlat.size());
     if (stepFlat.size() != nSb) { snprintf(err, errlen, "internal: wavefront order covers %zu of %u superblocks", stepFlat.size(), nSb); return -1; }
-    uint32_t *dSteps, *dDone, *dQctl, *dLnMask = nullptr;
+    uint32_t *dSteps, *dLnMask = nullptr;
     PackOff *dPO;
-    A.add(dSteps, stepFlat.size()); A.add(dPO, nSb); A.add(dDone, nSb); A.add(dQctl, 2);
+    A.add(dSteps, stepFlat.size()); A.add(dPO, nSb);
     uint8_t *dLnAct = nullptr;
     const uint32_t lnN = std::min(nSb, enc->lnChunk);
     if (enc->ln) { A.add(dLnMask, (size_t)nSb * 8); A.add(dLnAct, ln_scratch_bytes(lnN)); }
     float *dDpRel = nullptr;
     if (enc->dpExport) A.add(dDpRel, (size_t)nSb * 170);
-    HCK(A.alloc(st));
+    // ---- schedule knobs: the config's, or the autotuner's for this batch shape ----
+    std::vector<uint64_t> sig = {(uint64_t)n, (uint64_t)bd, (uint64_t)(enc->ln != nullptr), enc->gen, enc->lnChunk};
+    for (int i = 0; i < n; i++)
+        sig.insert(sig.end(), {(uint64_t)imgs[i].W, (uint64_t)imgs[i].H, (uint64_t)imgs[i].aw, (uint64_t)imgs[i].ah, (uint64_t)imgs[i].ss,
+                               (uint64_t)imgs[i].mono});
+    int anS = std::max(1, enc->cfg.anStreams), sched = enc->cfg.schedule, cand = -1;
+    av1mi_encoder::Tune *tu = nullptr;
+    if (enc->cfg.tune) {
+        tu = &enc->tunes[sig];
+        const int k = tu->next < 4 ? (cand = tu->next) : tu->best;
+        anS = (k & 1) ? 4 : 1;
+        sched = k >> 1;
+    }
+    const int slots = enc->slots[bd == 10] > 0 ? enc->slots[bd == 10] : 1;
+    const uint32_t qBlocks = enc->cfg.queueBlocks > 0 ? (uint32_t)enc->cfg.queueBlocks : (uint32_t)std::max(1, enc->cus * slots);
+    // ---- device memory: stream-ordered per call, or the captured plan's (graph mode) ----
+    bool capture = false;
+    if (enc->cfg.graph) {
+        std::vector<uint64_t> key = sig;
+        key.insert(key.end(), {(uint64_t)anS, (uint64_t)sched, (uint64_t)qBlocks, (uint64_t)A.total});
+        if (enc->plan.key != key) {
+            drop_plan(enc);
+            HCK(hipMalloc(&enc->plan.mem, A.total));
+            enc->plan.key = key;
+            capture = true;
+        }
+        A.bind(enc->plan.mem);
+    } else {
+        HCK(A.alloc(st));
+    }
     tm.deviceBytes = A.total;
     for (int i = 0; i < n; i++) {
         Img &I = imgs[i];
@@ -406,46 +492,81 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
         uint8_t *c = m + 5 * mi;
         I.cLvl[0] = c; I.cLvl[1] = c + mi; I.cLvl[2] = c + mi + c1;
         I.cDc[0] = c + mi + 2 * c1; I.cDc[1] = c + 2 * mi + 2 * c1; I.cDc[2] = c + 2 * mi + 3 * c1;
-        HCK(hipMemsetAsync(m, 0, 5 * mi + 2 * (mi + 2 * c1), st));
-        // the padding beyond the coded area is never reconstructed: zero, so the returned planes are fully defined
-        HCK(hipMemsetAsync(recMem[i], 0, ysz + 2 * csz, st));
     }
-    HCK(hipMemsetAsync(dStats, 0, 8 * 4, st));
-    HCK(hipMemsetAsync(dDone, 0, (size_t)nSb * 4, st));
-    HCK(hipMemsetAsync(dQctl, 0, 2 * 4, st));
-    HCK(hipMemcpyAsync(dImg, imgs.data(), n * sizeof(Img), hipMemcpyHostToDevice, st));
-    HCK(hipMemcpyAsync(dShStart, shStart.data(), shStart.size() * 4, hipMemcpyHostToDevice, st));
-    HCK(hipMemcpyAsync(dSbImg, sbImg.data(), (size_t)nSb * 4, hipMemcpyHostToDevice, st));
-    HCK(hipMemcpyAsync(dSteps, stepFlat.data(), stepFlat.size() * 4, hipMemcpyHostToDevice, st));
     Batch B;
     memset(&B, 0, sizeof B);
     B.nimg = n; B.img = dImg; B.T = enc->dT; B.blk = dBlk; B.shStart = dShStart; B.nSb = nSb; B.sbImg = dSbImg;
     B.leafCount = dLeafCount; B.leaves = dLeaves; B.sbJ = dSbJ; B.sbOut = dSbOut; B.oBlocks = dOB; B.oBlockTxb = dOBT;
     B.oTxb = dOT; B.oCoeffs = dOC; B.stats = dStats; B.txbCap = txbCap; B.coefCap = coefCap; B.dpRel = dDpRel;
-    (void)hipEventRecord(enc->ev[1], st);
-    if (enc->ln) {   // the pruning network runs first (its time counts as analysis)
-        if (ln_masks(*enc->ln, dLnAct, dImg, dSbImg, nSb, enc->lnConf, lnN, dLnMask, st)) { snprintf(err, errlen, "pruning network failed"); A.release(st); return -1; }
-        B.lnMask = dLnMask;
-    }
-    launch_analysis(B, P, shTotal, bd, st);
-    HCK(hipGetLastError());
-    (void)hipEventRecord(enc->ev[2], st);
-    launch_dp(B, P, st);
-    HCK(hipGetLastError());
-    (void)hipEventRecord(enc->ev[3], st);
-    const int slots = enc->slots[bd == 10] > 0 ? enc->slots[bd == 10] : 1;
-    const uint32_t qBlocks = enc->cfg.queueBlocks > 0 ? (uint32_t)enc->cfg.queueBlocks : (uint32_t)std::max(1, enc->cus * slots);
-    if (enc->cfg.schedule == 1) {
-        for (size_t s = 0; s + 1 < stepOff.size(); s++) launch_recon(B, P, dSteps + stepOff[s], stepOff[s + 1] - stepOff[s], bd, st);
+    if (enc->ln) B.lnMask = dLnMask;
+    // the device part: zeroing, the pruning network (its time counts as analysis), the analysis shapes over anS
+    // streams (fork / join by events), the DP, the closed loop.  Eager on the caller's stream, or captured once per batch
+    // shape on the capture stream and replayed; no memset nodes, only kernels.
+    auto device_part = [&](hipStream_t s, bool events) -> int {
+        launch_zero(A.base, zeroBytes, s);
+        if (events) (void)hipEventRecord(enc->ev[1], s);
+        if (enc->ln && ln_masks(*enc->ln, dLnAct, dImg, dSbImg, nSb, enc->lnConf, lnN, dLnMask, s)) return -1;
+        if (anS > 1) {
+            hipStream_t sts[4] = {s, enc->aux[0], enc->aux[1], enc->aux[2]};
+            if (hipEventRecord(enc->evFork, s) != hipSuccess) return -1;
+            for (int k = 1; k < anS; k++) if (hipStreamWaitEvent(sts[k], enc->evFork, 0) != hipSuccess) return -1;
+            launch_analysis_on(B, P, shTotal, bd, sts, anS);
+            for (int k = 1; k < anS; k++) {
+                if (hipEventRecord(enc->evJoin[k - 1], sts[k]) != hipSuccess || hipStreamWaitEvent(s, enc->evJoin[k - 1], 0) != hipSuccess)
+                    return -1;
+            }
+        } else {
+            launch_analysis(B, P, shTotal, bd, s);
+        }
+        if (events) (void)hipEventRecord(enc->ev[2], s);
+        launch_dp(B, P, s);
+        if (events) (void)hipEventRecord(enc->ev[3], s);
+        if (sched == 1) {
+            for (size_t k = 0; k + 1 < stepOff.size(); k++) launch_recon(B, P, dSteps + stepOff[k], stepOff[k + 1] - stepOff[k], bd, s);
+        } else {
+            RcQueue Q;
+            Q.order = dSteps; Q.n = nSb; Q.head = dQctl; Q.abort = dQctl + 1; Q.done = dDone;
+            Q.spinTicks = enc->clockHz * 30;   // no legitimate wait comes near 30 s
+            launch_recon_queue(B, P, Q, std::min<uint32_t>(qBlocks, nSb), bd, s);
+        }
+        return hipGetLastError() == hipSuccess ? 0 : -1;
+    };
+    if (enc->cfg.graph) {
+        if (capture) {   // shape data once, then the capture
+            HCK(hipMemcpy(dShStart, shStart.data(), shStart.size() * 4, hipMemcpyHostToDevice));
+            HCK(hipMemcpy(dSbImg, sbImg.data(), (size_t)nSb * 4, hipMemcpyHostToDevice));
+            HCK(hipMemcpy(dSteps, stepFlat.data(), stepFlat.size() * 4, hipMemcpyHostToDevice));
+            HCK(hipStreamBeginCapture(enc->cap, hipStreamCaptureModeThreadLocal));
+            const int r = device_part(enc->cap, false);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(enc->cap, &g);
+            if (r || ec != hipSuccess || !g) {
+                if (g) (void)hipGraphDestroy(g);
+                drop_plan(enc);
+                snprintf(err, errlen, "graph capture of the encode failed");
+                return -1;
+            }
+            const hipError_t ei = hipGraphInstantiate(&enc->plan.exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) { drop_plan(enc); snprintf(err, errlen, "HIP: %s (graph instantiate)", hipGetErrorString(ei)); return -1; }
+        }
+        HCK(hipMemcpyAsync(dImg, imgs.data(), n * sizeof(Img), hipMemcpyHostToDevice, st));   // pointers, qindex of this call
+        (void)hipEventRecord(enc->ev[1], st);
+        HCK(hipGraphLaunch(enc->plan.exec, st));
+        (void)hipEventRecord(enc->ev[2], st);   // the whole device part counts as analysis in graph mode
+        (void)hipEventRecord(enc->ev[3], st);
     } else {
-        RcQueue Q;
-        Q.order = dSteps; Q.n = nSb; Q.head = dQctl; Q.abort = dQctl + 1; Q.done = dDone;
-        Q.spinTicks = enc->clockHz * 30;   // no legitimate wait comes near 30 s
-        launch_recon_queue(B, P, Q, std::min<uint32_t>(qBlocks, nSb), bd, st);
+        HCK(hipMemcpyAsync(dImg, imgs.data(), n * sizeof(Img), hipMemcpyHostToDevice, st));
+        HCK(hipMemcpyAsync(dShStart, shStart.data(), shStart.size() * 4, hipMemcpyHostToDevice, st));
+        HCK(hipMemcpyAsync(dSbImg, sbImg.data(), (size_t)nSb * 4, hipMemcpyHostToDevice, st));
+        HCK(hipMemcpyAsync(dSteps, stepFlat.data(), stepFlat.size() * 4, hipMemcpyHostToDevice, st));
+        if (device_part(st, true)) { snprintf(err, errlen, "HIP: launch failed (%s)", hipGetErrorString(hipGetLastError())); A.release(st); return -1; }
     }
-    HCK(hipGetLastError());
-    tm.schedule = (uint32_t)enc->cfg.schedule;
-    tm.queueBlocks = enc->cfg.schedule == 1 ? 0 : std::min<uint32_t>(qBlocks, nSb);
+    tm.schedule = (uint32_t)sched;
+    tm.queueBlocks = sched == 1 ? 0 : std::min<uint32_t>(qBlocks, nSb);
+    tm.anStreams = (uint32_t)anS;
+    tm.graph = (uint32_t)enc->cfg.graph;
+    tm.tuned = tu && cand < 0 ? 1u : 0u;
     (void)hipEventRecord(enc->ev[4], st);
     tm.steps = (uint32_t)steps.size();
     tm.sbs = nSb;
@@ -550,6 +671,11 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     for (int k = 0; k < 7; k++) (void)hipEventElapsedTime(&ms[k], enc->ev[k], enc->ev[k + 1]);
     tm.setup = ms[0]; tm.analysis = ms[1]; tm.dp = ms[2]; tm.recon = ms[3]; tm.pack = ms[5]; tm.download = ms[6] + ms[4];
     (void)hipEventElapsedTime(&tm.total, enc->ev[0], enc->ev[7]);
+    if (cand >= 0) {   // autotuner: the device part's time of this candidate; after the 4th, keep the fastest
+        tu->ms[cand] = ms[1] + ms[2] + ms[3];
+        if (++tu->next == 4)
+            for (int k = 1; k < 4; k++) if (tu->ms[k] < tu->ms[tu->best]) tu->best = k;
+    }
     tm.blocks = stats[0]; tm.skipBlocks = stats[1]; tm.cflBlocks = stats[2]; tm.rangeFixes = stats[3];
     if (t) *t = tm;
     return 0;

@@ -244,27 +244,33 @@ static void launch_an(const Batch &B, const SearchParams &P, uint32_t n, int bd,
     else hipLaunchKernelGGL((k_analysis<W, H, BS, uint8_t>), dim3(grid), dim3(NG * 32), 0, st, B, P, n);
 }
 
-// all shapes of the batch (shTotal[s]: blocks of shape s over the batch; 0 = not analysed), largest first
+// all shapes of the batch (shTotal[s]: blocks of shape s over the batch; 0 = not analysed), largest first, the i-th
+// shape on sts[i % ns] (the shapes are independent: several streams fill each other's launch tails)
+void launch_analysis_on(const Batch &B, const SearchParams &P, const uint32_t *shTotal, int bd, const hipStream_t *sts, int ns) {
+    int i = 0;
+    auto st = [&]() { return sts[(i++) % ns]; };
+    launch_an<64, 64, BLOCK_64X64>(B, P, shTotal[BLOCK_64X64], bd, st());
+    launch_an<64, 32, BLOCK_64X32>(B, P, shTotal[BLOCK_64X32], bd, st());
+    launch_an<32, 64, BLOCK_32X64>(B, P, shTotal[BLOCK_32X64], bd, st());
+    launch_an<64, 16, BLOCK_64X16>(B, P, shTotal[BLOCK_64X16], bd, st());
+    launch_an<16, 64, BLOCK_16X64>(B, P, shTotal[BLOCK_16X64], bd, st());
+    launch_an<32, 32, BLOCK_32X32>(B, P, shTotal[BLOCK_32X32], bd, st());
+    launch_an<32, 16, BLOCK_32X16>(B, P, shTotal[BLOCK_32X16], bd, st());
+    launch_an<16, 32, BLOCK_16X32>(B, P, shTotal[BLOCK_16X32], bd, st());
+    launch_an<32, 8, BLOCK_32X8>(B, P, shTotal[BLOCK_32X8], bd, st());
+    launch_an<8, 32, BLOCK_8X32>(B, P, shTotal[BLOCK_8X32], bd, st());
+    launch_an<16, 16, BLOCK_16X16>(B, P, shTotal[BLOCK_16X16], bd, st());
+    launch_an<16, 8, BLOCK_16X8>(B, P, shTotal[BLOCK_16X8], bd, st());
+    launch_an<8, 16, BLOCK_8X16>(B, P, shTotal[BLOCK_8X16], bd, st());
+    launch_an<16, 4, BLOCK_16X4>(B, P, shTotal[BLOCK_16X4], bd, st());
+    launch_an<4, 16, BLOCK_4X16>(B, P, shTotal[BLOCK_4X16], bd, st());
+    launch_an<8, 8, BLOCK_8X8>(B, P, shTotal[BLOCK_8X8], bd, st());
+    launch_an<8, 4, BLOCK_8X4>(B, P, shTotal[BLOCK_8X4], bd, st());
+    launch_an<4, 8, BLOCK_4X8>(B, P, shTotal[BLOCK_4X8], bd, st());
+    launch_an<4, 4, BLOCK_4X4>(B, P, shTotal[BLOCK_4X4], bd, st());
+}
 void launch_analysis(const Batch &B, const SearchParams &P, const uint32_t *shTotal, int bd, hipStream_t st) {
-    launch_an<64, 64, BLOCK_64X64>(B, P, shTotal[BLOCK_64X64], bd, st);
-    launch_an<64, 32, BLOCK_64X32>(B, P, shTotal[BLOCK_64X32], bd, st);
-    launch_an<32, 64, BLOCK_32X64>(B, P, shTotal[BLOCK_32X64], bd, st);
-    launch_an<64, 16, BLOCK_64X16>(B, P, shTotal[BLOCK_64X16], bd, st);
-    launch_an<16, 64, BLOCK_16X64>(B, P, shTotal[BLOCK_16X64], bd, st);
-    launch_an<32, 32, BLOCK_32X32>(B, P, shTotal[BLOCK_32X32], bd, st);
-    launch_an<32, 16, BLOCK_32X16>(B, P, shTotal[BLOCK_32X16], bd, st);
-    launch_an<16, 32, BLOCK_16X32>(B, P, shTotal[BLOCK_16X32], bd, st);
-    launch_an<32, 8, BLOCK_32X8>(B, P, shTotal[BLOCK_32X8], bd, st);
-    launch_an<8, 32, BLOCK_8X32>(B, P, shTotal[BLOCK_8X32], bd, st);
-    launch_an<16, 16, BLOCK_16X16>(B, P, shTotal[BLOCK_16X16], bd, st);
-    launch_an<16, 8, BLOCK_16X8>(B, P, shTotal[BLOCK_16X8], bd, st);
-    launch_an<8, 16, BLOCK_8X16>(B, P, shTotal[BLOCK_8X16], bd, st);
-    launch_an<16, 4, BLOCK_16X4>(B, P, shTotal[BLOCK_16X4], bd, st);
-    launch_an<4, 16, BLOCK_4X16>(B, P, shTotal[BLOCK_4X16], bd, st);
-    launch_an<8, 8, BLOCK_8X8>(B, P, shTotal[BLOCK_8X8], bd, st);
-    launch_an<8, 4, BLOCK_8X4>(B, P, shTotal[BLOCK_8X4], bd, st);
-    launch_an<4, 8, BLOCK_4X8>(B, P, shTotal[BLOCK_4X8], bd, st);
-    launch_an<4, 4, BLOCK_4X4>(B, P, shTotal[BLOCK_4X4], bd, st);
+    launch_analysis_on(B, P, shTotal, bd, &st, 1);
 }
 
 // ---------------------------------------------------------------------------------------------------------------

@@ -68,6 +68,7 @@ def main():
     ap.add_argument("--no-bd", action="store_true", help="skip the BD-rate leg")
     ap.add_argument("--schedule", type=int, default=0, help="closed loop: 0 persistent queue, 1 one launch per wavefront step")
     ap.add_argument("--no-444", action="store_true", help="skip the 10-bit 4:4:4 batched+learned leg")
+    ap.add_argument("--no-modes", action="store_true", help="skip the launch-mode leg (streams / hipGraph / autotuner)")
     ap.add_argument("--serve", type=int, default=64, help="requests of the serving leg (0 = skip it)")
     ap.add_argument("--rate", type=float, default=50.0, help="serving leg: Poisson arrival rate, requests per second")
     a = ap.parse_args()
@@ -150,6 +151,8 @@ def main():
     if not a.no_bd:
         line["bd_rate"] = bd_leg(A, learn, params, a.conf, spec, bplanes)
     del bplanes
+    if not a.no_modes:
+        line["launch_modes"] = modes_leg(A, learn, params, a.conf)
     if not a.no_444:
         line["learned_10bit_444"] = leg_444(A, learn, params, a.conf, a.batch, a.schedule)
     if a.serve > 0:
@@ -229,6 +232,36 @@ def bd_leg(A, learn, params, conf, spec, bplanes, n=16, max_mp=2.0, qs=(60, 100,
     print("bd leg done: %.3f %%" % bd, file=sys.stderr, flush=True)
     return {"bd_rate_psnr_y_pct": round(float(bd), 3), "images": [spec[i][3] for i in idx],
             "megapixels": round(sum(W * H for _, W, H in images) / 1e6, 3), "qindex": list(qs), "curves": curves}
+
+
+def modes_leg(A, learn, params, conf, n=16, W=512, H=512, calls=10):
+    """Launch modes on a workload where launches matter: batches of n small images (the same batch shape every call,
+    learned pruning on): eager on one stream, analysis over 4 streams, hipGraph replay, and the autotuner (its 4 trial
+    calls in the warm-up).  Each mode: untimed calls until warm, then `calls` timed calls."""
+    frames = [A.DeviceFrame(A.synth_image(3000 + i, W, H), W, H, qindex=120) for i in range(n)]
+    mp = n * W * H / 1e6
+    res = {"workload": "%d images of %dx%d 8-bit 4:2:0 (seeds 3000..%d), qindex 120, learned pruning, the same batch every "
+                       "call" % (n, W, H, 3000 + n - 1), "megapixels_per_call": mp}
+    for name, kw, warm in (("eager_1_stream", {}, 2), ("eager_4_streams", {"anStreams": 4}, 2),
+                           ("graph_4_streams", {"anStreams": 4, "graph": 1}, 2), ("tuned_graph", {"tune": 1, "graph": 1}, 5)):
+        enc = A.GpEncoder().init(A.GpConfig(qindex=120, **kw))
+        learn.set_shortlist(enc, params, conf)
+        for _ in range(warm):
+            enc.encode(frames, want_recon=False)
+        A.lib().av1mi_device_sync()
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            out, t = enc.encode(frames, want_recon=False)
+        A.lib().av1mi_device_sync()
+        dt = (time.perf_counter() - t0) / calls
+        res[name] = {"ms_per_call": round(dt * 1e3, 3), "value": round(mp / dt, 3), "unit": "MP/s",
+                     "device_ms": round(t.analysis + t.dp + t.recon, 3), "anStreams": t.anStreams, "schedule": t.schedule,
+                     "graph": t.graph, "tuned": t.tuned}
+        enc.close()
+    for f in frames:
+        f.close()
+    print("modes leg done: %s" % {k: v["value"] for k, v in res.items() if isinstance(v, dict)}, file=sys.stderr, flush=True)
+    return res
 
 
 def leg_444(A, learn, params, conf, n, schedule, qs=(60, 100, 140, 180)):

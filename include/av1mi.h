@@ -57,6 +57,14 @@ typedef struct av1mi_config {
      * workgroups of the persistent kernel, 0 = every resident slot of the device.  Results do not depend on either. */
     int schedule;
     int queueBlocks;
+    /* anStreams: the 19 analysis shape launches spread over 1..4 streams (0 = 1).  graph: capture the device part
+     * of an encode (zeroing, pruning network, analysis, DP, closed loop) once per batch shape (image sizes, bit depth,
+     * chroma format) as a hipGraph and replay it; per-image qindex and plane pointers may change between calls.
+     * tune: the first 4 calls of each batch shape try anStreams 1 / 4 x schedule 0 / 1 and keep the fastest.  None of
+     * these changes a record. */
+    int anStreams;
+    int graph;
+    int tune;
 } av1mi_config;
 
 /* GpFrameIn (pipeline.h:149-163): device planes padded (edge replicated) to aw x ah, multiples of 64; chroma
@@ -92,6 +100,7 @@ typedef struct av1mi_times {
     uint32_t steps, sbs, blocks, skipBlocks, cflBlocks, rangeFixes;
     size_t deviceBytes;
     uint32_t schedule, queueBlocks;   /* the closed-loop schedule used (0 queue, 1 steps) and its workgroups */
+    uint32_t anStreams, graph, tuned; /* analysis streams used; 1 = replayed / captured graph; 1 = the tuner's choice */
 } av1mi_times;
 
 typedef struct av1mi_encoder av1mi_encoder;
