@@ -134,12 +134,16 @@ def dp_export(enc, on=True):
     return out.reshape(-1, 170)
 
 
-def rd_labels(rel, lab, tau):
+def rd_labels(rel, lab, tau, tau_root=None):
     """Cost-aware node labels: a node needs its children (own shapes) when the DP's best cost without them is more
-    than tau (relative) above its best; the mode labels of lab stay.  rel: [nSb, 170] of dp_export for one image."""
+    than tau (relative) above its best -- tau_root for the 64x64 node, if given; the mode labels of lab stay.
+    rel: [nSb, 170] of dp_export for one image."""
+    t = np.full(N_SPLIT, tau, np.float32)
+    if tau_root is not None:
+        t[0] = tau_root
     out = lab.copy()
-    out[:, :N_SPLIT] = (rel[:, :N_SPLIT] > tau).astype(np.float32)
-    out[:, OWN0:MODE0] = (rel[:, N_SPLIT:] > tau).astype(np.float32)
+    out[:, :N_SPLIT] = (rel[:, :N_SPLIT] > t).astype(np.float32)
+    out[:, OWN0:MODE0] = (rel[:, N_SPLIT:] > t).astype(np.float32)
     return out
 
 

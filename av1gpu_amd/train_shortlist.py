@@ -36,7 +36,7 @@ def image_spec(n, seed, mp_lo=0.2, mp_hi=4.0, qs=(80, 120, 160)):
     return out
 
 
-def labelled_data(n_sbs, seed=20000, log=print, rd_tau=None):
+def labelled_data(n_sbs, seed=20000, log=print, rd_tau=None, rd_tau_root=None):
     """Tiles, qindex, labels and masks of about n_sbs superblocks of exhaustively encoded images.  rd_tau: node labels
     from the DP's relative cost increases (learn.rd_labels) instead of the leaves' use."""
     enc = A.GpEncoder().init(A.GpConfig(qindex=120))
@@ -59,7 +59,7 @@ def labelled_data(n_sbs, seed=20000, log=print, rd_tau=None):
         for p, (W, H, q, _), o in zip(planes, spec, out):
             lab, msk = Ln.sb_labels(o.blocks, W, H)
             if rel is not None:
-                lab = Ln.rd_labels(rel[s0:s0 + len(lab)], lab, rd_tau)
+                lab = Ln.rd_labels(rel[s0:s0 + len(lab)], lab, rd_tau, rd_tau_root)
                 s0 += len(lab)
             t = Ln.sb_tiles(p[0], W, H)
             tiles.append(t); qs.append(np.full(len(t), q, np.uint8)); labs.append(lab); msks.append(msk)
@@ -126,14 +126,15 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--train-budget", type=float, default=None, help="seconds of training at most")
     ap.add_argument("--rd-tau", type=float, default=None, help="cost-aware node labels at this relative threshold")
+    ap.add_argument("--rd-tau-root", type=float, default=None, help="the threshold of the 64x64 node (default: --rd-tau)")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "shortlist_params.npy"))
     a = ap.parse_args()
     log = lambda m: print(m, flush=True)
-    data = labelled_data(a.sbs, log=log, rd_tau=a.rd_tau)
+    data = labelled_data(a.sbs, log=log, rd_tau=a.rd_tau, rd_tau_root=a.rd_tau_root)
     params, losses = train(Ln.init_params(1), data, a.steps, a.batch, a.lr, log=log, budget=a.train_budget)
     np.save(a.out, params)
     ev = evaluate(params, log=log)
-    summary = dict(superblocks=int(len(data[0])), steps=len(losses), batch=a.batch, lr=a.lr, rd_tau=a.rd_tau,
+    summary = dict(superblocks=int(len(data[0])), steps=len(losses), batch=a.batch, lr=a.lr, rd_tau=a.rd_tau, rd_tau_root=a.rd_tau_root,
                    loss_first=float(np.mean(losses[:50])), loss_last=float(np.mean(losses[-50:])), eval=ev)
     with open(os.path.splitext(a.out)[0] + ".json", "w") as f:
         json.dump(summary, f, indent=1)
