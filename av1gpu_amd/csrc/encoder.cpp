@@ -119,8 +119,10 @@ struct av1mi_encoder {
     LnNet *ln = nullptr;              // learned pruning network (null: exhaustive search)
     float lnConf = 0.8f;
     uint32_t lnChunk = 65536;         // superblocks per network pass (bounds the activation memory, ~100 KB each)
-    void *stage = nullptr;            // pinned host staging of the records (grow-only)
+    void *stage = nullptr;            // pinned host staging of the records (grow-only, by doubling)
     size_t stageBytes = 0;
+    std::vector<void *> oldStages;    // outgrown staging buffers, freed at destroy: hipHostFree would wait for the whole
+                                      // device, i.e. for the other encoders' work in a multi-stream server
     bool dpExport = false;            // keep the DP's per-node relative costs of the last call (training labels)
     std::vector<float> dpRel;
     hipStream_t aux[3] = {};          // analysis streams besides the caller's
@@ -327,6 +329,7 @@ void av1mi_destroy(av1mi_encoder *e) {
     drop_plan(e);
     ln_delete(e->ln);
     if (e->stage) (void)hipHostFree(e->stage);
+    for (void *p : e->oldStages) (void)hipHostFree(p);
     for (auto &s : e->aux) if (s) (void)hipStreamDestroy(s);
     if (e->cap) (void)hipStreamDestroy(e->cap);
     if (e->evFork) (void)hipEventDestroy(e->evFork);
@@ -622,9 +625,10 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     const size_t need = R.total + reconBytes;
     if (need > enc->stageBytes) {
         HCK(hipStreamSynchronize(st));
-        if (enc->stage) (void)hipHostFree(enc->stage);
-        enc->stage = nullptr; enc->stageBytes = 0;
-        const size_t cap = need + need / 4;
+        if (enc->stage) enc->oldStages.push_back(enc->stage);
+        enc->stage = nullptr;
+        const size_t cap = std::max(need + need / 4, 2 * enc->stageBytes);
+        enc->stageBytes = 0;
         HCK(hipHostMalloc(&enc->stage, cap, hipHostMallocDefault));
         enc->stageBytes = cap;
     }

@@ -305,7 +305,11 @@ __global__ void __launch_bounds__(64) k_dp(Batch B, SearchParams P) {
             for (int nx = 0; nx < per; nx++) {
                 const int ni = lvBase[Lv] + ny * per + nx;
                 const int x = sbx + nx * n, y = sby + ny * n;
-                if (x >= FW || y >= FH) { nJ[ni] = 0.f; nP[ni] = -1; continue; }
+                if (x >= FW || y >= FH) {   // outside the frame: no partition (exported as 0, masked in training)
+                    nJ[ni] = 0.f; nP[ni] = -1;
+                    if (B.dpRel) { B.dpRel[(size_t)sb * 170 + ni] = 0.f; B.dpRel[(size_t)sb * 170 + 85 + ni] = 0.f; }
+                    continue;
+                }
                 const bool hasRows = y + half < FH, hasCols = x + half < FW;
                 float splitJ = 0.f;
                 if (n == 8) {
