@@ -12,9 +12,9 @@ The encoder then analyses a node's children / own shapes unless the network is c
 modes where it is confident (the product of 1 - p over the 5 others at least conf) that none of the others is used.
 
 Training data are the exhaustive search's own decisions on seeded synthetic images; training runs on the GPU with the
-library's forward (MFMA) and backward kernels.  ``ref_conv`` / ``ref_fc`` / ``ref_grads`` restate the same network in
-float64 NumPy with the BF16 rounding at the same points; the tests check the kernels' activations and gradients with
-them.
+library's forward (MFMA) and backward kernels.  tests/ln_ref.py restates the same network in float64 NumPy with the
+BF16 rounding at the same points, and the decisions; the tests check the kernels' activations, gradients and the
+encoder's decisions with it.
 """
 import ctypes as C
 import math
@@ -215,37 +215,6 @@ class Learner:
             pass
 
 
-def decide(logits, W, H, conf):
-    """The decisions the encoder takes from the logits of an image's superblocks (raster order): expand [nSb, 85] bool
-    (a node's children are analysed), own [nSb, 85] bool (its own shapes are) and the luma-mode mask [nSb] (bit m:
-    mode m screened)."""
-    z = np.asarray(logits, np.float64)
-    MiCols, MiRows = 2 * ((W + 7) >> 3), 2 * ((H + 7) >> 3)
-    FW, FH = 4 * MiCols, 4 * MiRows
-    sbc = (MiCols + 15) >> 4
-    idx = np.arange(len(z))
-    sx, sy = (idx % sbc) * 64, (idx // sbc) * 64
-    p = 1.0 / (1.0 + np.exp(-z))
-    expand = np.zeros((len(z), N_SPLIT), bool)
-    own = np.zeros((len(z), N_SPLIT), bool)
-    for lv in range(4):
-        n, per = 64 >> lv, 1 << lv
-        for ny in range(per):
-            for nx in range(per):
-                nd = LEVEL_BASE[lv] + ny * per + nx
-                edge = (sx + nx * n + n > FW) | (sy + ny * n + n > FH)
-                expand[:, nd] = edge | (p[:, nd] > 1.0 - conf)
-                own[:, nd] = edge | (p[:, OWN0 + nd] > 1.0 - conf) | ~expand[:, nd]
-    pm = p[:, MODE0:]
-    order = np.argsort(-pm, axis=1, kind="stable")      # ties: the lower mode first
-    top, rest = order[:, :8], order[:, 8:]
-    none = np.prod(1.0 - np.take_along_axis(pm, rest, 1), axis=1)
-    mask = np.full(len(z), 0x1FFF, np.int64)
-    bits = np.sum(1 << top, axis=1)
-    mask[none >= conf] = bits[none >= conf]
-    return expand, own, mask
-
-
 PARAMS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "shortlist_params.npy")
 
 
@@ -264,83 +233,6 @@ def set_shortlist(enc, params, confidence=0.8):
     p = None if params is None else np.ascontiguousarray(params, np.float32)
     if L.av1mi_set_shortlist(enc._h, None if p is None else p.ctypes.data, confidence, err, 256) != 0:
         raise RuntimeError("set_shortlist: " + err.value.decode())
-
-
-# ---------------------------------------------------------------- float64 restatement (checks)
-def bf16(x):
-    """Round float32 values to BF16 (nearest even), returned as float32."""
-    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
-    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
-    return r.astype(np.uint32).view(np.float32)
-
-
-def input_tensor(tiles, q):
-    t = np.asarray(tiles, np.float32).reshape(-1, 64, 64)
-    x0 = np.empty((len(t), 64, 64, 2), np.float32)
-    x0[..., 0] = bf16(t * np.float32(1.0 / 255.0) - np.float32(0.5))
-    x0[..., 1] = bf16(np.asarray(q, np.float32) / np.float32(255.0))[:, None, None]
-    return x0
-
-
-def _im2col(x, cin):
-    """x [n, H, H, cin] -> [n, H/2 * H/2, 9 cin] for the 3x3 stride-2 pad-1 convolution (tap-major, then channel)."""
-    n, H = x.shape[0], x.shape[1]
-    Ho = H // 2
-    xp = np.zeros((n, H + 2, H + 2, cin), x.dtype)
-    xp[:, 1:H + 1, 1:H + 1] = x
-    cols = [xp[:, ky:ky + 2 * Ho:2, kx:kx + 2 * Ho:2, :] for ky in range(3) for kx in range(3)]
-    return np.stack(cols, axis=3).reshape(n, Ho * Ho, 9 * cin)
-
-
-def ref_conv(x, w, b, l):
-    """One convolution + ReLU in float64 on BF16 inputs and weights; output rounded to BF16 as the kernel stores it."""
-    K = 9 * CIN[l]
-    col = _im2col(x.astype(np.float64), CIN[l])
-    wt = bf16(w[:, :K]).astype(np.float64)
-    z = col @ wt.T + b.astype(np.float64)
-    H = x.shape[1] // 2
-    return bf16(np.maximum(z, 0.0).astype(np.float32)).reshape(x.shape[0], H, H, 32)
-
-
-def ref_fc(a4, w5, b5):
-    return a4.reshape(len(a4), -1).astype(np.float64) @ w5.astype(np.float64).T + b5
-
-
-def ref_grads(params, acts, label, lmask):
-    """Float64 gradient of the mean masked BCE, backpropagated through the given (kernel) activations
-    acts = [x0, a1, a2, a3, a4] (NHWC float32 holding BF16 values)."""
-    convs, w5, b5 = unpack(params)
-    n = len(label)
-    z = ref_fc(acts[4], w5, b5)
-    s = 1.0 / (1.0 + np.exp(-z))
-    dlog = lmask * (s - label) / n
-    g = np.zeros(param_count(), np.float64)
-    gconv, gw5, gb5 = unpack(g)
-    a4 = acts[4].reshape(n, -1).astype(np.float64)
-    gw5[:] = dlog.T @ a4
-    gb5[:] = dlog.sum(0)
-    d = (dlog @ w5.astype(np.float64)) * (a4 > 0)
-    d = d.reshape(n, 4, 4, 32)
-    for l in (3, 2, 1, 0):
-        x = acts[l].astype(np.float64)
-        K = 9 * CIN[l]
-        col = _im2col(x, CIN[l])
-        dz = d.reshape(n, -1, 32)
-        gconv[l][0][:, :K] = np.einsum("npc,npk->ck", dz, col)
-        gconv[l][1][:] = dz.sum((0, 1))
-        if l > 0:
-            wk = convs[l][0][:, :K].astype(np.float64)
-            dcol = dz @ wk                                   # [n, P, 9 cin]
-            H = x.shape[1]
-            Ho = H // 2
-            dx = np.zeros((n, H + 2, H + 2, CIN[l]))
-            dcol = dcol.reshape(n, Ho, Ho, 9, CIN[l])
-            for t in range(9):
-                ky, kx = t // 3, t % 3
-                dx[:, ky:ky + 2 * Ho:2, kx:kx + 2 * Ho:2, :] += dcol[:, :, :, t, :]
-            d = dx[:, 1:H + 1, 1:H + 1] * (x > 0)
-    loss = float(np.sum(lmask * (np.maximum(z, 0) - z * label + np.log1p(np.exp(-np.abs(z))))) / n)
-    return loss, g
 
 
 # ---------------------------------------------------------------- rate / distortion

@@ -1,4 +1,4 @@
-"""The pruning network's kernels against the float64 NumPy restatement (av1gpu_amd/learn.py), layer by layer: the MFMA
+"""The pruning network's kernels against the float64 NumPy restatement (tests/ln_ref.py), layer by layer: the MFMA
 convolutions and the FC head (forward), the backward kernels (gradients), Adam training lowering the loss, and the
 encoder integration (a network that expands every node and keeps every mode gives the exhaustive records bit for bit;
 one that prunes gives a record the reference's backend decodes to the GPU reconstruction)."""
@@ -7,6 +7,8 @@ import pytest
 
 import av1gpu_amd as A
 from av1gpu_amd import learn as Ln
+
+import ln_ref as R
 
 pytestmark = pytest.mark.gpu
 
@@ -29,18 +31,18 @@ def test_forward_matches_reference():
     logits = L.forward(tiles, q)
     n = len(tiles)
     acts = [L.activations(k, n) for k in range(5)]
-    x0 = Ln.input_tensor(tiles, q)
+    x0 = R.input_tensor(tiles, q)
     assert np.array_equal(acts[0].reshape(x0.shape), x0)
     convs, w5, b5 = Ln.unpack(p)
     shapes = [(n, 64, 64, 2), (n, 32, 32, 32), (n, 16, 16, 32), (n, 8, 8, 32), (n, 4, 4, 32)]
     for l in range(4):
         inp = acts[l].reshape(shapes[l])
-        ref = Ln.ref_conv(inp, convs[l][0], convs[l][1], l)
+        ref = R.ref_conv(inp, convs[l][0], convs[l][1], l)
         got = acts[l + 1].reshape(shapes[l + 1])
         # FP32 accumulation in another order than float64: outputs agree to BF16 rounding (one step at most)
         assert np.allclose(got, ref, rtol=2 ** -7, atol=1e-4), "layer %d" % (l + 1)
         assert np.mean(got != ref) < 0.02
-    zr = Ln.ref_fc(acts[4].reshape(n, -1), w5, b5)
+    zr = R.ref_fc(acts[4].reshape(n, -1), w5, b5)
     assert np.allclose(logits, zr, rtol=1e-4, atol=1e-4)
     L.close()
 
@@ -57,7 +59,7 @@ def test_gradient_matches_reference():
     acts = [L.activations(k, len(tiles)) for k in range(5)]
     shapes = [(64, 64, 2), (32, 32, 32), (16, 16, 32), (8, 8, 32), (4, 4, 32)]
     acts = [a.reshape((len(tiles),) + s) for a, s in zip(acts, shapes)]
-    lr, gr = Ln.ref_grads(p, acts, lab, msk)
+    lr, gr = R.ref_grads(p, acts, lab, msk)
     assert abs(loss - lr) < 1e-4 * max(1.0, abs(lr))
     gc, gw5, gb5 = Ln.unpack(g.astype(np.float64))
     rc, rw5, rb5 = Ln.unpack(gr)
@@ -156,7 +158,7 @@ def test_network_chunks_and_batching_do_not_change_records(monkeypatch):
 
 
 def test_records_obey_the_restated_decisions():
-    """Every block of a pruned record lies in a node the decisions (learn.decide over the network's logits) expand, is
+    """Every block of a pruned record lies in a node the decisions (ln_ref.decide over the network's logits) expand, is
     one of the own shapes of a node that keeps them, and uses a luma mode of its superblock's mask; a confidence where
     the masks are mixed is used (some nodes pruned, some superblocks shortlisted)."""
     p = Ln.init_params(22)
@@ -174,7 +176,7 @@ def test_records_obey_the_restated_decisions():
     conf = 0.8
     Ln.set_shortlist(enc, p, conf)
     g = enc.encode([f])[0][0]
-    expand, own, mmask = Ln.decide(logits, W, H, conf)
+    expand, own, mmask = R.decide(logits, W, H, conf)
     assert 0 < expand[:, 1:].mean() < 1 and 0 < own.mean() < 1 and (mmask != 0x1FFF).any()
     sbc = (2 * ((W + 7) >> 3) + 15) >> 4
     w4 = np.array([1 << v for v in Ln._MI_W_LOG2])
