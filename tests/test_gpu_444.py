@@ -1,4 +1,4 @@
-"""4:4:4 input (BASELINE config 4's layout: chroma at full resolution, 8 and 10 bits) on the GPU path against the oracle,
+"""4:4:4 input (chroma at full resolution, 8 and 10 bits) on the GPU path against the oracle,
 bit for bit, including blocks with a 64-sample side whose chroma holds 2 or 4 tx blocks (the high-qindex cases), frame
 edges, a batch of mixed sizes, the learned pruning, and the reference's backend / dav1d decoding the records."""
 import numpy as np
@@ -71,7 +71,7 @@ def test_gpu_mixed_chroma_formats_are_refused(enc):
 
 
 def test_gpu_444_10bit_learned_record_decodes(ref):
-    """The batched+learned path on 10-bit 4:4:4 (BASELINE config 4): the pruned record decodes to the GPU reconstruction."""
+    """The batched+learned path on 10-bit 4:4:4: the pruned record decodes to the GPU reconstruction."""
     W, H = 328, 264
     planes = A.synth_image(41, W, H, ssx=0, bd=10)
     e = A.GpEncoder().init(A.GpConfig(qindex=120))

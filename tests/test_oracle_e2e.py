@@ -36,7 +36,7 @@ def test_oracle_10bit_record_decodes(oracle, ref):
 
 @pytest.mark.parametrize("W,H,seed,q,bd", [(200, 136, 1, 120, 8), (256, 256, 11, 255, 8), (200, 264, 13, 240, 10), (130, 18, 4, 200, 8)])
 def test_oracle_444_record_decodes(oracle, ref, W, H, seed, q, bd):
-    """4:4:4 (BASELINE config 4's layout): chroma at full resolution, blocks with a 64-sample side carrying 2 or 4
+    """4:4:4: chroma at full resolution, blocks with a 64-sample side carrying 2 or 4
     chroma tx blocks (the high-qindex cases hold such blocks); the reference's backend reconstructs the record to the
     oracle's reconstruction and dav1d decodes the reference writer's AVIF of it to the same."""
     planes = A.synth_image(seed, W, H, ssx=0, bd=bd)
