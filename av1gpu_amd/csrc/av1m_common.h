@@ -519,6 +519,7 @@ struct DevTables {
     float lambda[2][256];        // [bd == 10][qindex], lambdaMul 1
     float lamS[2][256];          // sqrt(lambda): the screening weight with satdLambda 1
     CostTab cost[4];
+    uint16_t iscan[DSCAN_TOTAL]; // inverse default scans: iscan[dscan_off(t) + position] = scan index
 };
 
 // tunables of the search (the reference's GpParams, gp.h:29-68, for the knobs this core implements)
@@ -570,11 +571,11 @@ struct Batch {
     RecTxb *oTxb;                // [nSb * txbCap]
     int16_t *oCoeffs;            // [nSb * coefCap]
     uint32_t txbCap, coefCap;    // per-SB record slots: SB_TXB_CAP / SB_COEF_CAP, or the 4:4:4 ones
-    float *dpRel;                // [nSb][170] or null: per node, the DP's relative cost increase without its children
-                                 // (0..84) and without its own shapes (85..169), for training labels
+    float *dpRel;                // [nSb][255] or null: per node, the DP's relative cost increase without its children
+                                 // (0..84), without its own shapes (85..169), without its 4-way shapes (170..254)
     uint32_t *stats;             // [8]: blocks, skip blocks, CfL blocks, range repairs
-    const uint32_t *lnMask;      // [nSb][8] learned pruning: expanded nodes (words 0-2), screened luma modes (3), nodes
-                                 // keeping their own shapes (4-6); null = exhaustive
+    const uint32_t *lnMask;      // [nSb][12] learned pruning: expanded nodes (words 0-2), screened luma modes (3), nodes
+                                 // keeping their own shapes (4-6), their 4-way shapes (8-10); null = exhaustive
 };
 
 // the device work queue of the persistent closed-loop schedule

@@ -185,15 +185,11 @@ __device__ float coeff_bits(int c, int eob, QP q, SP dscan, OP cbOff, int pt, in
     return b;
 }
 
-// eob and rate of the levels q of a tx block of size TSZ, class CLS, by the group (all lanes return both)
+// eob and rate of the levels q of a tx block of size TSZ, class CLS, by the group (all lanes return both); last: the
+// lane's share of the eob (1 + the largest scan index of a nonzero level among the positions the lane quantized)
 template <int TSZ, int CLS, typename QP, typename SP, typename OP, typename E>
-__device__ void grp_rate(QP q, SP dscan, OP cbOff, const E &e, int &eobOut, float &bitsOut) {
-    constexpr int adj = av1t::Adjusted_Tx_Size[TSZ];
-    constexpr int TW = 1 << av1t::Tx_Width_Log2[adj], TH = av1t::Tx_Height[adj], N = TW * TH;
+__device__ void grp_rate(QP q, SP dscan, OP cbOff, const E &e, int last, int &eobOut, float &bitsOut) {
     const int lane = glane();
-    int last = 0;
-    for (int c = lane; c < N; c += GW)
-        if (q[scan_pos<TW, TH, CLS>(c, dscan)]) last = c + 1;
     const int eob = gmaxi(last);
     float bits = lane == 0 ? base_bits<TSZ>(e, eob) : 0.f;
     for (int c = lane; c < eob; c += GW) bits += coeff_bits<TSZ, CLS>(c, eob, q, dscan, cbOff, e.plane > 0, e.dcCtx, *e.C);
@@ -206,6 +202,8 @@ template <int TSZ>
 __device__ __forceinline__ const uint16_t *dscan_ptr(const DevTables &) { return scan_of(TSZ, DCT_DCT); }
 template <int TSZ>
 __device__ __forceinline__ const uint8_t *cboff_ptr(const DevTables &) { return &av1t::Coeff_Base_Ctx_Offset[TSZ][0][0]; }
+template <int TSZ>
+__device__ __forceinline__ const uint16_t *iscan_ptr(const DevTables &T) { return T.iscan + dscan_off(av1t::Adjusted_Tx_Size[TSZ]); }
 
 // Evaluation scratch of one group in LDS, for tx blocks up to W x H
 template <int W, int H>
@@ -293,9 +291,15 @@ __device__ EvalOut grp_qrate(FP tmp, int ts, QP q, const E &e, const TT &T) {
     constexpr int LTW = TW == 4 ? 2 : TW == 8 ? 3 : TW == 16 ? 4 : 5;
     constexpr int N = TW * TH;
     const int lane = glane();
-    // ---- quantization + distortion (virtual lane = physical lane) ----
+    constexpr int TSZ = tx_of(W, H);
+    const int cls = tx_class(e.txType);
+    // the scan of the rate: the default (2D) scan, rows (class VERT: raster order) or columns (class HORIZ)
+    const int sk = (W > 16 || H > 16 || cls == TX_CLASS_2D) ? 0 : cls == TX_CLASS_HORIZ ? 1 : 2;
+    const auto iscan = iscan_ptr<TSZ>(T);
+    // ---- quantization + distortion (virtual lane = physical lane), and the eob ----
     const int den = dq_denom(e.txSz), dsh = den == 4 ? 2 : den == 2 ? 1 : 0;
     float dist = 0.f, kept = 0.f;
+    int last = 0;
     for (int idx = lane; idx < N; idx += GW) {
         const float v = tmp[(idx >> LTW) * ts + (idx & (TW - 1))];
         const int step = idx == 0 ? e.dcQ : e.acQ;
@@ -304,6 +308,10 @@ __device__ EvalOut grp_qrate(FP tmp, int ts, QP q, const E &e, const TT &T) {
         l = min(l, 16383);
         const int qv = v < 0 ? -l : l;
         q[idx] = (int16_t)qv;
+        if (l) {
+            const int c = sk == 0 ? (int)iscan[idx] : sk == 1 ? (idx & (TW - 1)) * TH + (idx >> LTW) : idx;
+            last = max(last, c + 1);
+        }
         const float dq = l ? (float)dequant(qv, step, dsh, e.bd) : 0.f;
         const float d = v - dq;
         const float sq = d * d;
@@ -312,15 +320,13 @@ __device__ EvalOut grp_qrate(FP tmp, int ts, QP q, const E &e, const TT &T) {
     }
     const float distS = gsum(dist);
     gsync();
-    constexpr int TSZ = tx_of(W, H);
     const auto dscan = dscan_ptr<TSZ>(T);
     const auto cbOff = cboff_ptr<TSZ>(T);
-    const int cls = tx_class(e.txType);
     int eob;
     EvalOut o;
-    if (W > 16 || H > 16 || cls == TX_CLASS_2D) grp_rate<TSZ, TX_CLASS_2D>(q, dscan, cbOff, e, eob, o.R);
-    else if (cls == TX_CLASS_HORIZ) grp_rate<TSZ, TX_CLASS_HORIZ>(q, dscan, cbOff, e, eob, o.R);
-    else grp_rate<TSZ, TX_CLASS_VERT>(q, dscan, cbOff, e, eob, o.R);
+    if (sk == 0) grp_rate<TSZ, TX_CLASS_2D>(q, dscan, cbOff, e, last, eob, o.R);
+    else if (sk == 1) grp_rate<TSZ, TX_CLASS_HORIZ>(q, dscan, cbOff, e, last, eob, o.R);
+    else grp_rate<TSZ, TX_CLASS_VERT>(q, dscan, cbOff, e, last, eob, o.R);
     const float g = T.txg[e.txSz][e.txType];
     o.D = g * distS;
     if (W == 64 || H == 64) {

@@ -450,9 +450,9 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     A.add(dSteps, stepFlat.size()); A.add(dPO, nSb);
     uint8_t *dLnAct = nullptr;
     const uint32_t lnN = std::min(nSb, enc->lnChunk);
-    if (enc->ln) { A.add(dLnMask, (size_t)nSb * 8); A.add(dLnAct, ln_scratch_bytes(lnN)); }
+    if (enc->ln) { A.add(dLnMask, (size_t)nSb * 12); A.add(dLnAct, ln_scratch_bytes(lnN)); }
     float *dDpRel = nullptr;
-    if (enc->dpExport) A.add(dDpRel, (size_t)nSb * 170);
+    if (enc->dpExport) A.add(dDpRel, (size_t)nSb * 255);
     // ---- schedule knobs: the config's, or the autotuner's for this batch shape ----
     std::vector<uint64_t> sig = {(uint64_t)n, (uint64_t)bd, (uint64_t)(enc->ln != nullptr), enc->gen, enc->lnChunk};
     for (int i = 0; i < n; i++)
@@ -580,7 +580,7 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     HCK(hipMemcpyAsync(stats, dStats, sizeof stats, hipMemcpyDeviceToHost, st));
     HCK(hipMemcpyAsync(qctl, dQctl, sizeof qctl, hipMemcpyDeviceToHost, st));
     if (enc->dpExport) {
-        enc->dpRel.resize((size_t)nSb * 170);
+        enc->dpRel.resize((size_t)nSb * 255);
         HCK(hipMemcpyAsync(enc->dpRel.data(), dDpRel, enc->dpRel.size() * 4, hipMemcpyDeviceToHost, st));
     }
     HCK(hipStreamSynchronize(st));
@@ -733,7 +733,7 @@ size_t av1mi_frame_bytes(int W, int H, int bitDepth, int yuv444) {
                          nSb * (4 + 2 + MAX_LEAVES * sizeof(Leaf) + 4 + sizeof(SbOut) + MAX_LEAVES * (sizeof(RecBlock) + 2) +
                                 txbCap * sizeof(RecTxb) + coefCap * 2 + 4 + sizeof(PackOff) + 4);
     const size_t records = nSb * (MAX_LEAVES * (sizeof(RecBlock) + 4) + txbCap * sizeof(RecTxb) + coefCap * 2);
-    const size_t network = nSb * 32 + ln_scratch_bytes((uint32_t)std::min<size_t>(nSb, 65536));   // masks, activations
+    const size_t network = nSb * 48 + ln_scratch_bytes((uint32_t)std::min<size_t>(nSb, 65536));   // masks, activations
     return planes + arena + records + network + 64 * 1024;
 }
 int av1mi_dev_free(void *ptr) { return hipFree(ptr) == hipSuccess ? 0 : -1; }

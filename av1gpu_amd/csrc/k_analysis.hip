@@ -77,12 +77,13 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
             if (lane == 0) { out.J = INF; out.ncand = 0; }
             continue;
         }
-        // learned pruning: a block belongs to the node of its larger side (one of that node's own shapes); it is
-        // analysed when that node's parent was expanded (the 64x64 node has no parent) and the node keeps its own
-        // shapes (4x4 blocks: the 8x8 node's children, no own bit); the screened luma modes are the superblock's shortlist
+        // learned pruning: a block belongs to the node of its larger side (one of that node's own shapes, or of its
+        // 4-way shapes when the sides are 4:1); it is analysed when that node's parent was expanded (the 64x64 node has
+        // no parent) and the node keeps that kind of shape (4x4 blocks: the 8x8 node's children only); the screened
+        // luma modes are the superblock's shortlist
         uint32_t modeMask = 0x1FFFu;
         if (B.lnMask) {
-            const uint32_t *mk = B.lnMask + (size_t)(I.sbBase + (uint32_t)(y >> 6) * I.sbCols + (uint32_t)(x >> 6)) * 8;
+            const uint32_t *mk = B.lnMask + (size_t)(I.sbBase + (uint32_t)(y >> 6) * I.sbCols + (uint32_t)(x >> 6)) * 12;
             constexpr int M = W > H ? W : H;
             bool keep = true;
             if constexpr (M < 64) {
@@ -93,8 +94,9 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
             }
             if constexpr (M > 4) {
                 constexpr int LV = M == 64 ? 0 : M == 32 ? 1 : M == 16 ? 2 : 3, BASE = LV == 0 ? 0 : LV == 1 ? 1 : LV == 2 ? 5 : 21;
+                constexpr int WORD = (W == 4 * H || H == 4 * W) ? 8 : 4;   // 4-way shapes, or the node's other own shapes
                 const int node = BASE + ((y & 63) / M) * (64 / M) + (x & 63) / M;
-                keep = keep && ((mk[4 + (node >> 5)] >> (node & 31)) & 1);
+                keep = keep && ((mk[WORD + (node >> 5)] >> (node & 31)) & 1);
             }
             if (!keep) {
                 if (lane == 0) { out.J = INF; out.ncand = 0; }
@@ -307,7 +309,7 @@ __global__ void __launch_bounds__(64) k_dp(Batch B, SearchParams P) {
                 const int x = sbx + nx * n, y = sby + ny * n;
                 if (x >= FW || y >= FH) {   // outside the frame: no partition (exported as 0, masked in training)
                     nJ[ni] = 0.f; nP[ni] = -1;
-                    if (B.dpRel) { B.dpRel[(size_t)sb * 170 + ni] = 0.f; B.dpRel[(size_t)sb * 170 + 85 + ni] = 0.f; }
+                    if (B.dpRel) for (int k = 0; k < 3; k++) B.dpRel[(size_t)sb * 255 + 85 * k + ni] = 0.f;
                     continue;
                 }
                 const bool hasRows = y + half < FH, hasCols = x + half < FW;
@@ -321,14 +323,17 @@ __global__ void __launch_bounds__(64) k_dp(Batch B, SearchParams P) {
                     const int cb = lvBase[Lv + 1], cper = per * 2;
                     for (int k = 0; k < 4; k++) splitJ = splitJ + nJ[cb + (ny * 2 + (k >> 1)) * cper + nx * 2 + (k & 1)];
                 }
-                float best = INF, bestOwn = INF, splitCost = INF;   // all options; own shapes only; SPLIT
+                // all options; and the best without the children (no N/2 squares), without the node's N x N / 2:1
+                // shapes (SPLIT or the 4-way partitions), without the 4-way shapes
+                float best = INF, noKids = INF, noOwn = INF, noFour = INF;
                 int bp = PARTITION_SPLIT;
                 auto consider = [&](int p, float J, float bits) {
                     const float t = fmaf(lambda, bits, J);
                     if (t < best) { best = t; bp = p; }
-                    if (p == PARTITION_SPLIT) splitCost = t;
-                    else if (p == PARTITION_NONE || p == PARTITION_HORZ || p == PARTITION_VERT || p == PARTITION_HORZ_4 || p == PARTITION_VERT_4)
-                        bestOwn = fminf(bestOwn, t);
+                    const bool four = p == PARTITION_HORZ_4 || p == PARTITION_VERT_4;
+                    if (four || p == PARTITION_NONE || p == PARTITION_HORZ || p == PARTITION_VERT) noKids = fminf(noKids, t);
+                    if (four || p == PARTITION_SPLIT) noOwn = fminf(noOwn, t);
+                    if (!four) noFour = fminf(noFour, t);
                 };
                 const bool big = n > P.maxPart;
                 if (hasRows && hasCols) {
@@ -373,10 +378,11 @@ __global__ void __launch_bounds__(64) k_dp(Batch B, SearchParams P) {
                 }
                 nJ[ni] = best;
                 nP[ni] = (int8_t)bp;
-                if (B.dpRel) {   // relative increase of the node's cost without its children / without its own shapes
+                if (B.dpRel) {   // relative increase of the node's cost without its children / own shapes / 4-way shapes
                     const float den = fmaxf(best, 1e-3f);
-                    B.dpRel[(size_t)sb * 170 + ni] = fminf((bestOwn - best) / den, 1e6f);
-                    B.dpRel[(size_t)sb * 170 + 85 + ni] = fminf((splitCost - best) / den, 1e6f);
+                    B.dpRel[(size_t)sb * 255 + ni] = fminf((noKids - best) / den, 1e6f);
+                    B.dpRel[(size_t)sb * 255 + 85 + ni] = fminf((noOwn - best) / den, 1e6f);
+                    B.dpRel[(size_t)sb * 255 + 170 + ni] = fminf((noFour - best) / den, 1e6f);
                 }
             }
     }

@@ -1,15 +1,15 @@
 // Learned candidate pruning: a small BF16 CNN per 64x64 superblock, run on MFMA with FP32 accumulation, that predicts
-// for each square partition node whether its children and its own shapes are needed, and which luma modes to screen.  No counterpart in the
-// reference (its pruning stages are hand-written heuristics: encoder/src/k_leafprep.cu:1-14, an_stage.h:1-15, the
-// screener hook gp.h:136-143).  The network only removes candidates from the exhaustive search, so every decision it
-// leads to is still an ordinary, decodable AV1 decision.
+// for each square partition node whether its children, its own shapes and its 4-way shapes are needed, and which luma
+// modes to screen.  No counterpart in the reference (its pruning stages are hand-written heuristics:
+// encoder/src/k_leafprep.cu:1-14, an_stage.h:1-15, the screener hook gp.h:136-143).  The network only removes
+// candidates from the exhaustive search, so every decision it leads to is still an ordinary, decodable AV1 decision.
 //
 // Network (per superblock):  x0 = [64 x 64 luma / max - 0.5, qindex / 255]  (NHWC, 2 channels, BF16)
 //   L1..L4: 3x3 convolution, stride 2, zero padding 1, 32 output channels, bias, ReLU   (64 -> 32 -> 16 -> 8 -> 4)
-//   L5: fully connected 512 -> 183 logits: 85 "expand" logits (node 0: the 64x64 node, 1-4: 32x32 nodes, 5-20: 16x16,
-//       21-84: 8x8, raster order per level: the search uses a block inside the node at most half its size), 85 "own"
-//       logits (same nodes: it uses a block whose larger side is the node's size, i.e. one of the node's own partition
-//       shapes) and 13 luma-mode logits (the mode is used by some block of the superblock).
+//   L5: fully connected 512 -> 268 logits: 85 "expand" logits (node 0: the 64x64 node, 1-4: 32x32 nodes, 5-20: 16x16,
+//       21-84: 8x8, raster order per level: the node needs its children), 85 "own" logits (same nodes: it needs its
+//       N x N, N x N/2 or N/2 x N shapes), 85 "four" logits (it needs its N x N/4 or N/4 x N shapes: HORZ_4 / VERT_4)
+//       and 13 luma-mode logits (the mode is used by some block of the superblock).
 // Convolutions are implicit GEMMs on v_mfma_f32_16x16x16_bf16 (positions x input taps times taps x channels), weights
 // and activations BF16, accumulation and biases FP32.  Training runs the same forward kernels, then FP32 backward
 // kernels (convolution weight gradients accumulated with float atomics) and Adam on FP32 master weights.
@@ -24,7 +24,8 @@ typedef uint16_t bf16_t;
 typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
-constexpr int LN_NODES = 85, LN_OWN0 = 85, LN_MODE0 = 170, LN_OUT = 183, LN_FEAT = 512, LN_C = 32;
+constexpr int LN_NODES = 85, LN_OWN0 = 85, LN_FOUR0 = 170, LN_MODE0 = 255, LN_OUT = 268, LN_FEAT = 512, LN_C = 32;
+constexpr int LN_MASKW = 12;   // mask words per superblock
 
 __device__ __forceinline__ bf16_t to_bf16(float f) {   // round to nearest even
     const uint32_t u = __float_as_uint(f);
@@ -117,13 +118,13 @@ __global__ void __launch_bounds__(128) k_ln_fc(const bf16_t *a4, const float *W,
     }
 }
 
-// ---- decisions: per superblock the nodes whose children / own shapes are analysed and the luma modes screened ----
-// A node is expanded (its children analysed) unless the network is confident (p <= 1 - conf) that the search would not
-// use its children, and keeps its own shapes unless it is as confident that those go unused; a node where both would
-// be dropped keeps its own shapes (every node reached keeps a partition), and a node reaching past the frame keeps
-// both (the partition syntax may force a split there).  The modes are the 8 most likely when the network is confident
-// that none of the other 5 is used (product of their 1 - p at least conf), else all 13.
-// mask (8 words per superblock): 0-2 expand bits, 3 screened modes, 4-6 own bits, 7 zero.
+// ---- decisions: per superblock the nodes whose children / own / 4-way shapes are analysed, the luma modes screened ----
+// A node is expanded (its children analysed) unless the network is confident (p <= 1 - conf) that the search does not
+// need its children; likewise it keeps its own (N x N, N x N/2, N/2 x N) and its 4-way (N x N/4, N/4 x N) shapes.  A
+// node where all three would be dropped keeps its own shapes (every node reached keeps a partition), and a node reaching
+// past the frame keeps everything (the partition syntax may force a split there).  The modes are the 8 most likely
+// when the network is confident that none of the other 5 is used (product of their 1 - p at least conf), else all 13.
+// mask (12 words per superblock): 0-2 expand bits, 3 screened modes, 4-6 own bits, 8-10 four bits, 7 and 11 zero.
 // logits of slots 0 .. n - 1 belong to superblocks s0 .. s0 + n - 1 of the batch; mask is indexed by superblock
 __global__ void __launch_bounds__(64) k_ln_decide(const float *logits, const Img *imgs, const uint32_t *sbImg, uint32_t s0, uint32_t n,
                                                   float conf, uint32_t *mask) {
@@ -134,7 +135,7 @@ __global__ void __launch_bounds__(64) k_ln_decide(const float *logits, const Img
     const int sx = (int)(local % I.sbCols) * 64, sy = (int)(local / I.sbCols) * 64;
     const int FW = I.MiCols * 4, FH = I.MiRows * 4;
     const float *z = logits + (size_t)slot * LN_OUT;
-    uint32_t m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t m[LN_MASKW] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int node = 0;
     for (int lv = 0; lv < 4; lv++) {
         const int n = 64 >> lv, per = 1 << lv;
@@ -142,10 +143,12 @@ __global__ void __launch_bounds__(64) k_ln_decide(const float *logits, const Img
             for (int nx = 0; nx < per; nx++, node++) {
                 const bool edge = sx + nx * n + n > FW || sy + ny * n + n > FH;
                 const float pe = 1.f / (1.f + expf(-z[node])), po = 1.f / (1.f + expf(-z[LN_OWN0 + node]));
-                const bool ex = edge || pe > 1.f - conf;
-                const bool ow = edge || po > 1.f - conf || !ex;
+                const float pf = 1.f / (1.f + expf(-z[LN_FOUR0 + node]));
+                const bool ex = edge || pe > 1.f - conf, fo = edge || pf > 1.f - conf;
+                const bool ow = edge || po > 1.f - conf || (!ex && !fo);
                 if (ex) m[node >> 5] |= 1u << (node & 31);
                 if (ow) m[4 + (node >> 5)] |= 1u << (node & 31);
+                if (fo) m[8 + (node >> 5)] |= 1u << (node & 31);
             }
     }
     float p[13];
@@ -159,7 +162,7 @@ __global__ void __launch_bounds__(64) k_ln_decide(const float *logits, const Img
     float none = 1.f;   // probability that no mode outside the shortlist is used
     for (int k = 0; k < 13; k++) if (!((top >> k) & 1)) none *= 1.f - p[k];
     m[3] = none >= conf ? top : 0x1FFFu;
-    for (int k = 0; k < 8; k++) mask[(size_t)sb * 8 + k] = m[k];
+    for (int k = 0; k < LN_MASKW; k++) mask[(size_t)sb * LN_MASKW + k] = m[k];
 }
 
 // ================================ training ================================
@@ -379,7 +382,7 @@ void ln_delete(LnNet *N) {
     delete N;
 }
 
-// learned masks of a batch (encoder integration): mask[nSb][8].  scratch: ln_scratch_bytes(chunk) device bytes
+// learned masks of a batch (encoder integration): mask[nSb][12].  scratch: ln_scratch_bytes(chunk) device bytes
 // (stream-ordered memory of the encode call, so the network allocates and frees nothing itself).
 int ln_masks(const LnNet &N, void *scratch, const Img *dImg, const uint32_t *dSbImg, uint32_t nSb, float conf, uint32_t chunk,
              uint32_t *dMask, hipStream_t st) {

@@ -44,6 +44,7 @@ struct RcTabs {
     float txScale[19];
     float txg[19][16];
     uint16_t dscan[DSCAN_TOTAL];   // default scans (offsets dscan_off)
+    uint16_t iscan[DSCAN_TOTAL];   // their inverses: scan index of each position
     uint8_t cbOff[19][25];         // Coeff_Base_Ctx_Offset
 };
 using RcEvalIn = EvalInT<const lds<CostTab> *>;
@@ -51,6 +52,8 @@ template <int TSZ>
 __device__ __forceinline__ const lds<uint16_t> *dscan_ptr(const lds<RcTabs> &T) { return T.dscan + dscan_off(av1t::Adjusted_Tx_Size[TSZ]); }
 template <int TSZ>
 __device__ __forceinline__ const lds<uint8_t> *cboff_ptr(const lds<RcTabs> &T) { return T.cbOff[TSZ]; }
+template <int TSZ>
+__device__ __forceinline__ const lds<uint16_t> *iscan_ptr(const lds<RcTabs> &T) { return T.iscan + dscan_off(av1t::Adjusted_Tx_Size[TSZ]); }
 
 union RcScr {
     RcSmall s;
@@ -894,7 +897,7 @@ __device__ void stage_tables(lds<RcShared> &S, const DevTables &T, int qctx, boo
             if (av1t::Tx_Width[t] == 64 || av1t::Tx_Height[t] == 64) continue;
             const uint16_t *src = scan_of(t, DCT_DCT);
             const int n = av1t::Tx_Width[t] * av1t::Tx_Height[t], off = dscan_off(t);
-            for (int k = tid; k < n; k += RC_NT) S.tab.dscan[off + k] = src[k];
+            for (int k = tid; k < n; k += RC_NT) { S.tab.dscan[off + k] = src[k]; S.tab.iscan[off + src[k]] = (uint16_t)k; }
         }
         for (int k = tid; k < 19 * 25; k += RC_NT) S.tab.cbOff[k / 25][k % 25] = (&av1t::Coeff_Base_Ctx_Offset[0][0][0])[k];
     }
@@ -1073,6 +1076,7 @@ AV1M_RC_KERNEL k_recon_queue(Batch B, SearchParams P, RcQueue Q) {
     }
 }
 
+static_assert(sizeof(RcShared) <= 160 * 1024, "the closed loop's LDS exceeds a compute unit's 160 KiB");
 size_t recon_smem_bytes() { return sizeof(RcShared); }
 
 int recon_prepare(int bd) {

@@ -163,6 +163,12 @@ void build_tables(DevTables &T) {
             T.lambda[b][q] = (float)(rd / 2048.0 * 1.0);
             T.lamS[b][q] = sqrtf(T.lambda[b][q]);
         }
+    for (int t = 0; t < 19; t++) {
+        if (av1t::Tx_Width[t] == 64 || av1t::Tx_Height[t] == 64) continue;
+        const uint16_t *s = scan_of(t, DCT_DCT);
+        const int n = av1t::Tx_Width[t] * av1t::Tx_Height[t];
+        for (int k = 0; k < n; k++) T.iscan[dscan_off(t) + s[k]] = (uint16_t)k;
+    }
 }
 
 }  // namespace av1m

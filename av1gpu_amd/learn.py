@@ -4,12 +4,13 @@ The network (av1gpu_amd/csrc/k_learn.hip) sees the 64x64 luma of a superblock an
   * 85 "expand" outputs, one per square partition node (the 64x64 node, four 32x32, sixteen 16x16, sixty-four 8x8, raster
     order per level): 1 when some leaf of the exhaustive search inside the node is at most half its size in both
     dimensions (its children are used);
-  * 85 "own" outputs, same nodes: 1 when some leaf inside the node has the node's size as its larger side (one of the
-    node's own partition shapes: N x N, N x N/2, N/2 x N, N x N/4, N/4 x N);
+  * 85 "own" outputs, same nodes: 1 when some leaf inside the node is one of the node's N x N, N x N/2 or N/2 x N shapes;
+  * 85 "four" outputs, same nodes: 1 when some leaf inside the node is one of its 4-way shapes (N x N/4, N/4 x N);
   * 13 luma-mode outputs: 1 when some block of the superblock uses that luma mode in the exhaustive result.
-The encoder then analyses a node's children / own shapes unless the network is confident (probability at most
-1 - conf) that they are unused (a node would lose both: it keeps its own shapes), and screens only the 8 most likely
-modes where it is confident (the product of 1 - p over the 5 others at least conf) that none of the others is used.
+The encoder then analyses a node's children / own shapes / 4-way shapes unless the network is confident (probability
+at most 1 - conf) that they are unused (a node that would lose all three keeps its own shapes), and screens only the 8
+most likely modes where it is confident (the product of 1 - p over the 5 others at least conf) that none of the others
+is used.  With cost-aware labels (rd_labels) "used" becomes "dropping them costs more than tau".
 
 Training data are the exhaustive search's own decisions on seeded synthetic images; training runs on the GPU with the
 library's forward (MFMA) and backward kernels.  tests/ln_ref.py restates the same network in float64 NumPy with the
@@ -24,7 +25,7 @@ import numpy as np
 
 import av1gpu_amd as A
 
-N_SPLIT, OWN0, MODE0, N_OUT, N_FEAT = 85, 85, 170, 183, 512   # node count / first own / first mode output
+N_SPLIT, OWN0, FOUR0, MODE0, N_OUT, N_FEAT = 85, 85, 170, 255, 268, 512   # node count / first own, 4-way, mode output
 LEVEL_BASE = (0, 1, 5, 21)
 KP = (32, 288, 288, 288)
 CIN = (2, 32, 32, 32)
@@ -64,7 +65,7 @@ def init_params(seed=1):
 
 
 def unpack(p):
-    """Views of the parameter vector: [(W_l [32][KP_l], b_l [32]) for l in 0..3], W5 [183][512], b5 [183]."""
+    """Views of the parameter vector: [(W_l [32][KP_l], b_l [32]) for l in 0..3], W5 [268][512], b5 [268]."""
     out, off = [], 0
     for l in range(4):
         w = p[off:off + 32 * KP[l]].reshape(32, KP[l])
@@ -87,8 +88,8 @@ def sb_tiles(y, W, H):
 
 
 def sb_labels(blocks, W, H):
-    """Labels and label masks [nSb, 183] of a record's leaves (see the module docstring); nodes reaching past the
-    frame are masked out (the encoder keeps their children and own shapes regardless)."""
+    """Labels and label masks [nSb, 268] of a record's leaves (see the module docstring); nodes reaching past the
+    frame are masked out (the encoder keeps all their shapes regardless)."""
     MiCols, MiRows = 2 * ((W + 7) >> 3), 2 * ((H + 7) >> 3)
     sbc, sbr = (MiCols + 15) >> 4, (MiRows + 15) >> 4
     lab = np.zeros((sbr * sbc, N_OUT), np.float32)
@@ -100,13 +101,16 @@ def sb_labels(blocks, W, H):
     sb = (y // 64) * sbc + x // 64
     lx, ly = x % 64, y % 64
     m = np.maximum(bw, bh)
+    four = np.maximum(bw, bh) == 4 * np.minimum(bw, bh)
     for lv in range(4):
         n = 64 >> lv
         node = LEVEL_BASE[lv] + (ly // n) * (64 // n) + lx // n
         sel = n >= 2 * m
         lab[sb[sel], node[sel]] = 1.0
-        sel = n == m
+        sel = (n == m) & ~four
         lab[sb[sel], OWN0 + node[sel]] = 1.0
+        sel = (n == m) & four
+        lab[sb[sel], FOUR0 + node[sel]] = 1.0
         per = 1 << lv
         for ny in range(per):
             for nx in range(per):
@@ -116,6 +120,7 @@ def sb_labels(blocks, W, H):
                 out = (sxs + n > FW) | (sys + n > FH)
                 msk[out, nd] = 0.0
                 msk[out, OWN0 + nd] = 0.0
+                msk[out, FOUR0 + nd] = 0.0
     # modes: used by some block of the superblock
     lab[sb, MODE0 + blocks["y_mode"].astype(np.int64)] = 1.0
     return lab, msk
@@ -123,7 +128,7 @@ def sb_labels(blocks, W, H):
 
 def dp_export(enc, on=True):
     """Make the encoder keep the partition DP's per-node relative costs (on) and return those of its last call:
-    [nSb, 170] float32, superblocks in batch order (see av1mi_set_dp_export)."""
+    [nSb, 255] float32, superblocks in batch order (see av1mi_set_dp_export)."""
     L = _sigs()
     if on is not None:
         L.av1mi_set_dp_export(enc._h, 1 if on else 0)
@@ -131,19 +136,19 @@ def dp_export(enc, on=True):
     out = np.zeros(n, np.float32)
     if n:
         L.av1mi_dp_export(enc._h, out.ctypes.data, n)
-    return out.reshape(-1, 170)
+    return out.reshape(-1, 3 * N_SPLIT)
 
 
 def rd_labels(rel, lab, tau, tau_root=None):
-    """Cost-aware node labels: a node needs its children (own shapes) when the DP's best cost without them is more
-    than tau (relative) above its best -- tau_root for the 64x64 node, if given; the mode labels of lab stay.
-    rel: [nSb, 170] of dp_export for one image."""
+    """Cost-aware node labels: a node needs its children (own shapes, 4-way shapes) when the DP's best cost without
+    them is more than tau (relative) above its best -- tau_root for the 64x64 node, if given; the mode labels of lab
+    stay.  rel: [nSb, 255] of dp_export for one image."""
     t = np.full(N_SPLIT, tau, np.float32)
     if tau_root is not None:
         t[0] = tau_root
     out = lab.copy()
-    out[:, :N_SPLIT] = (rel[:, :N_SPLIT] > t).astype(np.float32)
-    out[:, OWN0:MODE0] = (rel[:, N_SPLIT:] > t).astype(np.float32)
+    for k, o in enumerate((0, OWN0, FOUR0)):
+        out[:, o:o + N_SPLIT] = (rel[:, k * N_SPLIT:(k + 1) * N_SPLIT] > t).astype(np.float32)
     return out
 
 

@@ -3,8 +3,9 @@
   python -m av1gpu_amd.train_shortlist [--sbs 2000000] [--steps 60000] [--train-budget SECONDS] [--out av1gpu_amd/shortlist_params.npy]
 
 1. Labels: seeded synthetic images (sizes log-uniform over 0.2-4 MP, aspect 0.5-2, qindex 80/120/160, seeds from
-   20000) are encoded with the exhaustive search; every superblock's leaves give its 85 node labels and 13 mode shares
-   (learn.sb_labels), its 64x64 luma tile is the input.
+   20000) are encoded with the exhaustive search; every superblock's leaves give its 3 x 85 node labels and 13 mode
+   labels (learn.sb_labels; with --rd-tau the node labels come from the partition DP's costs, learn.rd_labels), its
+   64x64 luma tile is the input.
 2. Training: Adam (lr cosine-decayed from --lr), batches of --batch superblocks, on the library's own kernels.
 3. Evaluation on held-out images (seeds from 30000) at qindex 60/100/140/180: BD-rate (PSNR-Y, bits of this build's
    own bitstream writer) of the pruned search against the exhaustive one, and the share of analysis work pruned.

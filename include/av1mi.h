@@ -123,15 +123,17 @@ void av1mi_frame_out_free(av1mi_frame_out *out);
 
 /* Learned candidate pruning (no counterpart in the reference, whose pruning stages are heuristics:
  * encoder/src/k_leafprep.cu, an_stage.h, gp.h:136-143).  A BF16 CNN per 64x64 superblock predicts, per square
- * partition node, whether the analysis needs its children and its own shapes, and a top-8 luma-mode shortlist; below
+ * partition node, whether the analysis needs its children, its N x N / 2:1 shapes and its 4-way (4:1) shapes, and a
+ * top-8 luma-mode shortlist; below
  * the confidence the search stays exhaustive (per node, and for the shortlist per superblock).  params:
  * av1mi_ln_param_count() floats (NULL disables).  The network runs in the encode call's stream-ordered memory,
  * AV1MI_LN_CHUNK superblocks at a time (environment, read here; default 65536, ~100 KB of activations each). */
 int av1mi_set_shortlist(av1mi_encoder *enc, const float *params, float confidence, char *err, int errlen);
 size_t av1mi_ln_param_count(void);
 /* Training data for the pruning network (no counterpart in the reference): with on != 0, each later av1mi_encode keeps
- * the partition DP's per-node relative cost increases, superblocks in batch order, 170 floats each: (best cost without
- * the node's children - best) / best for the 85 square nodes, then the same without the node's own shapes.
+ * the partition DP's per-node relative cost increases, superblocks in batch order, 255 floats each: (best cost without
+ * the node's children - best) / best for the 85 square nodes, then the same without the node's N x N / 2:1 shapes,
+ * then without its 4-way shapes.
  * av1mi_dp_export copies up to n floats of the last call and returns how many there are. */
 int av1mi_set_dp_export(av1mi_encoder *enc, int on);
 size_t av1mi_dp_export(const av1mi_encoder *enc, float *out, size_t n);

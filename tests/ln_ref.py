@@ -3,7 +3,7 @@ its logits: the checker of the network kernels' activations and gradients and of
 (tests/test_gpu_learn.py).  Test infrastructure only."""
 import numpy as np
 
-from av1gpu_amd.learn import CIN, KP, LEVEL_BASE, MODE0, N_SPLIT, OWN0, param_count, unpack
+from av1gpu_amd.learn import CIN, FOUR0, KP, LEVEL_BASE, MODE0, N_SPLIT, OWN0, param_count, unpack
 
 
 def bf16(x):
@@ -84,8 +84,8 @@ def ref_grads(params, acts, label, lmask):
 
 def decide(logits, W, H, conf):
     """The decisions the encoder takes from the logits of an image's superblocks (raster order): expand [nSb, 85] bool
-    (a node's children are analysed), own [nSb, 85] bool (its own shapes are) and the luma-mode mask [nSb] (bit m:
-    mode m screened)."""
+    (a node's children are analysed), own [nSb, 85] bool (its N x N / 2:1 shapes are), four [nSb, 85] bool (its 4-way
+    shapes are) and the luma-mode mask [nSb] (bit m: mode m screened)."""
     z = np.asarray(logits, np.float64)
     MiCols, MiRows = 2 * ((W + 7) >> 3), 2 * ((H + 7) >> 3)
     FW, FH = 4 * MiCols, 4 * MiRows
@@ -95,6 +95,7 @@ def decide(logits, W, H, conf):
     p = 1.0 / (1.0 + np.exp(-z))
     expand = np.zeros((len(z), N_SPLIT), bool)
     own = np.zeros((len(z), N_SPLIT), bool)
+    four = np.zeros((len(z), N_SPLIT), bool)
     for lv in range(4):
         n, per = 64 >> lv, 1 << lv
         for ny in range(per):
@@ -102,7 +103,8 @@ def decide(logits, W, H, conf):
                 nd = LEVEL_BASE[lv] + ny * per + nx
                 edge = (sx + nx * n + n > FW) | (sy + ny * n + n > FH)
                 expand[:, nd] = edge | (p[:, nd] > 1.0 - conf)
-                own[:, nd] = edge | (p[:, OWN0 + nd] > 1.0 - conf) | ~expand[:, nd]
+                four[:, nd] = edge | (p[:, FOUR0 + nd] > 1.0 - conf)
+                own[:, nd] = edge | (p[:, OWN0 + nd] > 1.0 - conf) | (~expand[:, nd] & ~four[:, nd])
     pm = p[:, MODE0:]
     order = np.argsort(-pm, axis=1, kind="stable")      # ties: the lower mode first
     top, rest = order[:, :8], order[:, 8:]
@@ -110,4 +112,4 @@ def decide(logits, W, H, conf):
     mask = np.full(len(z), 0x1FFF, np.int64)
     bits = np.sum(1 << top, axis=1)
     mask[none >= conf] = bits[none >= conf]
-    return expand, own, mask
+    return expand, own, four, mask

@@ -86,12 +86,13 @@ def test_training_lowers_the_loss():
     L.close()
 
 
-def _params_bias(split_bias, mode_bias, own_bias=20.0):
+def _params_bias(split_bias, mode_bias, own_bias=20.0, four_bias=20.0):
     p = Ln.init_params(1)
     convs, w5, b5 = Ln.unpack(p)
     w5[:] = 0.0
     b5[:Ln.N_SPLIT] = split_bias
-    b5[Ln.OWN0:Ln.MODE0] = own_bias
+    b5[Ln.OWN0:Ln.FOUR0] = own_bias
+    b5[Ln.FOUR0:Ln.MODE0] = four_bias
     b5[Ln.MODE0:] = mode_bias
     return p
 
@@ -159,8 +160,8 @@ def test_network_chunks_and_batching_do_not_change_records(monkeypatch):
 
 def test_records_obey_the_restated_decisions():
     """Every block of a pruned record lies in a node the decisions (ln_ref.decide over the network's logits) expand, is
-    one of the own shapes of a node that keeps them, and uses a luma mode of its superblock's mask; a confidence where
-    the masks are mixed is used (some nodes pruned, some superblocks shortlisted)."""
+    one of the own (or 4-way) shapes of a node that keeps them, and uses a luma mode of its superblock's mask; a
+    confidence where the masks are mixed is used (some nodes pruned, some superblocks shortlisted)."""
     p = Ln.init_params(22)
     convs, w5, b5 = Ln.unpack(p)
     w5 *= 4.0
@@ -176,8 +177,8 @@ def test_records_obey_the_restated_decisions():
     conf = 0.8
     Ln.set_shortlist(enc, p, conf)
     g = enc.encode([f])[0][0]
-    expand, own, mmask = R.decide(logits, W, H, conf)
-    assert 0 < expand[:, 1:].mean() < 1 and 0 < own.mean() < 1 and (mmask != 0x1FFF).any()
+    expand, own, four, mmask = R.decide(logits, W, H, conf)
+    assert 0 < expand[:, 1:].mean() < 1 and 0 < own.mean() < 1 and 0 < four.mean() < 1 and (mmask != 0x1FFF).any()
     sbc = (2 * ((W + 7) >> 3) + 15) >> 4
     w4 = np.array([1 << v for v in Ln._MI_W_LOG2])
     h4 = np.array([1 << v for v in Ln._MI_H_LOG2])
@@ -193,7 +194,7 @@ def test_records_obey_the_restated_decisions():
             assert expand[sb, node], (x, y, s)
         if M > 4:
             node = Ln.LEVEL_BASE[lvl[M]] + ((y % 64) // M) * (64 // M) + (x % 64) // M
-            assert own[sb, node], (x, y, s)
+            assert (four if max(w4[s], h4[s]) == 4 * min(w4[s], h4[s]) else own)[sb, node], (x, y, s)
     enc.close()
     f.close()
 
@@ -201,7 +202,9 @@ def test_records_obey_the_restated_decisions():
 def test_dp_export_is_consistent_with_the_records():
     """The partition DP's exported relative costs (training labels) change no record, are never negative, and agree
     with the choices at each superblock's root: one coded as a single 64x64 leaf chose NONE, so dropping the root's
-    children costs 0; one without any leaf of side 64 chose SPLIT, so dropping the root's own shapes costs 0."""
+    children or its 4-way shapes costs 0; one without any leaf of side 64 chose SPLIT, so dropping the root's own or
+    4-way shapes costs 0; one with a 64x16 or 16x64 leaf chose a 4-way partition, so dropping the root's children or
+    own shapes costs 0."""
     W, H = 328, 264
     f = A.DeviceFrame(A.synth_image(1300, W, H), W, H, qindex=255)
     e = A.GpEncoder().init(A.GpConfig(qindex=120))
@@ -213,7 +216,7 @@ def test_dp_export_is_consistent_with_the_records():
     for k in ("blocks", "txb", "coeffs"):
         assert np.array_equal(getattr(base, k), getattr(g, k)), k
     sbc = (2 * ((W + 7) >> 3) + 15) >> 4
-    assert rel.shape == ((2 * ((H + 7) >> 3) + 15) // 16 * sbc, 170) and (rel > -1e-6).all()
+    assert rel.shape == ((2 * ((H + 7) >> 3) + 15) // 16 * sbc, 255) and (rel > -1e-6).all()
     w4 = np.array([1 << v for v in Ln._MI_W_LOG2])
     h4 = np.array([1 << v for v in Ln._MI_H_LOG2])
     b = g.blocks
@@ -226,10 +229,12 @@ def test_dp_export_is_consistent_with_the_records():
             continue                                     # edge superblocks: forced partitions
         bs = b["bsize"][sb == s]
         if len(bs) == 1 and bs[0] == 12:                 # PARTITION_NONE at the root
-            assert rel[s, 0] == 0.0
+            assert rel[s, 0] == 0.0 and rel[s, 170] == 0.0
             none64 += 1
         if not (side[sb == s] == 64).any():              # SPLIT at the root
-            assert rel[s, 85] == 0.0
+            assert rel[s, 85] == 0.0 and rel[s, 170] == 0.0
+        if np.isin(bs, (20, 21)).any():                  # HORZ_4 / VERT_4 at the root
+            assert rel[s, 0] == 0.0 and rel[s, 85] == 0.0
         checked += 1
     assert checked > 0
     e.close()

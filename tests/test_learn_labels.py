@@ -2,8 +2,9 @@
 over oracle records: every 4x4 unit of the coded area is painted with the size and luma mode of the block that covers
 it (which also checks that the blocks tile the area exactly once, pinning the block-size tables), and a node's expand
 label is then "some unit inside it belongs to a block at most half the node's size", its own label "some unit inside it
-belongs to a block whose larger side is the node's size", its mode labels the modes that appear among the painted
-units."""
+belongs to a block whose larger side is the node's size and at most twice its smaller side", its four label the same
+with the larger side 4 times the smaller (HORZ_4 / VERT_4 shapes), its mode labels the modes that appear among the
+painted units."""
 import numpy as np
 import pytest
 
@@ -16,15 +17,17 @@ H4 = (1, 2, 1, 2, 4, 2, 4, 8, 4, 8, 16, 8, 16, 32, 16, 32, 4, 1, 8, 2, 16, 4)
 
 def _paint(blocks, mic, mir):
     size = np.zeros((mir, mic), np.int64)
+    ratio = np.zeros((mir, mic), np.int64)
     mode = np.full((mir, mic), -1, np.int64)
     cover = np.zeros((mir, mic), np.int64)
     for b in blocks:
         r, c, s = int(b["mi_row"]), int(b["mi_col"]), int(b["bsize"])
         h, w = min(H4[s], mir - r), min(W4[s], mic - c)
         size[r:r + h, c:c + w] = 4 * max(W4[s], H4[s])
+        ratio[r:r + h, c:c + w] = max(W4[s], H4[s]) // min(W4[s], H4[s])
         mode[r:r + h, c:c + w] = int(b["y_mode"])
         cover[r:r + h, c:c + w] += 1
-    return size, mode, cover
+    return size, ratio, mode, cover
 
 
 @pytest.mark.parametrize("W,H,q,seed", [(200, 136, 120, 41), (264, 200, 40, 42), (72, 264, 200, 43)])
@@ -32,7 +35,7 @@ def test_labels_match_raster_restatement(oracle, W, H, q, seed):
     planes = A.synth_image(seed, W, H)
     r = oracle.encode(A.pad_planes(planes, W, H)[0], W, H, qindex=q, threads=8)
     mic, mir = r["MiCols"], r["MiRows"]
-    size, mode, cover = _paint(r["blocks"], mic, mir)
+    size, ratio, mode, cover = _paint(r["blocks"], mic, mir)
     assert (cover == 1).all()                      # the record's blocks tile the coded area exactly once
     lab, msk = Ln.sb_labels(r["blocks"], W, H)
     sbc, sbr = (mic + 15) // 16, (mir + 15) // 16
@@ -46,13 +49,20 @@ def test_labels_match_raster_restatement(oracle, W, H, q, seed):
                     node = Ln.LEVEL_BASE[lv] + ny * (1 << lv) + nx
                     y0, x0 = r0 + ny * n4, c0 + nx * n4
                     inside = y0 + n4 <= mir and x0 + n4 <= mic
-                    assert msk[s, node] == (1.0 if inside else 0.0) and msk[s, Ln.OWN0 + node] == msk[s, node]
+                    assert msk[s, node] == (1.0 if inside else 0.0)
+                    assert msk[s, Ln.OWN0 + node] == msk[s, node] and msk[s, Ln.FOUR0 + node] == msk[s, node]
                     sub = size[y0:y0 + n4, x0:x0 + n4]
+                    rat = ratio[y0:y0 + n4, x0:x0 + n4]
                     if sub.size:
                         assert lab[s, node] == (1.0 if sub.min() * 2 <= 4 * n4 else 0.0), (s, lv, ny, nx)
-                        assert lab[s, Ln.OWN0 + node] == (1.0 if (sub == 4 * n4).any() else 0.0), (s, lv, ny, nx)
-                        if sub.max() <= 4 * n4:        # not inside a larger block: its children or own shapes used
-                            assert lab[s, node] + lab[s, Ln.OWN0 + node] >= 1.0
+                        own = ((sub == 4 * n4) & (rat <= 2)).any()
+                        four = ((sub == 4 * n4) & (rat == 4)).any()
+                        assert lab[s, Ln.OWN0 + node] == (1.0 if own else 0.0), (s, lv, ny, nx)
+                        assert lab[s, Ln.FOUR0 + node] == (1.0 if four else 0.0), (s, lv, ny, nx)
+                        if lv == 3:
+                            assert not four              # 8x8 nodes have no 4-way partitions
+                        if sub.max() <= 4 * n4:        # not inside a larger block: children, own or 4-way shapes used
+                            assert lab[s, node] + lab[s, Ln.OWN0 + node] + lab[s, Ln.FOUR0 + node] >= 1.0
                     if lv and lab[s, node]:            # children used implies the parent's children used
                         parent = Ln.LEVEL_BASE[lv - 1] + (ny // 2) * (1 << (lv - 1)) + nx // 2
                         assert lab[s, parent] == 1.0
