@@ -467,6 +467,18 @@ __device__ __forceinline__ int hadamard4(int *d) {
     return sum;
 }
 
+// the same transform with one value of d per lane: lanes 16k .. 16k + 15 of the group hold one 4x4 block in raster
+// order (lane bits 0-1: column, 2-3: row).  The butterflies are those of hadamard4 (rows, then columns), so the 16
+// lanes return the absolute values of its 16 coefficients, whose sum is its result.  Every lane of the group calls it.
+__device__ __forceinline__ int hadamard4_lanes(int v) {
+    const int l = glane();
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+        const int t = __shfl_xor(v, o, 32);
+        v = (l & o) ? t - v : v + t;
+    }
+    return abs(v);
+}
 
 // runtime tx-size dispatch to a template taking <W, H>
 #define AV1M_TX_SWITCH(txSz, M)                                                                                   \

@@ -129,9 +129,32 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
         const int n13 = __popc(modeMask);
         if (lane < 13 && ((modeMask >> lane) & 1)) S.cand[__popc(modeMask & ((1u << lane) - 1))] = cand_pack(lane, 0);
         gsync();
-        // item = (candidate, 4x4); a candidate's NSUB items are consecutive, so each segment of min(NSUB, 32) lanes
-        // sums its Hadamard costs with shuffles before one LDS atomic
+        // blocks of up to 8 4x4s: item = (candidate, pixel), 16 lanes per 4x4 and the Hadamard across them, so a
+        // round holds at most two candidates (modes); larger blocks: item = (candidate, 4x4).  A candidate's items are
+        // consecutive, so each segment of up to 32 lanes sums its Hadamard costs with shuffles before one LDS atomic.
         auto screen = [&](int c0, int c1) {
+            if constexpr (NSUB <= 8) {
+                constexpr int PIX = NSUB * 16, SEG = PIX < GW ? PIX : GW;
+                const int items = (c1 - c0) * PIX;
+                for (int base = 0; base < items; base += GW) {
+                    const int it = base + lane;
+                    int v = 0, ci = c0;
+                    if (it < items) {
+                        ci = c0 + it / PIX;
+                        const int px = it % PIX, sb = px >> 4;
+                        const int sx = (sb % NSX) * 4 + (px & 3), sy = (sb / NSX) * 4 + ((px >> 2) & 3);
+                        const int c = S.cand[ci], m = cand_mode(c), pa = pangle(m, cand_delta(c));
+                        const int p = m == DC_PRED ? dcv : pred_px(m, pa, sy, sx, LW, LH, A, L, 0, 0);
+                        v = (int)src[sy * sp + sx] - p;
+                    }
+                    v = hadamard4_lanes(v);
+#pragma unroll
+                    for (int o = SEG / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+                    if (it < items && (it & (SEG - 1)) == 0) atomicAdd(&S.satdI[ci], v);
+                }
+                gsync();
+                return;
+            }
             constexpr int SEG = NSUB < GW ? NSUB : GW;
             const int items = (c1 - c0) * NSUB;
             for (int base = 0; base < items; base += GW) {

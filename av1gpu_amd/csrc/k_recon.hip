@@ -738,9 +738,35 @@ __device__ void rc_chroma(lds<RcShared> &S, const Batch &B, const SearchParams &
     if (tid < 13) S.uvc[tid] = cand_pack(tid, 0);
     __syncthreads();
     constexpr int NSUB = (W / 4) * (H / 4), NSX = W / 4;
-    // screening: item = (candidate, plane, 4x4); a candidate's 2 NSUB items are consecutive, so each segment of
-    // min(2 NSUB, 32) lanes sums one candidate's Hadamard costs with shuffles before one LDS atomic
+    // screening of blocks of up to 16 4x4s: item = (candidate, plane, pixel), 16 lanes per 4x4 and the Hadamard
+    // across them, so a wave holds at most two candidates (modes); larger blocks: item = (candidate, plane, 4x4).  A
+    // candidate's items are consecutive, so each segment of up to 32 lanes sums its costs with shuffles before one LDS
+    // atomic.
     auto screen = [&](int c0, int c1) {
+        if constexpr (NSUB <= 16) {
+            constexpr int PIX = NSUB * 16;
+            const int items = (c1 - c0) * PIX * 2;
+            for (int base = 0; base < items; base += RC_NT) {
+                const int it = base + tid;
+                int v = 0, ci = c0;
+                if (it < items) {
+                    ci = c0 + it / (2 * PIX);
+                    const int rr = it % (2 * PIX), p = 1 + rr / PIX, px = rr % PIX, sbk = px >> 4;
+                    const int sx = (sbk % NSX) * 4 + (px & 3), sy = (sbk / NSX) * 4 + ((px >> 2) & 3);
+                    const int c = S.uvc[ci], m = cand_mode(c), pa = pangle(m, cand_delta(c));
+                    const lds<uint16_t> *s = p == 1 ? srcU : srcV;
+                    const lds<uint16_t> *A = S.ea[p] + EDGE_OFF, *L = S.el[p] + EDGE_OFF;
+                    const int pr = m == DC_PRED ? S.dcv[p] : pred_px(m, pa, sy, sx, LW, LH, A, L, 0, 0);
+                    v = (int)s[sy * sp + sx] - pr;
+                }
+                v = hadamard4_lanes(v);
+#pragma unroll
+                for (int o = GW / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+                if (it < items && (it & (GW - 1)) == 0) atomicAdd(gen(&S.satdI[ci]), v);
+            }
+            __syncthreads();
+            return;
+        }
         constexpr int SEG = 2 * NSUB < GW ? 2 * NSUB : GW;
         const int items = (c1 - c0) * NSUB * 2;
         for (int base = 0; base < items; base += RC_NT) {
