@@ -81,6 +81,7 @@ struct RcShared {
     int cflAvg, cflAlpha[2];
     unsigned long long acc64[8];
     int satdI[32], uvc[32];
+    float uvJ[32];                           // screening costs of the UV candidates
     int nTxb, nCoef, errFlag, nzFlag, eobOut, culOut;
     int am, lm, mode, delta, uvMode, uvDelta, nuv, bsize;
     uint32_t qItem;                          // persistent schedule: the superblock taken
@@ -807,16 +808,24 @@ __device__ void rc_chroma(lds<RcShared> &S, const Batch &B, const SearchParams &
     __syncthreads();
     if (S.nuv > 13) screen(13, 19);
     if (tid < GW) {   // the uvK candidates of smallest screening cost (first minimum each round), U and V tasks each
+        // taken at once: a candidate's place in that order is its rank under (cost, index), so the ones of rank < K
+        // (finite cost) are the rounds' picks and rank k fills tasks 2k (U) and 2k + 1 (V)
         const int l = tid, nuv = S.nuv, K = max(1, min(P.uvK, 6));
-        float js = l < nuv ? fmaf(lamS, uvcbits(S.uvc[l]), (float)S.satdI[l] * 0.25f) : INF;
-        int n = 0;
-        for (int k = 0; k < K; k++) {
-            const int b = gargmin(js, l);
-            if (b < 0) break;
-            if (l == b) js = INF;
-            if (l < 2) { S.taskC[n + l] = S.uvc[b]; S.taskT[n + l] = chroma_tx_type(txSz, cand_mode(S.uvc[b])); S.taskP[n + l] = 1 + l; }
-            n += 2;
+        const float js = l < nuv ? fmaf(lamS, uvcbits(S.uvc[l]), (float)S.satdI[l] * 0.25f) : INF;
+        S.uvJ[l] = js;
+        gsync();
+        int rank = 0, fin = 0;
+        for (int j = 0; j < nuv; j++) {
+            const float o = S.uvJ[j];
+            rank += (o < js || (o == js && j < l)) ? 1 : 0;
+            fin += o < INF ? 1 : 0;
         }
+        if (js < INF && rank < K) {
+            const int c = S.uvc[l], tt = chroma_tx_type(txSz, cand_mode(c));
+            S.taskC[2 * rank] = c; S.taskT[2 * rank] = tt; S.taskP[2 * rank] = 1;
+            S.taskC[2 * rank + 1] = c; S.taskT[2 * rank + 1] = tt; S.taskP[2 * rank + 1] = 2;
+        }
+        int n = 2 * min(K, fin);
         if (cflOk && (S.cflAlpha[0] || S.cflAlpha[1])) {
             if (l < 2) { S.taskC[n + l] = 255; S.taskT[n + l] = chroma_tx_type(txSz, UV_CFL_PRED); S.taskP[n + l] = 1 + l; }
             n += 2;
