@@ -562,25 +562,37 @@ __device__ void rc_luma(lds<RcShared> &S, const Batch &B, const SearchParams &P,
         rc_edges<Px>(S, I, 0, x, y, LW, LH, availL, availU, S.bdm[0][sbr4][sbc4 + (W >> 2) + 1], S.bdm[0][sbr4 + (H >> 2) + 1][sbc4],
                      mi_row, mi_col);
     if (gid() == 2) grp_txb_ctx(I, 0, txSz, mi_col, mi_row, bsize, tx0, ty0, &S.skipCtx[0], &S.dcCtx[0]);
-    if (tid == 64) {
-        S.am = av1t::Intra_Mode_Context[availU ? I.mY[(size_t)(mi_row - 1) * I.MiCols + mi_col] : DC_PRED];
-        S.lm = av1t::Intra_Mode_Context[availL ? I.mY[(size_t)mi_row * I.MiCols + mi_col - 1] : DC_PRED];
+    if (tid >= 128 && tid < 192) {   // wave 2: the tasks (candidate k, tx type t), k-major, types ascending
+        const int w = tid - 128;
         const BlockRes &br = B.blk[I.shOff[bsize] + (uint32_t)(y / (4 * bh4(bsize))) * I.shNbx[bsize] + x / (4 * bw4(bsize))];
         const int nc = max(1, min((int)br.ncand, P.Krecheck));
         const int set = tx_set_intra(txSz);
-        int n = 0;
-        for (int k = 0; k < nc; k++) S.candC[k] = br.cand[k];
-        S.nCand = nc;
+        if (w < nc) S.candC[w] = br.cand[w];
+        if (w == 0) {
+            S.am = av1t::Intra_Mode_Context[availU ? I.mY[(size_t)(mi_row - 1) * I.MiCols + mi_col] : DC_PRED];
+            S.lm = av1t::Intra_Mode_Context[availL ? I.mY[(size_t)mi_row * I.MiCols + mi_col - 1] : DC_PRED];
+            S.nCand = nc;
+            S.bsize = bsize;
+        }
         if (P.txAllCL && set != TX_SET_DCTONLY) {
-            for (int k = 0; k < nc; k++)
-                for (int t = 0; t < 16; t++)
-                    if (tx_in_set(set, t)) { S.taskC[n] = br.cand[k]; S.taskT[n] = t; S.taskP[n] = 0; S.taskK[n] = k; n++; }
-        } else {
+            // every pair at once: with nt types in the set, pair (k, t) is task k * nt + (types of the set below t)
+            uint32_t sm = 0;
+            for (int t = 0; t < 16; t++) sm |= tx_in_set(set, t) ? 1u << t : 0u;
+            const int nt = __popc(sm);
+            for (int i = w; i < nc * 16; i += 64) {
+                const int k = i >> 4, t = i & 15;
+                if ((sm >> t) & 1) {
+                    const int n = k * nt + __popc(sm & ((1u << t) - 1));
+                    S.taskC[n] = br.cand[k]; S.taskT[n] = t; S.taskP[n] = 0; S.taskK[n] = k;
+                }
+            }
+            if (w == 0) S.nTask = nc * nt;
+        } else if (w == 0) {
+            int n = 0;
             for (int k = 0; k < nc; k++) { S.taskC[n] = br.cand[k]; S.taskT[n] = DCT_DCT; S.taskP[n] = 0; S.taskK[n] = k; n++; }
             if (br.tx != DCT_DCT) { S.taskC[n] = br.cand[0]; S.taskT[n] = br.tx; S.taskP[n] = 0; S.taskK[n] = 0; n++; }
+            S.nTask = n;
         }
-        S.nTask = n;
-        S.bsize = bsize;
     }
     __syncthreads();
     if constexpr (W <= 16 && H <= 16) wg_luma_residuals<W, H>(S, I);
