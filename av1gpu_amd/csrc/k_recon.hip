@@ -418,14 +418,22 @@ __device__ void wg_write_pred(const lds<RcShared> &S, const lds<uint16_t> *gea, 
 __device__ int wg_record_txb(lds<RcShared> &S, const Batch &B, uint32_t sb, const Img &I, int p, int txSz, int txType, int x4, int y4,
                              const lds<int16_t> *q) {
     const int tid = threadIdx.x, n = tx_ncoef(txSz);
-    const uint16_t *scan = scan_of(txSz, txType);
     const uint32_t off = (uint32_t)S.nCoef;
     if (tid < 32) {
-        int last = 0;
-        for (int c = tid; c < n; c += 32) if (q[scan[c]]) last = c + 1;
+        // eob: 1 + the largest scan index of a nonzero level (inverse default scan; rows for class VERT, columns for
+        // HORIZ); cul: the sum of |level| before the eob, i.e. over every level (later ones are 0), capped at 63
+        const int cls = tx_class(txType), tw = tx_w(txSz) > 32 ? 32 : tx_w(txSz), th = tx_h(txSz) > 32 ? 32 : tx_h(txSz);
+        const lds<uint16_t> *isc = S.tab.iscan + dscan_off(av1t::Adjusted_Tx_Size[txSz]);
+        int last = 0, cul = 0;
+        for (int i = tid; i < n; i += 32) {
+            const int v = q[i];
+            if (v) {
+                const int c = cls == TX_CLASS_2D ? (int)isc[i] : cls == TX_CLASS_VERT ? i : (i % tw) * th + i / tw;
+                last = max(last, c + 1);
+                cul += abs(v);
+            }
+        }
         const int eob = gmaxi(last);
-        int cul = 0;
-        for (int c = tid; c < eob; c += 32) cul += abs((int)q[scan[c]]);
         cul = gsumi(cul);
         if (tid == 0) { S.eobOut = eob; S.culOut = min(cul, 63); }
     }
