@@ -673,7 +673,78 @@ __device__ void rc_chroma(lds<RcShared> &S, const Batch &B, const SearchParams &
         return b;
     };
     // ---- CfL: luma values, average, least-squares alpha, +-1 refinement by exact SSE ----
-    if (cflOk) {
+    if constexpr (W * H <= 64) if (cflOk) {
+        // up to 64 samples: wave 0 alone, one sample per lane, with 64-lane integer sums (the same integers as the
+        // workgroup form below) and no barrier until the result
+        if (tid < 64) {
+            const int lt = av1t::Max_Tx_Size_Rect[bsize], tw = tx_w(lt), th = tx_h(lt);
+            int maxLumaW = 0, maxLumaH = 0;
+            for (int ly = mi_row * 4; ly < mi_row * 4 + 4 * bh4(bsize); ly += th)
+                for (int lx = mi_col * 4; lx < mi_col * 4 + 4 * bw4(bsize); lx += tw)
+                    if (lx < I.MiCols * 4 && ly < I.MiRows * 4) { maxLumaW = lx + tw; maxLumaH = ly + th; }
+            const Px *luma = rec_plane<Px>(I, 0);
+            const int lp = I.pitch[0], i = tid >> LW, j = tid & (W - 1);
+            const bool in = tid < W * H;
+            int v = 0;
+            if (in) {
+                if (ss) {
+                    const int ly = min((cy + i) << 1, maxLumaH - 2), lx = min((cx + j) << 1, maxLumaW - 2);
+                    v = ((int)luma[(size_t)ly * lp + lx] + luma[(size_t)ly * lp + lx + 1] + luma[(size_t)(ly + 1) * lp + lx] + luma[(size_t)(ly + 1) * lp + lx + 1]) << 1;
+                } else {
+                    v = (int)luma[(size_t)min(cy + i, maxLumaH - 1) * lp + min(cx + j, maxLumaW - 1)] << 3;
+                }
+                S.cflL[tid] = v;
+            }
+            int sum = v;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+            const int avg = round2(sum, LW + LH);
+            const long long l = in ? (long long)(v - avg) : 0;
+            long long nU = in ? l * ((int)srcU[i * sp + j] - S.dcv[1]) : 0, nV = in ? l * ((int)srcV[i * sp + j] - S.dcv[2]) : 0, dd = l * l;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                nU += __shfl_xor(nU, o, 64);
+                nV += __shfl_xor(nV, o, 64);
+                dd += __shfl_xor(dd, o, 64);
+            }
+            int alpha[2];
+            for (int k = 0; k < 2; k++) {
+                const long long n = k == 0 ? nU : nV;
+                const int a = dd > 0 ? (int)rintf(64.f * (float)n / (float)dd) : 0;
+                alpha[k] = max(-16, min(16, a));
+            }
+            long long sse[6] = {0, 0, 0, 0, 0, 0};
+            if (in) {
+#pragma unroll
+                for (int pl = 0; pl < 2; pl++) {
+                    const int s = pl == 0 ? srcU[i * sp + j] : srcV[i * sp + j];
+#pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        const int a = max(-16, min(16, alpha[pl] + k - 1));
+                        const int d = s - clip3(0, (1 << bd) - 1, S.dcv[pl + 1] + round2signed(a * (int)l, 6));
+                        sse[pl * 3 + k] = (long long)d * d;
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 6; k++)
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) sse[k] += __shfl_xor(sse[k], o, 64);
+            if (tid == 0) {
+                S.cflAvg = avg;
+                for (int pl = 0; pl < 2; pl++) {
+                    int bk = 1;
+                    for (int k = 0; k < 3; k++) if (sse[pl * 3 + k] < sse[pl * 3 + bk]) bk = k;
+                    S.cflAlpha[pl] = max(-16, min(16, alpha[pl] + bk - 1));
+                }
+            }
+        }
+        if (tid < 32) S.satdI[tid] = 0;
+        __syncthreads();
+    } else if (tid == 0) {
+        S.cflAlpha[0] = S.cflAlpha[1] = 0;
+    }
+    if constexpr (W * H > 64) if (cflOk) {
         const int lt = av1t::Max_Tx_Size_Rect[bsize], tw = tx_w(lt), th = tx_h(lt);
         int maxLumaW = 0, maxLumaH = 0;
         for (int ly = mi_row * 4; ly < mi_row * 4 + 4 * bh4(bsize); ly += th)
