@@ -337,6 +337,138 @@ __device__ EvalOut grp_qrate(FP tmp, int ts, QP q, const E &e, const TT &T) {
     return o;
 }
 
+// ---- the same evaluation by half a group (16 lanes) for tx blocks of up to 64 coefficients ----
+// Lane l of the half holds virtual lanes l and l + 16 of the 32-lane form: their partial sums stay apart, and the sums
+// over the group add them first (the xor-16 step of gsum), then run the remaining butterfly steps across the 16 lanes,
+// so every result is the 32-lane form's to the bit.  The two halves of a group run two evaluations at once.
+__device__ __forceinline__ int hlane() { return threadIdx.x & 15; }
+__device__ __forceinline__ float hsum2(float a, float b) {
+    float v = a + b;
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 16);
+    return v;
+}
+__device__ __forceinline__ int hmaxi(int v) {
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 16));
+    return v;
+}
+
+template <int W, int H, typename FP, typename QP, typename E, typename TT>
+__device__ EvalOut half_qrate(FP tmp, int ts, QP q, const E &e, const TT &T) {
+    static_assert(W <= 16 && H <= 16 && W * H <= 64, "half-group evaluation: up to 64 coefficients");
+    constexpr int TW = W, TH = H, LTW = TW == 4 ? 2 : TW == 8 ? 3 : 4, N = TW * TH, TSZ = tx_of(W, H);
+    const int lane = hlane();
+    const int cls = tx_class(e.txType);
+    const int sk = cls == TX_CLASS_2D ? 0 : cls == TX_CLASS_HORIZ ? 1 : 2;
+    const auto iscan = iscan_ptr<TSZ>(T);
+    const int den = dq_denom(e.txSz), dsh = den == 4 ? 2 : den == 2 ? 1 : 0;
+    float dist[2] = {0.f, 0.f};
+    int last = 0;
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+        for (int idx = lane + 16 * h; idx < N; idx += GW) {
+            const float v = tmp[(idx >> LTW) * ts + (idx & (TW - 1))];
+            const int step = idx == 0 ? e.dcQ : e.acQ;
+            const float a = fabsf(v) * (float)den / (float)step;
+            int l = a < e.zbin ? 0 : (int)(a + e.rnd);
+            l = min(l, 16383);
+            const int qv = v < 0 ? -l : l;
+            q[idx] = (int16_t)qv;
+            if (l) {
+                const int c = sk == 0 ? (int)iscan[idx] : sk == 1 ? (idx & (TW - 1)) * TH + (idx >> LTW) : idx;
+                last = max(last, c + 1);
+            }
+            const float dq = l ? (float)dequant(qv, step, dsh, e.bd) : 0.f;
+            const float d = v - dq;
+            const float sq = d * d;
+            dist[h] = dist[h] + sq;
+        }
+    const float distS = hsum2(dist[0], dist[1]);
+    gsync();
+    const auto dscan = dscan_ptr<TSZ>(T);
+    const auto cbOff = cboff_ptr<TSZ>(T);
+    const int eob = hmaxi(last);
+    float b0 = lane == 0 ? base_bits<TSZ>(e, eob) : 0.f, b1 = 0.f;
+    if (sk == 0) {
+        for (int c = lane; c < eob; c += GW) b0 += coeff_bits<TSZ, TX_CLASS_2D>(c, eob, q, dscan, cbOff, e.plane > 0, e.dcCtx, *e.C);
+        for (int c = lane + 16; c < eob; c += GW) b1 += coeff_bits<TSZ, TX_CLASS_2D>(c, eob, q, dscan, cbOff, e.plane > 0, e.dcCtx, *e.C);
+    } else if (sk == 1) {
+        for (int c = lane; c < eob; c += GW) b0 += coeff_bits<TSZ, TX_CLASS_HORIZ>(c, eob, q, dscan, cbOff, e.plane > 0, e.dcCtx, *e.C);
+        for (int c = lane + 16; c < eob; c += GW) b1 += coeff_bits<TSZ, TX_CLASS_HORIZ>(c, eob, q, dscan, cbOff, e.plane > 0, e.dcCtx, *e.C);
+    } else {
+        for (int c = lane; c < eob; c += GW) b0 += coeff_bits<TSZ, TX_CLASS_VERT>(c, eob, q, dscan, cbOff, e.plane > 0, e.dcCtx, *e.C);
+        for (int c = lane + 16; c < eob; c += GW) b1 += coeff_bits<TSZ, TX_CLASS_VERT>(c, eob, q, dscan, cbOff, e.plane > 0, e.dcCtx, *e.C);
+    }
+    EvalOut o;
+    o.R = hsum2(b0, b1);
+    o.D = T.txg[e.txSz][e.txType] * distS;
+    o.eob = eob;
+    return o;
+}
+
+// grp_eval by half a group (W x H up to 64 coefficients): the same column / row passes, items spread over 16 lanes
+template <int W, int H, typename RP, typename FP, typename QP, typename E, typename TT>
+__device__ EvalOut half_eval(RP res, FP tmp, QP q, const E &e, const TT &T) {
+    constexpr int TW = W, TH = H, TS = W + 1;
+    constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : 4, LH = H == 4 ? 2 : H == 8 ? 3 : 4;
+    const int lane = hlane();
+    const int rk = row_kind(e.txType), ck = col_kind(e.txType), fud = flip_ud(e.txType), flr = flip_lr(e.txType);
+    {
+        constexpr int OC = TH < 8 ? TH : 8, NI = W * (TH / OC);
+        const auto *FcT = T.fwdT + T.fwdOff[ck][LH];
+        const auto *Fc = T.fwd + T.fwdOff[ck][LH];
+        for (int it = lane; it < NI; it += 16) {
+            const int j = it % W, i0 = (it / W) * OC, jj = flr ? W - 1 - j : j;
+            float acc[OC];
+            if (ck == K_IDT) {
+#pragma unroll
+                for (int o = 0; o < OC; o++) acc[o] = Fc[(i0 + o) * H + i0 + o] * (float)res[(fud ? H - 1 - (i0 + o) : i0 + o) * W + jj];
+            } else {
+#pragma unroll
+                for (int o = 0; o < OC; o++) acc[o] = 0.f;
+#pragma unroll 4
+                for (int k = 0; k < H; k++) {
+                    const float x = (float)res[(fud ? H - 1 - k : k) * W + jj];
+                    const auto *f = FcT + k * H + i0;
+#pragma unroll
+                    for (int o = 0; o < OC; o++) acc[o] = fmaf(f[o], x, acc[o]);
+                }
+            }
+#pragma unroll
+            for (int o = 0; o < OC; o++) tmp[(i0 + o) * TS + j] = acc[o];
+        }
+    }
+    gsync();
+    {
+        const auto *FrT = T.fwdT + T.fwdOff[rk][LW];
+        const auto *Fr = T.fwd + T.fwdOff[rk][LW];
+        const float scale = T.txScale[e.txSz];
+        if (lane < TH) {
+            float acc[TW];
+            auto *row = tmp + lane * TS;
+            if (rk == K_IDT) {
+#pragma unroll
+                for (int o = 0; o < TW; o++) acc[o] = Fr[o * W + o] * row[o];
+            } else {
+#pragma unroll
+                for (int o = 0; o < TW; o++) acc[o] = 0.f;
+#pragma unroll 2
+                for (int k = 0; k < W; k++) {
+                    const float x = row[k];
+                    const auto *f = FrT + k * W;
+#pragma unroll
+                    for (int o = 0; o < TW; o++) acc[o] = fmaf(f[o], x, acc[o]);
+                }
+            }
+#pragma unroll
+            for (int o = 0; o < TW; o++) row[o] = acc[o] * scale;
+        }
+    }
+    gsync();
+    return half_qrate<W, H>(tmp, TS, q, e, T);
+}
+
 // sum of r^2 over a W x H residual in raster order (32 virtual lanes)
 template <int W, int H, typename RP>
 __device__ float grp_res_energy(RP res) {

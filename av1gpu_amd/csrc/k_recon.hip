@@ -201,6 +201,19 @@ __device__ EvalOut grp_task(lds<RcShared> &S, lds<RcGroup> &X, const Img &I, int
     e.C = &S.cost;
     return grp_eval<W, H>(res, X.tmp, lev, e, S.tab);
 }
+// the same task by half the group (blocks of up to 64 coefficients), with tmp its own part of the group's scratch
+template <int W, int H>
+__device__ EvalOut half_task(lds<RcShared> &S, lds<float> *tmp, const Img &I, int p, int c, int t, const lds<int16_t> *res,
+                             lds<int16_t> *lev, const SearchParams &P) {
+    RcEvalIn e;
+    e.txSz = tx_of(W, H); e.txType = t; e.plane = p;
+    e.dcQ = dc_q(I.qindex, I.bd); e.acQ = ac_q(I.qindex, I.bd);
+    e.skipCtx = S.skipCtx[p]; e.dcCtx = S.dcCtx[p]; e.intraDir = p == 0 ? cand_mode(c) : 0; e.bd = I.bd;
+    e.rnd = p ? P.rndC : P.rnd; e.zbin = p ? P.zbinC : P.zbin;
+    e.resEnergy = 0.f;
+    e.C = &S.cost;
+    return half_eval<W, H>(res, tmp, lev, e, S.tab);
+}
 
 // 2-D forward DCT_DCT of a W x H residual with a side of 32 or 64 by the 32 lanes of the group, the residual produced
 // on the fly by res(i, j).  Column pass: lane = column (two columns, j and j + 32, for W = 64), one fmaf chain per kept
@@ -479,7 +492,23 @@ __device__ void wg_run_tasks(lds<RcShared> &S, const Img &I, const DevTables &T,
     constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
     const int g = gid(), lane = glane();
     const lds<CostTab> &C = S.cost;
-    for (int t = g; t < S.nTask; t += RC_NG) {
+    // luma blocks of up to 64 coefficients (residuals ready): 16 half groups, two tasks per group at once
+    bool done = false;
+    if constexpr (!big && W * H <= 64) if (lumaBits) {
+        done = true;
+        const int h = lane >> 4;
+        for (int t = 2 * g + h; t < S.nTask; t += 2 * RC_NG) {
+            const int c = S.taskC[t], tt = S.taskT[t];
+            const EvalOut o = half_task<W, H>(S, S.u.s.g[g].tmp + h * 136, I, 0, c, tt, S.u.s.res[S.taskK[t]], S.u.s.lev[t], P);
+            const int m = cand_mode(c);
+            float mb = C.y_mode[S.am][S.lm][m];
+            if (S.bsize >= BLOCK_8X8 && is_dir(m)) mb += C.angle_delta[m - V_PRED][cand_delta(c) + 3];
+            const float J = fmaf(lambda, o.R + mb, o.D);
+            if ((lane & 15) == 0) { S.taskJ[t] = J; S.taskE[t] = o.eob; }
+            gsync();
+        }
+    }
+    if (!done) for (int t = g; t < S.nTask; t += RC_NG) {
         const int c = S.taskC[t], tt = S.taskT[t], p = S.taskP[t];
         EvalOut o;
         if constexpr (big) {
