@@ -492,18 +492,30 @@ __device__ void wg_run_tasks(lds<RcShared> &S, const Img &I, const DevTables &T,
     constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
     const int g = gid(), lane = glane();
     const lds<CostTab> &C = S.cost;
-    // luma blocks of up to 64 coefficients (residuals ready): 16 half groups, two tasks per group at once
+    // blocks of up to 64 coefficients: 16 half groups, two tasks per group at once; luma residuals are ready, chroma
+    // tasks first make theirs (one task per group, as below), then the halves evaluate
     bool done = false;
-    if constexpr (!big && W * H <= 64) if (lumaBits) {
+    if constexpr (!big && W * H <= 64) {
         done = true;
+        if (!lumaBits) {
+            for (int t = g; t < S.nTask; t += RC_NG)
+                grp_residual(S, I.edgeFilter ? S.u.s.g[g].ea : nullptr, S.u.s.g[g].el, S.taskP[t], S.taskC[t], LW, LH, stage_src(S, S.taskP[t]), W,
+                             S.u.s.res[t], I.bd);
+            __syncthreads();
+        }
         const int h = lane >> 4;
         for (int t = 2 * g + h; t < S.nTask; t += 2 * RC_NG) {
-            const int c = S.taskC[t], tt = S.taskT[t];
-            const EvalOut o = half_task<W, H>(S, S.u.s.g[g].tmp + h * 136, I, 0, c, tt, S.u.s.res[S.taskK[t]], S.u.s.lev[t], P);
-            const int m = cand_mode(c);
-            float mb = C.y_mode[S.am][S.lm][m];
-            if (S.bsize >= BLOCK_8X8 && is_dir(m)) mb += C.angle_delta[m - V_PRED][cand_delta(c) + 3];
-            const float J = fmaf(lambda, o.R + mb, o.D);
+            const int c = S.taskC[t], tt = S.taskT[t], p = S.taskP[t];
+            const EvalOut o = half_task<W, H>(S, S.u.s.g[g].tmp + h * 136, I, p, c, tt, S.u.s.res[lumaBits ? S.taskK[t] : t], S.u.s.lev[t], P);
+            float J;
+            if (lumaBits) {
+                const int m = cand_mode(c);
+                float mb = C.y_mode[S.am][S.lm][m];
+                if (S.bsize >= BLOCK_8X8 && is_dir(m)) mb += C.angle_delta[m - V_PRED][cand_delta(c) + 3];
+                J = fmaf(lambda, o.R + mb, o.D);
+            } else {
+                J = fmaf(lambda, o.R, o.D);
+            }
             if ((lane & 15) == 0) { S.taskJ[t] = J; S.taskE[t] = o.eob; }
             gsync();
         }
