@@ -106,15 +106,31 @@ __global__ void __launch_bounds__(64) k_ln_conv(const bf16_t *in, const bf16_t *
 }
 
 // ---- fully connected head: logits[item][o] = b[o] + sum_f W[o][f] a4[item][f] (FP32, f ascending) ----
+// LN_FC_IT items per block share each weight read; their features sit in LDS (one fmaf chain per item and output)
+constexpr int LN_FC_IT = 16;
 __global__ void __launch_bounds__(128) k_ln_fc(const bf16_t *a4, const float *W, const float *b, float *logits, uint32_t n) {
-    const uint32_t item = blockIdx.x;
-    if (item >= n) return;
-    const bf16_t *f = a4 + (size_t)item * LN_FEAT;
+    __shared__ float f[LN_FC_IT][LN_FEAT];
+    const uint32_t i0 = blockIdx.x * LN_FC_IT;
+    if (i0 >= n) return;
+    const int ni = (int)min((uint32_t)LN_FC_IT, n - i0);
+    for (int e = threadIdx.x; e < LN_FC_IT * LN_FEAT; e += 128) {
+        const int i = e / LN_FEAT, k = e % LN_FEAT;
+        f[i][k] = i < ni ? from_bf16(a4[(size_t)(i0 + i) * LN_FEAT + k]) : 0.f;
+    }
+    __syncthreads();
     for (int o = threadIdx.x; o < LN_OUT; o += 128) {
         const float *w = W + (size_t)o * LN_FEAT;
-        float s = b[o];
-        for (int k = 0; k < LN_FEAT; k++) s = fmaf(w[k], from_bf16(f[k]), s);
-        logits[(size_t)item * LN_OUT + o] = s;
+        float s[LN_FC_IT];
+#pragma unroll
+        for (int i = 0; i < LN_FC_IT; i++) s[i] = b[o];
+        for (int k = 0; k < LN_FEAT; k++) {
+            const float wk = w[k];
+#pragma unroll
+            for (int i = 0; i < LN_FC_IT; i++) s[i] = fmaf(wk, f[i][k], s[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < LN_FC_IT; i++)
+            if (i < ni) logits[(size_t)(i0 + i) * LN_OUT + o] = s[i];
     }
 }
 
@@ -346,7 +362,8 @@ static void ln_forward(const LnNet &N, const LnActs &a, uint32_t n, hipStream_t 
     hipLaunchKernelGGL((k_ln_conv<32, 32>), dim3(n * 16), dim3(64), 0, st, a.a1, N.wb + ln_bf_off(1), w + LN_W_OFF[1] + 32 * LN_KP, a.a2, n);
     hipLaunchKernelGGL((k_ln_conv<32, 16>), dim3(n * 4), dim3(64), 0, st, a.a2, N.wb + ln_bf_off(2), w + LN_W_OFF[2] + 32 * LN_KP, a.a3, n);
     hipLaunchKernelGGL((k_ln_conv<32, 8>), dim3(n), dim3(64), 0, st, a.a3, N.wb + ln_bf_off(3), w + LN_W_OFF[3] + 32 * LN_KP, a.a4, n);
-    hipLaunchKernelGGL(k_ln_fc, dim3(n), dim3(128), 0, st, a.a4, w + LN_FC_OFF, w + LN_FC_OFF + (size_t)LN_OUT * LN_FEAT, a.logits, n);
+    hipLaunchKernelGGL(k_ln_fc, dim3((n + LN_FC_IT - 1) / LN_FC_IT), dim3(128), 0, st, a.a4, w + LN_FC_OFF, w + LN_FC_OFF + (size_t)LN_OUT * LN_FEAT,
+                       a.logits, n);
 }
 
 static void ln_refresh_bf16(LnNet &N, hipStream_t st) {
