@@ -599,16 +599,39 @@ __device__ __forceinline__ int hadamard4(int *d) {
     return sum;
 }
 
+// the value of lane (this lane xor M) within the 32-lane group, every lane active: M = 1, 2 by DPP quad permutes
+// ([1,0,3,2], [2,3,0,1]), M = 4, 8, 16 by ds_swizzle in bit-mask mode (and 0x1f, or 0, xor M)
+template <int M>
+__device__ __forceinline__ int lane_xor(int v) {
+    static_assert(M == 1 || M == 2 || M == 4 || M == 8 || M == 16, "lane_xor: one lane bit of the 32-lane group");
+    if constexpr (M == 1) return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+    else if constexpr (M == 2) return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
+    else return __builtin_amdgcn_ds_swizzle(v, 0x1F | (M << 10));
+}
+// sum over the SEG consecutive lanes of a segment (SEG a power of 2 up to 32), in every lane of it
+template <int SEG>
+__device__ __forceinline__ int seg_sumi(int v) {
+    if constexpr (SEG >= 32) v += lane_xor<16>(v);
+    if constexpr (SEG >= 16) v += lane_xor<8>(v);
+    if constexpr (SEG >= 8) v += lane_xor<4>(v);
+    if constexpr (SEG >= 4) v += lane_xor<2>(v);
+    if constexpr (SEG >= 2) v += lane_xor<1>(v);
+    return v;
+}
+
 // the same transform with one value of d per lane: lanes 16k .. 16k + 15 of the group hold one 4x4 block in raster
 // order (lane bits 0-1: column, 2-3: row).  The butterflies are those of hadamard4 (rows, then columns), so the 16
 // lanes return the absolute values of its 16 coefficients, whose sum is its result.  Every lane of the group calls it.
 __device__ __forceinline__ int hadamard4_lanes(int v) {
     const int l = glane();
-#pragma unroll
-    for (int o = 1; o < 16; o <<= 1) {
-        const int t = __shfl_xor(v, o, 32);
-        v = (l & o) ? t - v : v + t;
-    }
+    int t = lane_xor<1>(v);
+    v = (l & 1) ? t - v : v + t;
+    t = lane_xor<2>(v);
+    v = (l & 2) ? t - v : v + t;
+    t = lane_xor<4>(v);
+    v = (l & 4) ? t - v : v + t;
+    t = lane_xor<8>(v);
+    v = (l & 8) ? t - v : v + t;
     return abs(v);
 }
 

@@ -147,9 +147,7 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
                         const int p = m == DC_PRED ? dcv : pred_px(m, pa, sy, sx, LW, LH, A, L, 0, 0);
                         v = (int)src[sy * sp + sx] - p;
                     }
-                    v = hadamard4_lanes(v);
-#pragma unroll
-                    for (int o = SEG / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+                    v = seg_sumi<SEG>(hadamard4_lanes(v));
                     if (it < items && (it & (SEG - 1)) == 0) atomicAdd(&S.satdI[ci], v);
                 }
                 gsync();

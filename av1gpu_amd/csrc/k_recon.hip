@@ -892,9 +892,7 @@ __device__ void rc_chroma(lds<RcShared> &S, const Batch &B, const SearchParams &
                     const int pr = m == DC_PRED ? S.dcv[p] : pred_px(m, pa, sy, sx, LW, LH, A, L, 0, 0);
                     v = (int)s[sy * sp + sx] - pr;
                 }
-                v = hadamard4_lanes(v);
-#pragma unroll
-                for (int o = GW / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+                v = seg_sumi<GW>(hadamard4_lanes(v));
                 if (it < items && (it & (GW - 1)) == 0) atomicAdd(gen(&S.satdI[ci]), v);
             }
             __syncthreads();
