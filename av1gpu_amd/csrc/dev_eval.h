@@ -38,40 +38,56 @@ template <typename T> __device__ __forceinline__ cst<T> *uni(const T *p) {
 
 __device__ __forceinline__ int glane() { return threadIdx.x & 31; }
 __device__ __forceinline__ int gid() { return threadIdx.x >> 5; }
+// the value of lane (this lane xor M) within the 32-lane group, every lane active: M = 1, 2 by DPP quad permutes
+// ([1,0,3,2], [2,3,0,1]), M = 4, 8, 16 by ds_swizzle in bit-mask mode (and 0x1f, or 0, xor M)
+template <int M>
+__device__ __forceinline__ int lane_xor(int v) {
+    static_assert(M == 1 || M == 2 || M == 4 || M == 8 || M == 16, "lane_xor: one lane bit of the 32-lane group");
+    if constexpr (M == 1) return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+    else if constexpr (M == 2) return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
+    else return __builtin_amdgcn_ds_swizzle(v, 0x1F | (M << 10));
+}
+template <int M>
+__device__ __forceinline__ float lane_xorf(float v) { return __int_as_float(lane_xor<M>(__float_as_int(v))); }
+template <int M>
+__device__ __forceinline__ long long lane_xorl(long long v) {
+    const unsigned lo = (unsigned)lane_xor<M>((int)(unsigned)(v & 0xFFFFFFFFll)), hi = (unsigned)lane_xor<M>((int)(v >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
 // orders the group's LDS accesses (one wave executes both groups in lockstep; this stops compiler reordering)
 __device__ __forceinline__ void gsync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+// butterfly reductions over the 32 lanes of the group: partners lane xor 16, 8, 4, 2, 1 in that order, every lane ends
+// with the result
 __device__ __forceinline__ float gsum(float v) {
-#pragma unroll
-    for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+    v += lane_xorf<16>(v); v += lane_xorf<8>(v); v += lane_xorf<4>(v); v += lane_xorf<2>(v); v += lane_xorf<1>(v);
     return v;
 }
 __device__ __forceinline__ int gsumi(int v) {
-#pragma unroll
-    for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+    v += lane_xor<16>(v); v += lane_xor<8>(v); v += lane_xor<4>(v); v += lane_xor<2>(v); v += lane_xor<1>(v);
     return v;
 }
 __device__ __forceinline__ long long gsuml(long long v) {
-#pragma unroll
-    for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+    v += lane_xorl<16>(v); v += lane_xorl<8>(v); v += lane_xorl<4>(v); v += lane_xorl<2>(v); v += lane_xorl<1>(v);
     return v;
 }
 __device__ __forceinline__ int gmaxi(int v) {
-#pragma unroll
-    for (int o = 16; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 32));
+    v = max(v, lane_xor<16>(v)); v = max(v, lane_xor<8>(v)); v = max(v, lane_xor<4>(v)); v = max(v, lane_xor<2>(v));
+    v = max(v, lane_xor<1>(v));
     return v;
 }
 // index of the smallest (v, i) over the group, the lowest index among equal values; -1 if every v is INF
+template <int M>
+__device__ __forceinline__ void argmin_step(float &v, int &i) {
+    const float ov = lane_xorf<M>(v);
+    const int oi = lane_xor<M>(i);
+    if (ov < v || (ov == v && oi < i)) { v = ov; i = oi; }
+}
 __device__ __forceinline__ int gargmin(float v, int i) {
-#pragma unroll
-    for (int o = 16; o > 0; o >>= 1) {
-        const float ov = __shfl_xor(v, o, 32);
-        const int oi = __shfl_xor(i, o, 32);
-        if (ov < v || (ov == v && oi < i)) { v = ov; i = oi; }
-    }
+    argmin_step<16>(v, i); argmin_step<8>(v, i); argmin_step<4>(v, i); argmin_step<2>(v, i); argmin_step<1>(v, i);
     return v < INF ? i : -1;
 }
 
@@ -344,13 +360,11 @@ __device__ EvalOut grp_qrate(FP tmp, int ts, QP q, const E &e, const TT &T) {
 __device__ __forceinline__ int hlane() { return threadIdx.x & 15; }
 __device__ __forceinline__ float hsum2(float a, float b) {
     float v = a + b;
-#pragma unroll
-    for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 16);
+    v += lane_xorf<8>(v); v += lane_xorf<4>(v); v += lane_xorf<2>(v); v += lane_xorf<1>(v);
     return v;
 }
 __device__ __forceinline__ int hmaxi(int v) {
-#pragma unroll
-    for (int o = 8; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 16));
+    v = max(v, lane_xor<8>(v)); v = max(v, lane_xor<4>(v)); v = max(v, lane_xor<2>(v)); v = max(v, lane_xor<1>(v));
     return v;
 }
 
@@ -599,15 +613,6 @@ __device__ __forceinline__ int hadamard4(int *d) {
     return sum;
 }
 
-// the value of lane (this lane xor M) within the 32-lane group, every lane active: M = 1, 2 by DPP quad permutes
-// ([1,0,3,2], [2,3,0,1]), M = 4, 8, 16 by ds_swizzle in bit-mask mode (and 0x1f, or 0, xor M)
-template <int M>
-__device__ __forceinline__ int lane_xor(int v) {
-    static_assert(M == 1 || M == 2 || M == 4 || M == 8 || M == 16, "lane_xor: one lane bit of the 32-lane group");
-    if constexpr (M == 1) return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
-    else if constexpr (M == 2) return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
-    else return __builtin_amdgcn_ds_swizzle(v, 0x1F | (M << 10));
-}
 // sum over the SEG consecutive lanes of a segment (SEG a power of 2 up to 32), in every lane of it
 template <int SEG>
 __device__ __forceinline__ int seg_sumi(int v) {
