@@ -172,8 +172,7 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
                         }
                     v = hadamard4(d);
                 }
-#pragma unroll
-                for (int o = SEG / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+                v = seg_sumi<SEG>(v);
                 if (it < items && (it & (SEG - 1)) == 0) atomicAdd(&S.satdI[ci], v);
             }
             gsync();

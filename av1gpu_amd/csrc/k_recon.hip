@@ -919,8 +919,7 @@ __device__ void rc_chroma(lds<RcShared> &S, const Batch &B, const SearchParams &
                     }
                 v = hadamard4(d);
             }
-#pragma unroll
-            for (int o = SEG / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+            v = seg_sumi<SEG>(v);
             if (it < items && (it & (SEG - 1)) == 0) atomicAdd(gen(&S.satdI[ci]), v);
         }
         __syncthreads();
