@@ -129,6 +129,14 @@ void orc_eval_txb(const int16_t *res, int txSz, int txType, int plane, int qinde
     const EvalOut o = eval_txb(res, e, q, coef);
     out[0] = o.D; out[1] = o.R; out[2] = (float)o.eob;
 }
+// coefficient rate (bits) of levels q (row-major tw x th) of one tx block with the given contexts; *eob: its eob
+float orc_rate_levels(const int16_t *q, int txSz, int txType, int plane, int qindex, int skipCtx, int dcCtx, int intraDir, int *eob) {
+    EvalIn e;
+    e.txSz = txSz; e.txType = txType; e.plane = plane; e.dcQ = e.acQ = 1; e.rnd = e.zbin = 0.f;
+    e.skipCtx = skipCtx; e.dcCtx = dcCtx; e.intraDir = intraDir; e.bd = 8; e.resEnergy = 0.f;
+    e.C = &tables().cost[qindex <= 20 ? 0 : qindex <= 60 ? 1 : qindex <= 120 ? 2 : 3];
+    return rate_of_levels(q, e, eob);
+}
 // dequantize + inverse transform (no repair); res: h x w; returns the conformance flags
 uint32_t orc_dequant_inverse(const int16_t *q, int txSz, int txType, int dcQ, int acQ, int bd, int32_t *res) {
     int32_t dq[1024];

@@ -295,6 +295,19 @@ float res_energy(const int16_t *res, int n) {   // k_analysis.cuh:326-333 (raste
     return lane_sum(e);
 }
 
+float rate_of_levels(const int16_t *q, const EvalIn &e, int *eobOut) {   // dev_tx.cuh:301-316
+    const int t = e.txSz, n = std::min(txw(t), 32) * std::min(txh(t), 32);
+    const uint16_t *scan = get_scan(t, e.txType);
+    int eob = 0;
+    for (int c = 0; c < n; c++) if (q[scan[c]]) eob = c + 1;
+    float bits[32] = {0};
+    bits[0] = base_bits(e, eob);
+    const int cls = tx_class(e.txType);
+    for (int c = 0; c < eob; c++) bits[c & 31] += coeff_bits(c, eob, q, scan, t, cls, e.plane > 0, e.dcCtx, *e.C);
+    *eobOut = eob;
+    return lane_sum(bits);
+}
+
 EvalOut eval_txb(const int16_t *res, const EvalIn &e, int16_t *q, float *coefOut) {
     const Tables &T = tables();
     const int t = e.txSz, W = txw(t), H = txh(t), tw = W > 32 ? 32 : W, th = H > 32 ? 32 : H;
@@ -346,22 +359,14 @@ EvalOut eval_txb(const int16_t *res, const EvalIn &e, int16_t *q, float *coefOut
     }
     if (coefOut) memcpy(coefOut, coef, sizeof(float) * n);
     const float distS = lane_sum(dist);
-    const uint16_t *scan = get_scan(t, e.txType);
-    int eob = 0;
-    for (int c = 0; c < n; c++) if (q[scan[c]]) eob = c + 1;
-    float bits[32] = {0};
-    bits[0] = base_bits(e, eob);
-    const int cls = tx_class(e.txType);
-    for (int c = 0; c < eob; c++) bits[c & 31] += coeff_bits(c, eob, q, scan, t, cls, e.plane > 0, e.dcCtx, *e.C);
     EvalOut o;
-    o.R = lane_sum(bits);
+    o.R = rate_of_levels(q, e, &o.eob);
     const float g = T.txg[t][e.txType];
     o.D = g * distS;
     if (W == 64 || H == 64) {
         const float keptS = lane_sum(kept);
         o.D = o.D + fmaxf(0.f, fmaf(-g, keptS, e.resEnergy));
     }
-    o.eob = eob;
     return o;
 }
 

@@ -64,6 +64,8 @@ struct EvalIn {
 struct EvalOut { float D, R; int eob; };
 // forward transform + dead-zone quantizer + distortion + exact coefficient rate of one tx block (dev_tx.cuh:271-316)
 EvalOut eval_txb(const int16_t *res, const EvalIn &e, int16_t *q, float *coef = nullptr);
+// the rate part alone: coefficient bits of the levels q (row-major tw x th) and their eob (dev_tx.cuh:47-165, 301-316)
+float rate_of_levels(const int16_t *q, const EvalIn &e, int *eob);
 float res_energy(const int16_t *res, int n);
 
 struct BlockRes { float J; uint8_t cand[MAXK]; uint8_t ncand; uint8_t tx; };

@@ -56,11 +56,20 @@ class Oracle:
         L.orc_misc_tables.argtypes = [C.c_void_p] * 7
         L.orc_cost_table.argtypes = [C.c_int, C.c_void_p]
         L.orc_eval_txb.argtypes = [C.c_void_p] + [C.c_int] * 8 + [C.c_float, C.c_float] + [C.c_void_p] * 3
+        L.orc_rate_levels.restype = C.c_float
+        L.orc_rate_levels.argtypes = [C.c_void_p] + [C.c_int] * 7 + [C.POINTER(C.c_int)]
         L.orc_dequant_inverse.restype = C.c_uint32
         L.orc_dequant_inverse.argtypes = [C.c_void_p] + [C.c_int] * 5 + [C.c_void_p]
         L.orc_predict.argtypes = [C.c_int] * 11 + [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         L.orc_build_edges.argtypes = [C.c_void_p] + [C.c_int] * 11 + [C.c_void_p, C.c_void_p]
         L.orc_tile_layout.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 4
+
+    def rate_levels(self, q, tx_size, tx_type, plane, qindex, skip_ctx, dc_ctx, intra_dir):
+        """(R in bits, eob) of the levels q (int16, row-major) of one tx block under the oracle's rate model."""
+        q = np.ascontiguousarray(q, dtype=np.int16)
+        eob = C.c_int()
+        r = self.lib.orc_rate_levels(_ptr(q), tx_size, tx_type, plane, qindex, skip_ctx, dc_ctx, intra_dir, C.byref(eob))
+        return float(r), eob.value
 
     def params(self, **kw):
         p = OrcParams()
@@ -178,6 +187,59 @@ class Ref:
         L.ref_build_edges.argtypes = [C.c_void_p] + [C.c_int] * 11 + [C.c_void_p, C.c_void_p]
         L.av1g_parse.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.POINTER(Parsed))]
         L.av1g_parsed_free.argtypes = [C.POINTER(Parsed)]
+        L.ref_tile_symbols.restype = C.c_long
+        L.ref_tile_symbols.argtypes = [C.POINTER(DecisionRecord), C.c_void_p, C.c_size_t, C.c_char_p, C.c_size_t]
+        L.ref_default_cdfs.restype = C.c_size_t
+        L.ref_default_cdfs.argtypes = [C.c_int, C.c_void_p]
+        L.ref_coeff_cdf_fields.argtypes = [C.c_void_p]
+
+    COEFF_CDF_FIELDS = ("intra_tx_set1", "intra_tx_set2", "txb_skip", "eob_pt_16", "eob_pt_32", "eob_pt_64", "eob_pt_128",
+                        "eob_pt_256", "eob_pt_512", "eob_pt_1024", "eob_extra", "dc_sign", "coeff_base_eob", "coeff_base",
+                        "coeff_br")
+
+    def coeff_cdf_fields(self):
+        """{field: (cdf id of its first entry, size in uint16)} of the CDFs the coefficient syntax uses (cdfctx.h)."""
+        a = np.zeros(2 * len(self.COEFF_CDF_FIELDS), np.uint32)
+        assert self.lib.ref_coeff_cdf_fields(_ptr(a)) == len(self.COEFF_CDF_FIELDS)
+        return {f: (int(a[2 * i]), int(a[2 * i + 1])) for i, f in enumerate(self.COEFF_CDF_FIELDS)}
+
+    def default_cdfs(self, qindex):
+        """The uint16 CDF state a tile of a frame with base_q_idx = qindex starts from (av1g_cdfs_init)."""
+        n = self.lib.ref_default_cdfs(qindex, None)
+        a = np.zeros(n, np.uint16)
+        self.lib.ref_default_cdfs(qindex, _ptr(a))
+        return a
+
+    def layout(self, blocks, W, H, qindex, ssx=1, mono=0, bd=8):
+        """av1g_layout of a block list: (block_txb, txb) with coeff_off / x4 / y4 / plane / tx_size filled."""
+        blocks = np.ascontiguousarray(blocks)
+        r = DecisionRecord()
+        r.fp = self.frame_params(W, H, qindex, ssx, mono, bd)
+        r.n_blocks = len(blocks)
+        r.blocks = _ptr(blocks)
+        assert self.lib.av1g_layout(C.byref(r), 0, 0) != 0   # counts only
+        bt = np.zeros(len(blocks) + 1, np.uint32)
+        txb = np.zeros(r.n_txb, TXB_DTYPE)
+        r.block_txb, r.txb = _ptr(bt), _ptr(txb)
+        rc = self.lib.av1g_layout(C.byref(r), len(txb), r.n_coeffs)
+        if rc != 0:
+            raise AssertionError("av1g_layout rc %d" % rc)
+        return bt, txb, int(r.n_coeffs)
+
+    def symbols(self, rec, W, H, qindex, ssx=1, mono=0, bd=8, edge_filter=1):
+        """The reference writer's symbol stream of a record (all tiles, in order) as av1g_sym_pack words."""
+        arrs = [np.ascontiguousarray(rec[k]).copy() for k in ("blocks", "block_txb", "txb", "coeffs")]
+        r = DecisionRecord()
+        r.fp = self.frame_params(W, H, qindex, ssx, mono, bd, edge_filter)
+        r.n_blocks, r.n_txb, r.n_coeffs = len(arrs[0]), len(arrs[2]), len(arrs[3])
+        r.blocks, r.block_txb, r.txb, r.coeffs = (_ptr(a) for a in arrs)
+        err = C.create_string_buffer(256)
+        n = self.lib.ref_tile_symbols(C.byref(r), None, 0, err, 256)
+        if n < 0:
+            raise AssertionError("ref_tile_symbols: " + err.value.decode())
+        out = np.zeros(n, np.uint32)
+        self.lib.ref_tile_symbols(C.byref(r), _ptr(out), n, err, 256)
+        return out
 
     def parse(self, obus):
         """The reference's OBU parser (av1core/src/encoder.cpp:1683-1969): the record a stream codes, as a dict."""
