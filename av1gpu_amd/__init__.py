@@ -21,6 +21,10 @@ BLOCK_DTYPE = np.dtype([("mi_row", "<u2"), ("mi_col", "<u2"), ("bsize", "u1"), (
 TXB_DTYPE = np.dtype([("coeff_off", "<u4"), ("x4", "<u2"), ("y4", "<u2"), ("plane", "u1"), ("tx_size", "u1"),
                       ("tx_type", "u1"), ("flags", "u1"), ("eob", "<u2"), ("reserved", "<u2")])
 
+BLOCKRES_DTYPE = np.dtype([("J", "<f4"), ("cand", "u1", 6), ("ncand", "u1"), ("tx", "u1")])
+LEAF_DTYPE = np.dtype([("bsize", "u1"), ("x4", "u1"), ("y4", "u1"), ("pad", "u1")])
+SHAPES = [s for s in range(22) if not 13 <= s <= 15]   # the 19 block sizes the analysis evaluates
+
 _PARAM_FLOATS = ("lambdaMul", "satdLambda", "rnd", "zbin", "rndC", "zbinC")
 _PARAM_INTS = ("K", "Krecheck", "txSearch", "txSearchCL", "txAllCL", "deltas", "uvDeltas", "cfl", "uvK", "shapeMask",
                "partMask8", "partMaskN", "maxPart")
@@ -92,6 +96,13 @@ def lib():
         L.av1mi_buffer_free.argtypes = [C.c_void_p]
         L.av1mi_synth_image.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.POINTER(C.c_int)]
+        L.av1mi_set_search_export.argtypes = [C.c_void_p, C.c_int]
+        L.av1mi_blockres_export.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_size_t]
+        L.av1mi_blockres_export.restype = C.c_size_t
+        L.av1mi_sbj_export.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]
+        L.av1mi_sbj_export.restype = C.c_size_t
+        L.av1mi_leaves_export.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t]
+        L.av1mi_leaves_export.restype = C.c_size_t
         L.av1mi_fwd_matrix.argtypes = [C.c_int, C.c_int, C.c_void_p]
         L.av1mi_misc_tables.argtypes = [C.c_void_p] * 7
         L.av1mi_cost_table.argtypes = [C.c_int, C.c_void_p]
@@ -418,6 +429,28 @@ class GpEncoder:
                 lib().av1mi_frame_out_free(C.byref(outs[i]))
         times = GpTimes(**{k: getattr(t, k) for k, _ in _Times._fields_})
         return res, times
+
+    def set_search_export(self, on=True):
+        """Keep the search internals of each later encode (analysis records, superblock costs, leaves)."""
+        lib().av1mi_set_search_export(self._h, 1 if on else 0)
+
+    def search_export(self, img=0):
+        """The last encode's search internals of frame img: {"blockres": {shape: records}, "sbJ", "leaf_count",
+        "leaves" [nSb, 256]}."""
+        L, h = lib(), self._h
+        br = {}
+        for s in SHAPES:
+            n = int(L.av1mi_blockres_export(h, img, s, None, 0))
+            a = np.zeros(n, BLOCKRES_DTYPE)
+            L.av1mi_blockres_export(h, img, s, a.ctypes.data, n)
+            br[s] = a
+        nsb = int(L.av1mi_sbj_export(h, img, None, 0))
+        sbj = np.zeros(nsb, np.float32)
+        L.av1mi_sbj_export(h, img, sbj.ctypes.data, nsb)
+        cnt = np.zeros(nsb, np.uint16)
+        lv = np.zeros(nsb * 256, LEAF_DTYPE)
+        L.av1mi_leaves_export(h, img, cnt.ctypes.data, lv.ctypes.data, nsb)
+        return {"blockres": br, "sbJ": sbj, "leaf_count": cnt, "leaves": lv.reshape(nsb, 256)}
 
     def close(self):
         if self._h:

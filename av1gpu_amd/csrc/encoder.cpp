@@ -36,6 +36,7 @@ size_t ln_scratch_bytes(uint32_t n);
 int recon_prepare(int bd);
 size_t recon_smem_bytes();
 void launch_pack(const Batch &B, const PackOff *dOff, RecBlock *oB, uint32_t *oBT, RecTxb *oT, int16_t *oC, hipStream_t st);
+void launch_to10(const uint8_t *src, int sp, uint16_t *dst, int dp, int w, int h, int chroma, hipStream_t st);
 }  // namespace av1m
 
 using namespace av1m;
@@ -43,6 +44,7 @@ using namespace av1m;
 static_assert(sizeof(av1mi_block) == 16 && sizeof(av1mi_txb) == 16, "DecisionRecord entries are 16 bytes");
 static_assert(sizeof(RecBlock) == 16 && sizeof(RecTxb) == 16, "DecisionRecord entries are 16 bytes");
 static_assert(sizeof(av1mi_params) == sizeof(SearchParams), "av1mi_params mirrors SearchParams");
+static_assert(sizeof(av1mi_frame_params) == 88, "av1mi_frame_params has av1g_frame_params layout");
 
 namespace {
 
@@ -125,6 +127,12 @@ struct av1mi_encoder {
                                       // device, i.e. for the other encoders' work in a multi-stream server
     bool dpExport = false;            // keep the DP's per-node relative costs of the last call (training labels)
     std::vector<float> dpRel;
+    bool searchExport = false;        // keep the last call's analysis records, superblock costs and leaves (checks)
+    std::vector<Img> exImg;
+    std::vector<BlockRes> exBlk;
+    std::vector<float> exSbJ;
+    std::vector<uint16_t> exLeafCount;
+    std::vector<Leaf> exLeaves;
     hipStream_t aux[3] = {};          // analysis streams besides the caller's
     hipStream_t cap = nullptr;        // graph capture stream
     hipEvent_t evFork = nullptr, evJoin[3] = {};
@@ -324,6 +332,36 @@ size_t av1mi_dp_export(const av1mi_encoder *e, float *out, size_t n) {
     return e->dpRel.size();
 }
 
+int av1mi_set_search_export(av1mi_encoder *e, int on) {
+    drop_plan(e);
+    e->gen++;
+    e->searchExport = on != 0;
+    e->exImg.clear(); e->exBlk.clear(); e->exSbJ.clear(); e->exLeafCount.clear(); e->exLeaves.clear();
+    return 0;
+}
+size_t av1mi_blockres_export(const av1mi_encoder *e, int img, int shape, void *out, size_t n) {
+    if (img < 0 || img >= (int)e->exImg.size() || shape < 0 || shape >= NSHAPES) return 0;
+    const Img &I = e->exImg[img];
+    const size_t cnt = (size_t)I.shNbx[shape] * I.shNby[shape];
+    if (out) memcpy(out, e->exBlk.data() + I.shOff[shape], std::min(n, cnt) * sizeof(BlockRes));
+    return cnt;
+}
+size_t av1mi_sbj_export(const av1mi_encoder *e, int img, float *out, size_t n) {
+    if (img < 0 || img >= (int)e->exImg.size()) return 0;
+    const Img &I = e->exImg[img];
+    const size_t cnt = (size_t)I.sbCols * I.sbRows;
+    if (out) memcpy(out, e->exSbJ.data() + I.sbBase, std::min(n, cnt) * 4);
+    return cnt;
+}
+size_t av1mi_leaves_export(const av1mi_encoder *e, int img, uint16_t *counts, void *leaves, size_t nsb) {
+    if (img < 0 || img >= (int)e->exImg.size()) return 0;
+    const Img &I = e->exImg[img];
+    const size_t cnt = (size_t)I.sbCols * I.sbRows, k = std::min(nsb, cnt);
+    if (counts) memcpy(counts, e->exLeafCount.data() + I.sbBase, k * 2);
+    if (leaves) memcpy(leaves, e->exLeaves.data() + (size_t)I.sbBase * MAX_LEAVES, k * MAX_LEAVES * sizeof(Leaf));
+    return cnt;
+}
+
 void av1mi_destroy(av1mi_encoder *e) {
     if (!e) return;
     drop_plan(e);
@@ -341,8 +379,12 @@ void av1mi_destroy(av1mi_encoder *e) {
 
 void av1mi_get_config(const av1mi_encoder *e, av1mi_config *cfg) { *cfg = e->cfg; }
 
-int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_frame_out *out, av1mi_times *t, void *stream, int wantRecon,
-                 char *err, int errlen) {
+}  // extern "C"
+
+// GpEncoder::encode (pipeline.cpp:323-1181) for the plain intra path.  dev != nullptr: the packed records stay on the
+// device and pass to the caller (GpDevRecords); otherwise they are copied into each frame's host arrays.
+static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_frame_out *out, av1mi_times *t, void *stream,
+                       int wantRecon, av1mi_dev_records *dev, char *err, int errlen) {
     hipStream_t st = (hipStream_t)stream;
     av1mi_times tm;
     memset(&tm, 0, sizeof tm);
@@ -583,6 +625,14 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
         enc->dpRel.resize((size_t)nSb * 255);
         HCK(hipMemcpyAsync(enc->dpRel.data(), dDpRel, enc->dpRel.size() * 4, hipMemcpyDeviceToHost, st));
     }
+    if (enc->searchExport) {
+        enc->exImg = imgs;
+        enc->exBlk.resize(blkTotal); enc->exSbJ.resize(nSb); enc->exLeafCount.resize(nSb); enc->exLeaves.resize((size_t)nSb * MAX_LEAVES);
+        HCK(hipMemcpyAsync(enc->exBlk.data(), dBlk, blkTotal * sizeof(BlockRes), hipMemcpyDeviceToHost, st));
+        HCK(hipMemcpyAsync(enc->exSbJ.data(), dSbJ, (size_t)nSb * 4, hipMemcpyDeviceToHost, st));
+        HCK(hipMemcpyAsync(enc->exLeafCount.data(), dLeafCount, (size_t)nSb * 2, hipMemcpyDeviceToHost, st));
+        HCK(hipMemcpyAsync(enc->exLeaves.data(), dLeaves, (size_t)nSb * MAX_LEAVES * sizeof(Leaf), hipMemcpyDeviceToHost, st));
+    }
     HCK(hipStreamSynchronize(st));
     if (qctl[1]) {
         snprintf(err, errlen, "closed-loop queue: a superblock dependency wait timed out (schedule aborted)");
@@ -622,7 +672,7 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     size_t reconBytes = 0;
     if (wantRecon)
         for (int i = 0; i < n; i++) reconBytes += ((size_t)imgs[i].aw * imgs[i].ah + 2 * (size_t)(imgs[i].aw >> imgs[i].ss) * (imgs[i].ah >> imgs[i].ss)) * bps;
-    const size_t need = R.total + reconBytes;
+    const size_t need = (dev ? 0 : R.total) + reconBytes;
     if (need > enc->stageBytes) {
         HCK(hipStreamSynchronize(st));
         if (enc->stage) enc->oldStages.push_back(enc->stage);
@@ -633,10 +683,10 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
         enc->stageBytes = cap;
     }
     char *stg = (char *)enc->stage;
-    HCK(hipMemcpyAsync(stg, R.base, R.total, hipMemcpyDeviceToHost, st));
+    if (!dev) HCK(hipMemcpyAsync(stg, R.base, R.total, hipMemcpyDeviceToHost, st));
     auto hostOf = [&](const void *dptr) { return stg + ((const char *)dptr - (const char *)R.base); };
     std::vector<std::tuple<void *, const void *, size_t>> jobs;
-    size_t reconOff = R.total;
+    size_t reconOff = dev ? 0 : R.total;
     for (int i = 0; i < n; i++) {
         const Img &I = imgs[i];
         av1mi_frame_out &o = out[i];
@@ -644,15 +694,17 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
         o.n_blocks = (uint32_t)(imgBlk[i + 1] - imgBlk[i]);
         o.n_txb = (uint32_t)(imgTxb[i + 1] - imgTxb[i]);
         o.n_coeffs = imgCoef[i + 1] - imgCoef[i];
-        o.blocks = (av1mi_block *)malloc(std::max<size_t>(o.n_blocks, 1) * 16);
-        o.block_txb = (uint32_t *)malloc(((size_t)o.n_blocks + 1) * 4);
-        o.txb = (av1mi_txb *)malloc(std::max<size_t>(o.n_txb, 1) * 16);
-        o.coeffs = (int16_t *)malloc(std::max<uint64_t>(o.n_coeffs, 1) * 2);
         o.aw = I.aw; o.ah = I.ah; o.tile_cols = I.tileCols; o.tile_rows = I.tileRows;
-        jobs.emplace_back(o.blocks, hostOf(rB + imgBlk[i]), (size_t)o.n_blocks * 16);
-        jobs.emplace_back(o.block_txb, hostOf(rBT + imgBlk[i]), (size_t)o.n_blocks * 4);
-        jobs.emplace_back(o.txb, hostOf(rT + imgTxb[i]), (size_t)o.n_txb * 16);
-        jobs.emplace_back(o.coeffs, hostOf(rC + imgCoef[i]), (size_t)o.n_coeffs * 2);
+        if (!dev) {
+            o.blocks = (av1mi_block *)malloc(std::max<size_t>(o.n_blocks, 1) * 16);
+            o.block_txb = (uint32_t *)malloc(((size_t)o.n_blocks + 1) * 4);
+            o.txb = (av1mi_txb *)malloc(std::max<size_t>(o.n_txb, 1) * 16);
+            o.coeffs = (int16_t *)malloc(std::max<uint64_t>(o.n_coeffs, 1) * 2);
+            jobs.emplace_back(o.blocks, hostOf(rB + imgBlk[i]), (size_t)o.n_blocks * 16);
+            jobs.emplace_back(o.block_txb, hostOf(rBT + imgBlk[i]), (size_t)o.n_blocks * 4);
+            jobs.emplace_back(o.txb, hostOf(rT + imgTxb[i]), (size_t)o.n_txb * 16);
+            jobs.emplace_back(o.coeffs, hostOf(rC + imgCoef[i]), (size_t)o.n_coeffs * 2);
+        }
         if (wantRecon) {
             for (int p = 0; p < 3; p++) {
                 const int pw = p ? I.aw >> I.ss : I.aw, ph = p ? I.ah >> I.ss : I.ah;
@@ -668,8 +720,21 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     (void)hipEventRecord(enc->ev[7], st);
     HCK(hipStreamSynchronize(st));
     parallel_copy(jobs);
-    for (int i = 0; i < n; i++) out[i].block_txb[out[i].n_blocks] = out[i].n_txb;
-    R.release(st);
+    if (dev) {   // the packed arena passes to the caller as it is: frames back to back, frame-relative offsets
+        memset(dev, 0, sizeof *dev);
+        dev->n_frames = (uint32_t)n;
+        dev->blk0 = (uint32_t *)malloc(((size_t)n + 1) * 4);
+        dev->txb0 = (uint32_t *)malloc(((size_t)n + 1) * 4);
+        dev->coef0 = (uint64_t *)malloc(((size_t)n + 1) * 8);
+        for (int i = 0; i <= n; i++) { dev->blk0[i] = (uint32_t)imgBlk[i]; dev->txb0[i] = (uint32_t)imgTxb[i]; dev->coef0[i] = imgCoef[i]; }
+        dev->blocks = (av1mi_block *)rB; dev->block_txb_rel = rBT; dev->txb = (av1mi_txb *)rT; dev->coeffs = rC;
+        dev->pool = R.base;
+        R.base = nullptr;
+        R.owned = false;
+    } else {
+        for (int i = 0; i < n; i++) out[i].block_txb[out[i].n_blocks] = out[i].n_txb;
+        R.release(st);
+    }
     A.release(st);
     float ms[7];
     for (int k = 0; k < 7; k++) (void)hipEventElapsedTime(&ms[k], enc->ev[k], enc->ev[k + 1]);
@@ -683,6 +748,76 @@ int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_fram
     tm.blocks = stats[0]; tm.skipBlocks = stats[1]; tm.cflBlocks = stats[2]; tm.rangeFixes = stats[3];
     if (t) *t = tm;
     return 0;
+}
+
+extern "C" {
+
+int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_frame_out *out, av1mi_times *t, void *stream, int wantRecon,
+                 char *err, int errlen) {
+    return encode_impl(enc, in, n, out, t, stream, wantRecon, nullptr, err, errlen);
+}
+int av1mi_encode_dev(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_frame_out *out, av1mi_times *t, void *stream,
+                     int wantRecon, av1mi_dev_records *dev, char *err, int errlen) {
+    if (!dev) { snprintf(err, errlen, "av1mi_encode_dev: no device record holder"); return -1; }
+    memset(dev, 0, sizeof *dev);
+    return encode_impl(enc, in, n, out, t, stream, wantRecon, dev, err, errlen);
+}
+void av1mi_dev_records_free(av1mi_dev_records *dev, void *stream) {
+    if (!dev) return;
+    if (dev->pool) (void)hipFreeAsync(dev->pool, (hipStream_t)stream);
+    free(dev->blk0); free(dev->txb0); free(dev->coef0);
+    memset(dev, 0, sizeof *dev);
+}
+
+int av1mi_frames_to_10bit(av1mi_frame_in *f, int n, void **buf, void *stream) {
+    hipStream_t st = (hipStream_t)stream;
+    *buf = nullptr;
+    auto planeBytes = [](const av1mi_frame_in &x, int p) -> size_t {
+        const int s = (p && !x.yuv444) ? 1 : 0;
+        return ((size_t)(x.aw >> s) * (x.ah >> s) * 2 + 255) & ~(size_t)255;
+    };
+    size_t total = 0;
+    for (int i = 0; i < n; i++)
+        if (f[i].bitDepth != 10)
+            for (int p = 0; p < (f[i].mono ? 1 : 3); p++) total += planeBytes(f[i], p);
+    if (!total) return 0;
+    uint8_t *d = nullptr;
+    if (hipMallocAsync((void **)&d, total, st) != hipSuccess) return -1;
+    size_t at = 0;
+    for (int i = 0; i < n; i++) {
+        av1mi_frame_in &x = f[i];
+        if (x.bitDepth == 10) continue;
+        for (int p = 0; p < (x.mono ? 1 : 3); p++) {
+            const int s = (p && !x.yuv444) ? 1 : 0, w = x.aw >> s, h = x.ah >> s;
+            uint16_t *o = (uint16_t *)(d + at);
+            launch_to10((const uint8_t *)x.dsrc[p], x.spitch[p], o, w, w, h, p > 0, st);
+            x.dsrc[p] = o;
+            x.spitch[p] = w;
+            at += planeBytes(x, p);
+        }
+        x.bitDepth = 10;
+    }
+    *buf = d;
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+void av1mi_frame_params_of(const av1mi_frame_out *o, int edgeFilter, int colorMatrix, av1mi_frame_params *fp) {
+    memset(fp, 0, sizeof *fp);
+    fp->version = 5;
+    fp->width = o->width; fp->height = o->height;
+    fp->monochrome = (uint8_t)o->monochrome;
+    fp->chroma_format = (uint8_t)(o->monochrome ? 0 : o->chroma_format);
+    fp->bit_depth = (uint8_t)o->bit_depth;
+    fp->base_q_idx = (uint8_t)o->base_q_idx;
+    fp->tile_cols_log2 = fp->tile_rows_log2 = -1;
+    fp->enable_intra_edge_filter = (uint8_t)(edgeFilter ? 1 : 0);
+    if (colorMatrix >= 0) {
+        fp->color_description = 1; fp->color_primaries = 1; fp->transfer_characteristics = 13;
+        fp->matrix_coefficients = (uint8_t)colorMatrix;
+    } else if (colorMatrix == -2) {
+        fp->color_description = 1;
+        fp->color_primaries = fp->transfer_characteristics = fp->matrix_coefficients = 2;
+    }
 }
 
 void av1mi_frame_out_free(av1mi_frame_out *o) {

@@ -39,4 +39,34 @@ void launch_zero(void *p, size_t bytes, hipStream_t st) {
     hipLaunchKernelGGL(k_zero, dim3((unsigned)grid), dim3(256), 0, st, (uint4 *)p, n);
 }
 
+// 8-bit planes to 10-bit ones (the reference's gp_frames_to_10bit mapping, pipeline.h:218-225, libavif's full range):
+// Y10 = round(Y8 * 1023 / 255), C10 = clamp(round((C8 - 128) * 1023 / 255) + 512, 0, 1023); no ties occur (1023 v / 255
+// is never an odd multiple of 1/2).  Four samples per thread along a row, rows on the grid's y dimension.
+__global__ void __launch_bounds__(256) k_to10(const uint8_t *src, int sp, uint16_t *dst, int dp, int w, int h, int chroma) {
+    const int x0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+    for (int y = blockIdx.y; y < h; y += gridDim.y) {
+        const uint8_t *s = src + (size_t)y * sp;
+        uint16_t *d = dst + (size_t)y * dp;
+        for (int k = 0; k < 4; k++) {
+            const int x = x0 + k;
+            if (x >= w) break;
+            const int v = s[x];
+            int o;
+            if (chroma) {
+                const int dv = v - 128, m = ((dv < 0 ? -dv : dv) * 2046 + 255) / 510;
+                o = 512 + (dv < 0 ? -m : m);
+                o = o < 0 ? 0 : o > 1023 ? 1023 : o;
+            } else {
+                o = (v * 2046 + 255) / 510;
+            }
+            d[x] = (uint16_t)o;
+        }
+    }
+}
+void launch_to10(const uint8_t *src, int sp, uint16_t *dst, int dp, int w, int h, int chroma, hipStream_t st) {
+    if (w <= 0 || h <= 0) return;
+    hipLaunchKernelGGL(k_to10, dim3((unsigned)((w + 1023) / 1024), (unsigned)std::min(h, 4096)), dim3(256), 0, st, src, sp, dst, dp, w,
+                       h, chroma);
+}
+
 }  // namespace av1m

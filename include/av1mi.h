@@ -3,8 +3,9 @@
  * The drop-in boundary for the reference's GPU intra RD path.  The entry points mirror the reference's C++ encoder API
  * (maddog107/av1gpu encoder/src/pipeline.h:191-216: GpEncoder::init / encode / config, gp_check_partitions) and its
  * launch layer (encoder/src/kernels.h:11-82), flattened to plain C: an opaque encoder handle, POD structs, plain
- * pointers and sizes, a hipStream_t passed as void *.  Host C++ (include/av1mi_encoder.hpp) and Python
- * (av1gpu_amd/__init__.py) sit on top of these functions only.
+ * pointers and sizes, a hipStream_t passed as void *.  Host C++ (include/av1mi_gp.hpp: the reference's GpEncoder /
+ * GpConfig / GpParams / GpFrameIn / GpFrameOut / GpTimes / GpDevRecords, implemented in av1gpu_amd/csrc/gp_encoder.cpp)
+ * and Python (av1gpu_amd/__init__.py) sit on top of these functions only.
  *
  * Output: for every frame, the DecisionRecord of the reference's av1core (av1core/include/av1g/decisions.h:225-322:
  * 16-byte av1g_block / av1g_txb entries in decode order, row-major int16 levels) and the exact reconstruction.
@@ -37,6 +38,27 @@ typedef struct av1mi_txb {
     uint8_t plane, tx_size, tx_type, flags;
     uint16_t eob, reserved;
 } av1mi_txb;
+
+/* Frame-level decisions of a record: av1g_frame_params (decisions.h:269-313), the same 88-byte layout. */
+typedef struct av1mi_frame_params {
+    uint32_t version;
+    uint32_t width, height;
+    uint8_t monochrome, base_q_idx, tx_mode_select, reduced_tx_set;
+    int8_t tile_cols_log2, tile_rows_log2;
+    int8_t delta_q_y_dc, delta_q_u_dc, delta_q_u_ac, delta_q_v_dc, delta_q_v_ac;
+    uint8_t enable_filter_intra, enable_intra_edge_filter;
+    uint8_t delta_q_present, delta_q_res;
+    uint8_t using_qmatrix, qm_y, qm_u, qm_v;
+    uint8_t loop_filter_level[4];
+    uint8_t loop_filter_sharpness;
+    uint8_t enable_cdef, cdef_damping_minus_3, cdef_bits;
+    uint8_t cdef_y_pri[8], cdef_y_sec[8], cdef_uv_pri[8], cdef_uv_sec[8];
+    uint8_t allow_screen_content_tools;
+    uint8_t chroma_format;
+    uint8_t bit_depth;
+    uint8_t color_description, color_primaries, transfer_characteristics, matrix_coefficients;
+    uint8_t reserved[9];
+} av1mi_frame_params;
 
 /* Search knobs: the reference's GpParams (encoder/src/gp.h:29-68) fields this core implements, same names and meaning.
  * av1mi_default_params() gives the plain (maximum-effort) search: K = Krecheck = 6, every tx type in the closed loop,
@@ -120,6 +142,36 @@ void av1mi_get_config(const av1mi_encoder *enc, av1mi_config *cfg);
 int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_frame_out *out, av1mi_times *t, void *stream,
                  int want_recon, char *err, int errlen);
 void av1mi_frame_out_free(av1mi_frame_out *out);
+/* The frame parameters of an encoded frame as the reference sets them (pipeline.cpp:938-1007): version 5, sizes, bit
+ * depth, chroma format, base_q_idx, TX_MODE_LARGEST, minimum-legal tiles, enable_intra_edge_filter; colorMatrix >= 0:
+ * BT.709 primaries, sRGB transfer and that matrix; -2: unspecified (2/2/2); -1: the default colour description. */
+void av1mi_frame_params_of(const av1mi_frame_out *out, int edge_filter, int color_matrix, av1mi_frame_params *fp);
+
+/* Device-resident records (GpDevRecords, encoder/src/entropy_dev.h:13-34; GpEncoder::encode with dev != nullptr,
+ * pipeline.cpp:1148-1168): the batch's records stay in device memory, frames back to back in decode order, one
+ * allocation (pool) the caller owns until av1mi_dev_records_free.  blk0 / txb0 / coef0: host arrays of n + 1 frame bases;
+ * block_txb_rel: each block's first tx block relative to its frame (no trailing entry); txb.coeff_off frame-relative. */
+typedef struct av1mi_dev_records {
+    uint32_t n_frames;
+    uint32_t *blk0, *txb0;
+    uint64_t *coef0;
+    av1mi_block *blocks;
+    uint32_t *block_txb_rel;
+    av1mi_txb *txb;
+    int16_t *coeffs;
+    void *pool;
+} av1mi_dev_records;
+/* av1mi_encode with the records left on the device (dev): out[i] then carries the frame's sizes, counts and (with
+ * want_recon) the reconstruction, and its record pointers are NULL. */
+int av1mi_encode_dev(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_frame_out *out, av1mi_times *t, void *stream,
+                     int want_recon, av1mi_dev_records *dev, char *err, int errlen);
+/* gp_dev_records_free (entropy_dev.h:43): releases dev's device allocation on stream and its host arrays. */
+void av1mi_dev_records_free(av1mi_dev_records *dev, void *stream);
+/* gp_frames_to_10bit (pipeline.h:218-225): the 8-bit frames of f become 10-bit device planes in place (Y10 =
+ * round(Y8 * 1023 / 255), C10 = round((C8 - 128) * 1023 / 255) + 512), all in one stream-ordered allocation returned in
+ * *buf (NULL if no frame was converted; free it with av1mi_dev_free_async after the encode).  The source planes must
+ * stay valid until the work on stream is done. */
+int av1mi_frames_to_10bit(av1mi_frame_in *f, int n, void **buf, void *stream);
 
 /* Learned candidate pruning (no counterpart in the reference, whose pruning stages are heuristics:
  * encoder/src/k_leafprep.cu, an_stage.h, gp.h:136-143).  A BF16 CNN per 64x64 superblock predicts, per square
@@ -137,6 +189,16 @@ size_t av1mi_ln_param_count(void);
  * av1mi_dp_export copies up to n floats of the last call and returns how many there are. */
 int av1mi_set_dp_export(av1mi_encoder *enc, int on);
 size_t av1mi_dp_export(const av1mi_encoder *enc, float *out, size_t n);
+/* Search internals of the last encode, for parity checks (the reference keeps them in device memory only: GpBlockRes,
+ * GpLeaf and the per-SB cost of gp.h:112-132).  With on != 0 each later av1mi_encode keeps them:
+ * av1mi_blockres_export: frame img's analysis records of one shape (block size), raster over the shape's grid, 12-byte
+ * {float J; uint8 cand[6], ncand, tx} entries; av1mi_sbj_export: the partition DP's cost of each superblock (raster);
+ * av1mi_leaves_export: per superblock the leaf count and 256 4-byte {bsize, x4, y4, pad} leaves in decode order.  Each
+ * copies up to n (superblocks for the leaves) and returns how many there are. */
+int av1mi_set_search_export(av1mi_encoder *enc, int on);
+size_t av1mi_blockres_export(const av1mi_encoder *enc, int img, int shape, void *out, size_t n);
+size_t av1mi_sbj_export(const av1mi_encoder *enc, int img, float *out, size_t n);
+size_t av1mi_leaves_export(const av1mi_encoder *enc, int img, uint16_t *counts, void *leaves, size_t nsb);
 void av1mi_ln_init_params(float *params, uint64_t seed);
 /* Training on the device with this library's own forward (MFMA) and backward kernels. */
 typedef struct av1mi_learner av1mi_learner;

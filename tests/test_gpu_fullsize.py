@@ -24,6 +24,7 @@ import av1gpu_amd as A
 import refcheck
 from av1gpu_amd import learn, serve
 from test_gate_image import gate_planes
+from test_gpu_parity import same_search
 
 pytestmark = pytest.mark.gpu
 
@@ -62,15 +63,21 @@ def enc():
     e.close()
 
 
-def test_gpu_4000x3000_equals_oracle(enc, oracle):
+def test_gpu_4000x3000_equals_oracle(oracle):
+    """Every record array, the reconstruction, and the search internals (analysis records, superblock costs, leaves)."""
     W, H = 4000, 3000
     print("4000x3000 seed 1000 qindex 120, oracle on %d threads" % THREADS, flush=True)
     planes = A.synth_image(1000, W, H)
-    g, t = enc.encode([A.DeviceFrame(planes, W, H, qindex=120)])
+    e = A.GpEncoder().init(A.GpConfig(qindex=120))
+    e.set_search_export(True)
+    g, t = e.encode([A.DeviceFrame(planes, W, H, qindex=120)])
+    ex = e.search_export(0)
+    e.close()
     g = g[0]
     assert g.tile_rows == 2 and g.tile_cols == 1 and t.sbs == 63 * 47 and t.steps == 109
-    r = oracle.encode(planes, W, H, qindex=120, threads=THREADS)
+    r = oracle.encode(planes, W, H, qindex=120, threads=THREADS, detail=True)
     _same_oracle(g, r)
+    same_search(ex, r)
 
 
 @pytest.mark.parametrize("W,H,seed,cols", [(4160, 192, 1101, 2), (8200, 128, 1102, 4), (6000, 1200, 1103, 2)])

@@ -1,7 +1,9 @@
 """GPU path (HIP kernels through the C ABI) against the oracle, bit for bit: every record array and the whole padded
 reconstruction, over sizes (partial superblocks, 8x8, two-tile frames), qindex values, 8 and 10 bits, non-default
 search knobs, and batches mixing sizes and qindex (each frame's result independent of the batch).  At full size the
-GPU record is checked by the reference's CPU backend: its decoder-exact reconstruction equals the GPU's."""
+GPU record is checked by the reference's CPU backend: its decoder-exact reconstruction equals the GPU's.  The search
+internals -- every analysis record (J, candidates, tx type), every superblock's DP cost and its leaves -- equal the
+oracle's bit for bit, so the decisions are made at the same RD costs."""
 import numpy as np
 import pytest
 
@@ -120,3 +122,35 @@ def test_gpu_schedules_agree(enc):
             for p in range(3):
                 assert np.array_equal(a.recon[p], b.recon[p])
         e.close()
+
+
+def same_search(g, r):
+    """The GPU's search internals (GpEncoder.search_export) equal the oracle's (detail=True), bit for bit."""
+    for s, a in g["blockres"].items():
+        b = r["blockres"][s]
+        assert len(a) == len(b), s
+        assert np.array_equal(a["J"].view(np.uint32), b["J"].view(np.uint32)), ("J", s)
+        assert np.array_equal(a["ncand"], b["ncand"]) and np.array_equal(a["tx"], b["tx"]), ("ncand/tx", s)
+        idx = np.arange(6)[None, :] < a["ncand"][:, None]
+        assert np.array_equal(np.where(idx, a["cand"], 0), np.where(idx, b["cand"], 0)), ("cand", s)
+    assert np.array_equal(g["sbJ"].view(np.uint32), r["sbJ"].view(np.uint32)), "sbJ"
+    assert np.array_equal(g["leaf_count"], r["leaf_count"]), "leaf counts"
+    for k in range(len(g["leaf_count"])):
+        n = int(g["leaf_count"][k])
+        for f in ("bsize", "x4", "y4"):
+            assert np.array_equal(g["leaves"][k][:n][f], r["leaves"][k][:n][f]), ("leaves", k)
+
+
+@pytest.mark.parametrize("W,H,seed,q,bd", [(64, 64, 1, 120, 8), (200, 136, 3, 160, 8), (130, 18, 8, 30, 8), (136, 72, 50, 120, 10),
+                                           (264, 200, 6, 80, 8)])
+def test_gpu_search_costs_equal_oracle(oracle, W, H, seed, q, bd):
+    print("search export", W, H, seed, q, bd, flush=True)
+    e = A.GpEncoder().init(A.GpConfig(qindex=q))
+    e.set_search_export(True)
+    planes = A.synth_image(1000 + seed, W, H, bd=bd)
+    g = e.encode([A.DeviceFrame(planes, W, H, bd=bd, qindex=q)])[0][0]
+    ex = e.search_export(0)
+    e.close()
+    r = oracle.encode(planes, W, H, bd=bd, qindex=q, threads=8, detail=True)
+    _same(g, r)
+    same_search(ex, r)
