@@ -39,11 +39,37 @@ constexpr int SB_TXB_CAP_444 = 768;      // 4:4:4: 3 x 256
 constexpr int SB_COEF_CAP_444 = 12288;   // 4:4:4: 3 x 64 x 64
 constexpr int FWD_FLOATS = (16 + 64 + 256 + 1024 + 4096) + (16 + 64 + 256) + (16 + 64 + 256 + 1024);
 
+// Geometry of the ten AV1 partitions of an n x n node (spec 5.11.4 decode_partition): up to four parts in decode order,
+// each the partition's sub-size (split = 0) or the node's split square (split = 1), placed at (dx, dy) quarters of n.
+// Shared by the partition DP (device) and the partition feasibility check (host).
+struct PartPlace { uint8_t split, dx, dy; };
+struct PartShape { uint8_t parts; PartPlace at[4]; };
+inline constexpr PartShape Part_Shape[10] = {
+    {1, {{0, 0, 0}}},                                      // NONE
+    {2, {{0, 0, 0}, {0, 0, 2}}},                           // HORZ
+    {2, {{0, 0, 0}, {0, 2, 0}}},                           // VERT
+    {4, {{1, 0, 0}, {1, 2, 0}, {1, 0, 2}, {1, 2, 2}}},     // SPLIT
+    {3, {{1, 0, 0}, {1, 2, 0}, {0, 0, 2}}},                // HORZ_A
+    {3, {{0, 0, 0}, {1, 0, 2}, {1, 2, 2}}},                // HORZ_B
+    {3, {{1, 0, 0}, {1, 0, 2}, {0, 2, 0}}},                // VERT_A
+    {3, {{0, 0, 0}, {1, 2, 0}, {1, 2, 2}}},                // VERT_B
+    {4, {{0, 0, 0}, {0, 0, 1}, {0, 0, 2}, {0, 0, 3}}},     // HORZ_4
+    {4, {{0, 0, 0}, {0, 1, 0}, {0, 2, 0}, {0, 3, 0}}},     // VERT_4
+};
+// the order in which the DP weighs the partitions of a node inside the frame (the first of equal costs is kept)
+inline constexpr uint8_t Part_Order[10] = {PARTITION_SPLIT, PARTITION_NONE, PARTITION_HORZ, PARTITION_VERT, PARTITION_HORZ_A,
+                                           PARTITION_HORZ_B, PARTITION_VERT_A, PARTITION_VERT_B, PARTITION_HORZ_4, PARTITION_VERT_4};
+
 AV1M_HD int clip3(int lo, int hi, int x) { return x < lo ? lo : x > hi ? hi : x; }
 AV1M_HD int round2(int x, int n) { return n == 0 ? x : (x + (1 << (n - 1))) >> n; }
 AV1M_HD int64_t round2l(int64_t x, int n) { return n == 0 ? x : (x + ((int64_t)1 << (n - 1))) >> n; }
 AV1M_HD int round2signed(int x, int n) { return x >= 0 ? round2(x, n) : -round2(-x, n); }
 AV1M_HD int bw4(int bs) { return av1t::Num_4x4_Blocks_Wide[bs]; }
+AV1M_HD int square_bsize(int n) { return n == 8 ? BLOCK_8X8 : n == 16 ? BLOCK_16X16 : n == 32 ? BLOCK_32X32 : BLOCK_64X64; }
+// block size of part k of partition p of an n x n node (n >= 8)
+AV1M_HD int part_bsize(int p, int k, int n) {
+    return av1t::Partition_Subsize[Part_Shape[p].at[k].split ? PARTITION_SPLIT : p][square_bsize(n)];
+}
 AV1M_HD int bh4(int bs) { return av1t::Num_4x4_Blocks_High[bs]; }
 AV1M_HD int tx_w(int t) { return av1t::Tx_Width[t]; }
 AV1M_HD int tx_h(int t) { return av1t::Tx_Height[t]; }

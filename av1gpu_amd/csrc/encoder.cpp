@@ -189,70 +189,43 @@ void av1mi_default_params(av1mi_params *p) {
     p->uvOverheadBits = 1.5f; p->splitBias = 1.f; p->lambdaCMul = 1.f;
 }
 
+// Whether an n x n node at (x, y) of a superblock whose frame part is FW x FH can be partitioned into blocks the analysis
+// evaluates, under the partitions the DP may choose there (the same edge rules and options as k_dp): recursively over
+// Part_Shape, a partition being usable when each of its in-frame parts is an evaluated shape or a partitionable node.
+static bool partitionable(const av1mi_params &P, int n, int x, int y, int FW, int FH) {
+    if (x >= FW || y >= FH) return true;
+    const int q = n >> 2;
+    const bool rows = y + 2 * q < FH, cols = x + 2 * q < FW, big = n > P.maxPart;
+    uint32_t allowed;
+    if (rows && cols) allowed = big ? (1u << PARTITION_SPLIT) : ((uint32_t)(n == 8 ? P.partMask8 : P.partMaskN) | (1u << PARTITION_SPLIT));
+    else if (cols) allowed = (1u << PARTITION_SPLIT) | (big ? 0u : 1u << PARTITION_HORZ);
+    else if (rows) allowed = (1u << PARTITION_SPLIT) | (big ? 0u : 1u << PARTITION_VERT);
+    else allowed = 1u << PARTITION_SPLIT;
+    if (n == 8) allowed &= 0xFu;
+    auto evaluated = [&](int bs) { return bs < BLOCK_INVALID && !(bs >= 13 && bs <= 15) && ((P.shapeMask >> bs) & 1); };
+    for (int p = 0; p < 10; p++) {
+        if (!((allowed >> p) & 1)) continue;
+        bool ok = true;
+        for (int k = 0; k < Part_Shape[p].parts && ok; k++) {
+            const int px = x + Part_Shape[p].at[k].dx * q, py = y + Part_Shape[p].at[k].dy * q;
+            if (px >= FW || py >= FH) continue;
+            ok = (p == PARTITION_SPLIT && n > 8) ? partitionable(P, n >> 1, px, py, FW, FH) : evaluated(part_bsize(p, k, n));
+        }
+        if (ok) return true;
+    }
+    return false;
+}
+
 int av1mi_check_partitions(const av1mi_params *p, char *err, int errlen) {
-    // every superblock at every distance from the frame edges keeps a partition of finite J (pipeline.cpp:136-205)
+    // every superblock, whatever part of it the frame covers (8 .. 64 samples each way), keeps a partition of finite
+    // cost (gp_check_partitions, pipeline.cpp:136-219)
     for (int FH = 8; FH <= 64; FH += 8)
-        for (int FW = 8; FW <= 64; FW += 8) {
-            auto blk = [&](int bs, int x, int y) {
-                return bs < BLOCK_INVALID && !(bs >= 13 && bs <= 15) && ((p->shapeMask >> bs) & 1) && x < FW && y < FH;
-            };
-            bool fin[85];
-            const int base[4] = {0, 1, 5, 21};
-            for (int L = 3; L >= 0; L--) {
-                const int n = 64 >> L, half = n >> 1, q4 = n >> 2, per = 1 << L;
-                const int sq = n == 8 ? BLOCK_8X8 : n == 16 ? BLOCK_16X16 : n == 32 ? BLOCK_32X32 : BLOCK_64X64;
-                auto sub = [&](int pt) { return (int)av1t::Partition_Subsize[pt][sq]; };
-                const int pm = n == 8 ? p->partMask8 : p->partMaskN;
-                auto on = [&](int pt) { return ((pm >> pt) & 1) != 0; };
-                for (int ny = 0; ny < per; ny++)
-                    for (int nx = 0; nx < per; nx++) {
-                        bool &f = fin[base[L] + ny * per + nx];
-                        const int x = nx * n, y = ny * n;
-                        if (x >= FW || y >= FH) { f = true; continue; }
-                        const bool hasRows = y + half < FH, hasCols = x + half < FW;
-                        bool split = true;
-                        if (n == 8) {
-                            for (int k = 0; k < 4; k++) {
-                                const int xx = x + (k & 1) * 4, yy = y + (k >> 1) * 4;
-                                if (xx < FW && yy < FH) split = split && blk(BLOCK_4X4, xx, yy);
-                            }
-                        } else {
-                            for (int k = 0; k < 4; k++) split = split && fin[base[L + 1] + (ny * 2 + (k >> 1)) * per * 2 + nx * 2 + (k & 1)];
-                        }
-                        f = split;
-                        if (n > p->maxPart) continue;
-                        const int hz = sub(PARTITION_HORZ), vt = sub(PARTITION_VERT);
-                        if (hasRows && hasCols) {
-                            const int sp = sub(PARTITION_SPLIT);
-                            f = f || (on(PARTITION_NONE) && blk(sq, x, y));
-                            f = f || (on(PARTITION_HORZ) && blk(hz, x, y) && blk(hz, x, y + half));
-                            f = f || (on(PARTITION_VERT) && blk(vt, x, y) && blk(vt, x + half, y));
-                            if (n > 8) {
-                                const bool s00 = blk(sp, x, y), s01 = blk(sp, x + half, y), s10 = blk(sp, x, y + half), s11 = blk(sp, x + half, y + half);
-                                f = f || (on(PARTITION_HORZ_A) && s00 && s01 && blk(hz, x, y + half));
-                                f = f || (on(PARTITION_HORZ_B) && blk(hz, x, y) && s10 && s11);
-                                f = f || (on(PARTITION_VERT_A) && s00 && s10 && blk(vt, x + half, y));
-                                f = f || (on(PARTITION_VERT_B) && blk(vt, x, y) && s01 && s11);
-                                bool fh = on(PARTITION_HORZ_4), fv = on(PARTITION_VERT_4);
-                                const int h4 = sub(PARTITION_HORZ_4), v4 = sub(PARTITION_VERT_4);
-                                for (int i = 0; i < 4; i++) {
-                                    if (i < 3 || y + 3 * q4 < FH) fh = fh && blk(h4, x, y + i * q4);
-                                    if (i < 3 || x + 3 * q4 < FW) fv = fv && blk(v4, x + i * q4, y);
-                                }
-                                f = f || fh || fv;
-                            }
-                        } else if (hasCols) {
-                            f = f || blk(hz, x, y);
-                        } else if (hasRows) {
-                            f = f || blk(vt, x, y);
-                        }
-                    }
-            }
-            if (!fin[0]) {
-                snprintf(err, errlen, "the shape / partition options leave some superblock no partition the analysis evaluates");
+        for (int FW = 8; FW <= 64; FW += 8)
+            if (!partitionable(*p, 64, 0, 0, FW, FH)) {
+                snprintf(err, errlen, "the shape / partition options leave some superblock no partition the analysis evaluates "
+                                      "(a frame part of %dx%d samples)", FW, FH);
                 return -1;
             }
-        }
     return 0;
 }
 

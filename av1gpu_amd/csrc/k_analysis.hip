@@ -296,108 +296,97 @@ void launch_analysis(const Batch &B, const SearchParams &P, const uint32_t *shTo
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// partition DP (one thread per SB)
-__device__ __forceinline__ int sq_bsize(int n) {
-    return n == 4 ? BLOCK_4X4 : n == 8 ? BLOCK_8X8 : n == 16 ? BLOCK_16X16 : n == 32 ? BLOCK_32X32 : BLOCK_64X64;
-}
-__device__ __forceinline__ int sub_bsize(int p, int n) { return av1t::Partition_Subsize[p][sq_bsize(n)]; }
+// Partition DP: one wave per 64x64 superblock, four superblocks per workgroup.  The 85 square nodes are solved level by
+// level from the 8x8 nodes up, one node per lane (64, 16, 4, 1 lanes); a node weighs every partition the frame edge and
+// the options allow (Part_Order), each costing fmaf(lambda, bits, sum of its in-frame parts' J in part order) -- the
+// analysis J of a block part, the already solved cost of a sub-node -- and keeps the first cheapest.  The leaves are
+// then placed without a walk: every node knows how many leaves its chosen subtree has, offsets flow down the split
+// nodes level by level, and each node that ends a branch writes its own blocks at its offset, so the decode order of
+// the reference's walk (k_dp.cuh:72-297) comes out of the prefix sums.
+constexpr int DP_WAVES = 4;
+constexpr uint16_t DP_DEAD = 0xFFFF;   // a node no chosen tree reaches
+struct DpNodes {
+    float J[85];
+    int8_t part[85];   // -1: the node lies outside the frame
+    uint16_t leaves[85], first[85];
+};
+__device__ __forceinline__ int dp_node(int L, int nx, int ny) { return (L == 0 ? 0 : L == 1 ? 1 : L == 2 ? 5 : 21) + ny * (1 << L) + nx; }
 
-__global__ void __launch_bounds__(64) k_dp(Batch B, SearchParams P) {
-    const uint32_t sb = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sb >= B.nSb) return;
-    const Img &I = B.img[B.sbImg[sb]];
-    const DevTables &T = *B.T;
-    const uint32_t local = sb - I.sbBase;
-    const int sbx = (int)(local % I.sbCols) * 64, sby = (int)(local / I.sbCols) * 64;
-    const int FW = I.MiCols * 4, FH = I.MiRows * 4;
-    const float lambda = T.lambda[I.bd == 10][I.qindex] * P.lambdaMul;
+__global__ void __launch_bounds__(64 * DP_WAVES) k_dp(Batch B, SearchParams P) {
+    __shared__ DpNodes S[DP_WAVES];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t sb = blockIdx.x * DP_WAVES + w;
+    const bool active = sb < B.nSb;
+    DpNodes &D = S[w];
+    const Img *Ip = active ? &B.img[B.sbImg[sb]] : nullptr;
+    int sbx = 0, sby = 0, FW = 0, FH = 0;
+    float lambda = 0.f;
+    if (active) {
+        const uint32_t local = sb - Ip->sbBase;
+        sbx = (int)(local % Ip->sbCols) * 64; sby = (int)(local / Ip->sbCols) * 64;
+        FW = Ip->MiCols * 4; FH = Ip->MiRows * 4;
+        lambda = B.T->lambda[Ip->bd == 10][Ip->qindex] * P.lambdaMul;
+    }
     auto blockJ = [&](int bs, int x, int y) -> float {
-        if (bs >= BLOCK_INVALID || I.shNbx[bs] == 0) return INF;
-        const int w = 4 * bw4(bs), h = 4 * bh4(bs);
-        return B.blk[I.shOff[bs] + (uint32_t)(y / h) * I.shNbx[bs] + x / w].J;
+        if (bs >= BLOCK_INVALID || Ip->shNbx[bs] == 0) return INF;
+        const int bw = 4 * bw4(bs), bh = 4 * bh4(bs);
+        return B.blk[Ip->shOff[bs] + (uint32_t)(y / bh) * Ip->shNbx[bs] + x / bw].J;
     };
-    float nJ[85];
-    int8_t nP[85];
-    const int lvBase[4] = {0, 1, 5, 21};
-    for (int Lv = 3; Lv >= 0; Lv--) {
-        const int n = 64 >> Lv, half = n >> 1, q4 = n >> 2, per = 1 << Lv;
-        const int bsl = Lv == 3 ? 1 : Lv == 2 ? 2 : Lv == 1 ? 3 : 4;
-        const int pmask = n == 8 ? P.partMask8 : P.partMaskN;
-        for (int ny = 0; ny < per; ny++)
-            for (int nx = 0; nx < per; nx++) {
-                const int ni = lvBase[Lv] + ny * per + nx;
-                const int x = sbx + nx * n, y = sby + ny * n;
-                if (x >= FW || y >= FH) {   // outside the frame: no partition (exported as 0, masked in training)
-                    nJ[ni] = 0.f; nP[ni] = -1;
-                    if (B.dpRel) for (int k = 0; k < 3; k++) B.dpRel[(size_t)sb * 255 + 85 * k + ni] = 0.f;
-                    continue;
-                }
-                const bool hasRows = y + half < FH, hasCols = x + half < FW;
-                float splitJ = 0.f;
-                if (n == 8) {
-                    for (int k = 0; k < 4; k++) {
-                        const int xx = x + (k & 1) * 4, yy = y + (k >> 1) * 4;
-                        if (xx < FW && yy < FH) splitJ = splitJ + blockJ(BLOCK_4X4, xx, yy);
-                    }
-                } else {
-                    const int cb = lvBase[Lv + 1], cper = per * 2;
-                    for (int k = 0; k < 4; k++) splitJ = splitJ + nJ[cb + (ny * 2 + (k >> 1)) * cper + nx * 2 + (k & 1)];
-                }
-                // all options; and the best without the children (no N/2 squares), without the node's N x N / 2:1
-                // shapes (SPLIT or the 4-way partitions), without the 4-way shapes
+    // bottom-up: node costs, choices, leaf counts
+    for (int L = 3; L >= 0; L--) {
+        const int per = 1 << L, n = 64 >> L;
+        if (active && lane < per * per) {
+            const int nx = lane % per, ny = lane / per, ni = dp_node(L, nx, ny);
+            const int x = sbx + nx * n, y = sby + ny * n, q = n >> 2;
+            if (x >= FW || y >= FH) {
+                D.J[ni] = 0.f; D.part[ni] = -1; D.leaves[ni] = 0;
+                if (B.dpRel) for (int k = 0; k < 3; k++) B.dpRel[(size_t)sb * 255 + 85 * k + ni] = 0.f;
+            } else {
+                const bool rows = y + 2 * q < FH, cols = x + 2 * q < FW, big = n > P.maxPart;
+                const int pmask = n == 8 ? P.partMask8 : P.partMaskN, bsl = L == 3 ? 1 : L == 2 ? 2 : L == 1 ? 3 : 4;
+                // what the frame edge leaves: inside, every partition the options allow at avgPart bits; past the
+                // bottom (right) edge SPLIT or HORZ (VERT) for one bit; past both, SPLIT for none
+                uint32_t allowed;
+                float edgeBits;
+                if (rows && cols) { allowed = big ? (1u << PARTITION_SPLIT) : ((uint32_t)pmask | (1u << PARTITION_SPLIT)); edgeBits = -1.f; }
+                else if (cols) { allowed = (1u << PARTITION_SPLIT) | (big ? 0u : 1u << PARTITION_HORZ); edgeBits = 1.f; }
+                else if (rows) { allowed = (1u << PARTITION_SPLIT) | (big ? 0u : 1u << PARTITION_VERT); edgeBits = 1.f; }
+                else { allowed = 1u << PARTITION_SPLIT; edgeBits = 0.f; }
+                if (n == 8) allowed &= 0xFu;   // no AB or 4-way partitions of 8x8 nodes
                 float best = INF, noKids = INF, noOwn = INF, noFour = INF;
                 int bp = PARTITION_SPLIT;
-                auto consider = [&](int p, float J, float bits) {
+                uint16_t bLeaves = 0;
+                for (int o = 0; o < 10; o++) {
+                    const int p = Part_Order[o];
+                    if (!((allowed >> p) & 1)) continue;
+                    float J = 0.f;
+                    uint16_t nl = 0;
+                    for (int k = 0; k < Part_Shape[p].parts; k++) {
+                        const PartPlace at = Part_Shape[p].at[k];
+                        const int px = x + at.dx * q, py = y + at.dy * q;
+                        if (px >= FW || py >= FH) continue;
+                        if (p == PARTITION_SPLIT && n > 8) {
+                            const int ci = dp_node(L + 1, nx * 2 + (at.dx >> 1), ny * 2 + (at.dy >> 1));
+                            J = J + D.J[ci];
+                            nl += D.leaves[ci];
+                        } else {
+                            J = J + blockJ(part_bsize(p, k, n), px, py);
+                            nl++;
+                        }
+                    }
+                    if (p == PARTITION_NONE) {
+                        if (!(J < INF)) continue;
+                        J = J * P.splitBias;
+                    }
+                    const float bits = edgeBits < 0.f ? B.T->avgPart[bsl][p] : edgeBits;
                     const float t = fmaf(lambda, bits, J);
-                    if (t < best) { best = t; bp = p; }
+                    if (t < best) { best = t; bp = p; bLeaves = nl; }
                     const bool four = p == PARTITION_HORZ_4 || p == PARTITION_VERT_4;
                     if (four || p == PARTITION_NONE || p == PARTITION_HORZ || p == PARTITION_VERT) noKids = fminf(noKids, t);
                     if (four || p == PARTITION_SPLIT) noOwn = fminf(noOwn, t);
                     if (!four) noFour = fminf(noFour, t);
-                };
-                const bool big = n > P.maxPart;
-                if (hasRows && hasCols) {
-                    const float *pb = T.avgPart[bsl];
-                    consider(PARTITION_SPLIT, splitJ, pb[PARTITION_SPLIT]);
-                    if (!big) {
-                        const int hz = sub_bsize(PARTITION_HORZ, n), vt = sub_bsize(PARTITION_VERT, n), sp = sub_bsize(PARTITION_SPLIT, n);
-                        if (pmask & (1 << PARTITION_NONE)) {
-                            const float jn = blockJ(sq_bsize(n), x, y);
-                            if (jn < INF) consider(PARTITION_NONE, jn * P.splitBias, pb[PARTITION_NONE]);
-                        }
-                        if (pmask & (1 << PARTITION_HORZ)) consider(PARTITION_HORZ, blockJ(hz, x, y) + blockJ(hz, x, y + half), pb[PARTITION_HORZ]);
-                        if (pmask & (1 << PARTITION_VERT)) consider(PARTITION_VERT, blockJ(vt, x, y) + blockJ(vt, x + half, y), pb[PARTITION_VERT]);
-                        if (n > 8) {
-                            const float s00 = blockJ(sp, x, y), s01 = blockJ(sp, x + half, y), s10 = blockJ(sp, x, y + half),
-                                        s11 = blockJ(sp, x + half, y + half);
-                            if (pmask & (1 << PARTITION_HORZ_A)) consider(PARTITION_HORZ_A, s00 + s01 + blockJ(hz, x, y + half), pb[PARTITION_HORZ_A]);
-                            if (pmask & (1 << PARTITION_HORZ_B)) consider(PARTITION_HORZ_B, blockJ(hz, x, y) + s10 + s11, pb[PARTITION_HORZ_B]);
-                            if (pmask & (1 << PARTITION_VERT_A)) consider(PARTITION_VERT_A, s00 + s10 + blockJ(vt, x + half, y), pb[PARTITION_VERT_A]);
-                            if (pmask & (1 << PARTITION_VERT_B)) consider(PARTITION_VERT_B, blockJ(vt, x, y) + s01 + s11, pb[PARTITION_VERT_B]);
-                            const int h4 = sub_bsize(PARTITION_HORZ_4, n), v4 = sub_bsize(PARTITION_VERT_4, n);
-                            if (pmask & (1 << PARTITION_HORZ_4)) {
-                                float J = 0.f;
-                                for (int i = 0; i < 4; i++) if (i < 3 || y + 3 * q4 < FH) J = J + blockJ(h4, x, y + i * q4);
-                                consider(PARTITION_HORZ_4, J, pb[PARTITION_HORZ_4]);
-                            }
-                            if (pmask & (1 << PARTITION_VERT_4)) {
-                                float J = 0.f;
-                                for (int i = 0; i < 4; i++) if (i < 3 || x + 3 * q4 < FW) J = J + blockJ(v4, x + i * q4, y);
-                                consider(PARTITION_VERT_4, J, pb[PARTITION_VERT_4]);
-                            }
-                        }
-                    }
-                } else if (hasCols) {
-                    consider(PARTITION_SPLIT, splitJ, 1.f);
-                    if (!big) consider(PARTITION_HORZ, blockJ(sub_bsize(PARTITION_HORZ, n), x, y), 1.f);
-                } else if (hasRows) {
-                    consider(PARTITION_SPLIT, splitJ, 1.f);
-                    if (!big) consider(PARTITION_VERT, blockJ(sub_bsize(PARTITION_VERT, n), x, y), 1.f);
-                } else {
-                    consider(PARTITION_SPLIT, splitJ, 0.f);
                 }
-                nJ[ni] = best;
-                nP[ni] = (int8_t)bp;
+                D.J[ni] = best; D.part[ni] = (int8_t)bp; D.leaves[ni] = bLeaves;
                 if (B.dpRel) {   // relative increase of the node's cost without its children / own shapes / 4-way shapes
                     const float den = fmaxf(best, 1e-3f);
                     B.dpRel[(size_t)sb * 255 + ni] = fminf((noKids - best) / den, 1e6f);
@@ -405,50 +394,50 @@ __global__ void __launch_bounds__(64) k_dp(Batch B, SearchParams P) {
                     B.dpRel[(size_t)sb * 255 + 170 + ni] = fminf((noFour - best) / den, 1e6f);
                 }
             }
+            D.first[ni] = ni == 0 ? 0 : DP_DEAD;
+        }
+        __syncthreads();
     }
-    B.sbJ[sb] = nJ[0];
+    // top-down: each reached split node hands its children their first leaf index (children in decode order)
+    for (int L = 0; L < 3; L++) {
+        const int per = 1 << L;
+        if (active && lane < per * per) {
+            const int nx = lane % per, ny = lane / per, ni = dp_node(L, nx, ny);
+            if (D.first[ni] != DP_DEAD && D.part[ni] == PARTITION_SPLIT) {
+                uint16_t f = D.first[ni];
+                for (int k = 0; k < 4; k++) {
+                    const int ci = dp_node(L + 1, nx * 2 + (k & 1), ny * 2 + (k >> 1));
+                    D.first[ci] = f;
+                    f += D.leaves[ci];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // every reached node that ends a branch writes its blocks
+    if (!active) return;
     Leaf *out = B.leaves + (size_t)sb * MAX_LEAVES;
-    int nl = 0;
-    auto leaf = [&](int bs, int x, int y) {
-        if (x >= FW || y >= FH) return;
-        Leaf lf;
-        lf.bsize = (uint8_t)bs; lf.x4 = (uint8_t)((x - sbx) >> 2); lf.y4 = (uint8_t)((y - sby) >> 2); lf.pad = 0;
-        out[nl++] = lf;
-    };
-    int stk[64][3];
-    int sp = 0;
-    stk[0][0] = 0; stk[0][1] = 0; stk[0][2] = 0; sp = 1;
-    while (sp > 0) {
-        sp--;
-        const int Lv = stk[sp][0], nx = stk[sp][1], ny = stk[sp][2];
-        const int n = 64 >> Lv, half = n >> 1, q4 = n >> 2, per = 1 << Lv;
-        const int p = nP[lvBase[Lv] + ny * per + nx];
-        const int x = sbx + nx * n, y = sby + ny * n;
-        if (p < 0) continue;
-        const int sub = sub_bsize(p, n), sp2 = sub_bsize(PARTITION_SPLIT, n);
-        const bool hasRows = y + half < FH, hasCols = x + half < FW;
-        switch (p) {
-        case PARTITION_NONE: leaf(sub, x, y); break;
-        case PARTITION_HORZ: leaf(sub, x, y); if (hasRows) leaf(sub, x, y + half); break;
-        case PARTITION_VERT: leaf(sub, x, y); if (hasCols) leaf(sub, x + half, y); break;
-        case PARTITION_SPLIT:
-            if (n == 8) { leaf(BLOCK_4X4, x, y); leaf(BLOCK_4X4, x + 4, y); leaf(BLOCK_4X4, x, y + 4); leaf(BLOCK_4X4, x + 4, y + 4); }
-            else for (int k = 3; k >= 0; k--) { stk[sp][0] = Lv + 1; stk[sp][1] = nx * 2 + (k & 1); stk[sp][2] = ny * 2 + (k >> 1); sp++; }
-            break;
-        case PARTITION_HORZ_A: leaf(sp2, x, y); leaf(sp2, x + half, y); leaf(sub, x, y + half); break;
-        case PARTITION_HORZ_B: leaf(sub, x, y); leaf(sp2, x, y + half); leaf(sp2, x + half, y + half); break;
-        case PARTITION_VERT_A: leaf(sp2, x, y); leaf(sp2, x, y + half); leaf(sub, x + half, y); break;
-        case PARTITION_VERT_B: leaf(sub, x, y); leaf(sp2, x + half, y); leaf(sp2, x + half, y + half); break;
-        case PARTITION_HORZ_4: for (int i = 0; i < 4; i++) if (i < 3 || y + 3 * q4 < FH) leaf(sub, x, y + i * q4); break;
-        default: for (int i = 0; i < 4; i++) if (i < 3 || x + 3 * q4 < FW) leaf(sub, x + i * q4, y); break;
+    for (int L = 0; L < 4; L++) {
+        const int per = 1 << L, n = 64 >> L;
+        if (lane >= per * per) continue;
+        const int nx = lane % per, ny = lane / per, ni = dp_node(L, nx, ny), p = D.part[ni];
+        if (D.first[ni] == DP_DEAD || p < 0 || (p == PARTITION_SPLIT && n > 8)) continue;
+        const int x = sbx + nx * n, y = sby + ny * n, q = n >> 2;
+        int at = D.first[ni];
+        for (int k = 0; k < Part_Shape[p].parts; k++) {
+            const int px = x + Part_Shape[p].at[k].dx * q, py = y + Part_Shape[p].at[k].dy * q;
+            if (px >= FW || py >= FH) continue;
+            Leaf lf;
+            lf.bsize = (uint8_t)part_bsize(p, k, n); lf.x4 = (uint8_t)((px - sbx) >> 2); lf.y4 = (uint8_t)((py - sby) >> 2); lf.pad = 0;
+            out[at++] = lf;
         }
     }
-    B.leafCount[sb] = (uint16_t)nl;
+    if (lane == 0) { B.sbJ[sb] = D.J[0]; B.leafCount[sb] = D.leaves[0]; }
 }
 
 void launch_dp(const Batch &B, const SearchParams &P, hipStream_t st) {
     if (!B.nSb) return;
-    hipLaunchKernelGGL(k_dp, dim3((B.nSb + 63) / 64), dim3(64), 0, st, B, P);
+    hipLaunchKernelGGL(k_dp, dim3((B.nSb + DP_WAVES - 1) / DP_WAVES), dim3(64 * DP_WAVES), 0, st, B, P);
 }
 
 }  // namespace av1m
