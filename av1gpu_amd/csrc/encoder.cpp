@@ -37,6 +37,7 @@ int recon_prepare(int bd);
 size_t recon_smem_bytes();
 void launch_pack(const Batch &B, const PackOff *dOff, RecBlock *oB, uint32_t *oBT, RecTxb *oT, int16_t *oC, hipStream_t st);
 void launch_to10(const uint8_t *src, int sp, uint16_t *dst, int dp, int w, int h, int chroma, hipStream_t st);
+hipError_t noalloc_scratch(void **ptr, size_t *bytes);
 }  // namespace av1m
 
 using namespace av1m;
@@ -127,6 +128,8 @@ struct av1mi_encoder {
                                       // device, i.e. for the other encoders' work in a multi-stream server
     bool dpExport = false;            // keep the DP's per-node relative costs of the last call (training labels)
     std::vector<float> dpRel;
+    void *scratch = nullptr;          // this device's copy of the no-allocation scratch array, and its size
+    size_t scratchBytes = 0;
     bool searchExport = false;        // keep the last call's analysis records, superblock costs and leaves (checks)
     std::vector<Img> exImg;
     std::vector<BlockRes> exBlk;
@@ -253,6 +256,12 @@ int av1mi_create(const av1mi_config *cfg, av1mi_encoder **out, char *err, int er
         delete e;
         return -1;
     }
+    // the no-allocation entry point's scratch array: this device's address, resolved with the device current
+    if ((he = noalloc_scratch(&e->scratch, &e->scratchBytes)) != hipSuccess) {
+        snprintf(err, errlen, "HIP: %s (scratch array)", hipGetErrorString(he));
+        av1mi_destroy(e);
+        return -1;
+    }
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, e->device) == hipSuccess) e->cus = v;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeWallClockRate, e->device) == hipSuccess && v > 0) e->clockHz = (uint64_t)v * 1000;
@@ -354,10 +363,19 @@ void av1mi_get_config(const av1mi_encoder *e, av1mi_config *cfg) { *cfg = e->cfg
 
 }  // extern "C"
 
+// The no-allocation variant of an encode: the arena laid over the device's scratch array, the packed records written
+// into the caller's buffers.
+struct NoAlloc {
+    void *mem;
+    size_t bytes;
+    const av1mi_noalloc_out *o;
+};
+
 // GpEncoder::encode (pipeline.cpp:323-1181) for the plain intra path.  dev != nullptr: the packed records stay on the
-// device and pass to the caller (GpDevRecords); otherwise they are copied into each frame's host arrays.
+// device and pass to the caller (GpDevRecords); na != nullptr: no device allocation (scratch array, caller's outputs);
+// otherwise they are copied into each frame's host arrays.
 static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_frame_out *out, av1mi_times *t, void *stream,
-                       int wantRecon, av1mi_dev_records *dev, char *err, int errlen) {
+                       int wantRecon, av1mi_dev_records *dev, const NoAlloc *na, char *err, int errlen) {
     hipStream_t st = (hipStream_t)stream;
     av1mi_times tm;
     memset(&tm, 0, sizeof tm);
@@ -464,7 +482,11 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
     PackOff *dPO;
     A.add(dSteps, stepFlat.size()); A.add(dPO, nSb);
     uint8_t *dLnAct = nullptr;
-    const uint32_t lnN = std::min(nSb, enc->lnChunk);
+    uint32_t lnN = std::min(nSb, enc->lnChunk);
+    const size_t fixedBytes = A.total + (enc->ln ? (((size_t)nSb * 12 * 4 + 255) & ~(size_t)255) : 0) +
+                              (enc->dpExport ? (((size_t)nSb * 255 * 4 + 255) & ~(size_t)255) : 0);
+    if (na && enc->ln)   // the network's activations in what the scratch array has left: fewer superblocks per pass
+        while (lnN > 1 && fixedBytes + ((ln_scratch_bytes(lnN) + 255) & ~(size_t)255) > na->bytes) lnN = (lnN + 1) / 2;
     if (enc->ln) { A.add(dLnMask, (size_t)nSb * 12); A.add(dLnAct, ln_scratch_bytes(lnN)); }
     float *dDpRel = nullptr;
     if (enc->dpExport) A.add(dDpRel, (size_t)nSb * 255);
@@ -485,7 +507,15 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
     const uint32_t qBlocks = enc->cfg.queueBlocks > 0 ? (uint32_t)enc->cfg.queueBlocks : (uint32_t)std::max(1, enc->cus * slots);
     // ---- device memory: stream-ordered per call, or the captured plan's (graph mode) ----
     bool capture = false;
-    if (enc->cfg.graph) {
+    const bool graph = enc->cfg.graph && !na;
+    if (na) {
+        if (A.total > na->bytes) {
+            snprintf(err, errlen, "av1mi_encode_noalloc: the batch needs %zu bytes of device scratch, the scratch array holds %zu",
+                     A.total, na->bytes);
+            return -1;
+        }
+        A.bind(na->mem);
+    } else if (graph) {
         std::vector<uint64_t> key = sig;
         key.insert(key.end(), {(uint64_t)anS, (uint64_t)sched, (uint64_t)qBlocks, (uint64_t)A.total});
         if (enc->plan.key != key) {
@@ -549,7 +579,7 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
         }
         return hipGetLastError() == hipSuccess ? 0 : -1;
     };
-    if (enc->cfg.graph) {
+    if (graph) {
         if (capture) {   // shape data once, then the capture
             HCK(hipMemcpy(dShStart, shStart.data(), shStart.size() * 4, hipMemcpyHostToDevice));
             HCK(hipMemcpy(dSbImg, sbImg.data(), (size_t)nSb * 4, hipMemcpyHostToDevice));
@@ -583,7 +613,7 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
     tm.schedule = (uint32_t)sched;
     tm.queueBlocks = sched == 1 ? 0 : std::min<uint32_t>(qBlocks, nSb);
     tm.anStreams = (uint32_t)anS;
-    tm.graph = (uint32_t)enc->cfg.graph;
+    tm.graph = (uint32_t)graph;
     tm.tuned = tu && cand < 0 ? 1u : 0u;
     (void)hipEventRecord(enc->ev[4], st);
     tm.steps = (uint32_t)steps.size();
@@ -633,80 +663,112 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
     imgBlk[n] = gBlk; imgTxb[n] = gTxb; imgCoef[n] = gCoef;
     Arena R;
     RecBlock *rB; uint32_t *rBT; RecTxb *rT; int16_t *rC;
-    R.add(rB, gBlk); R.add(rBT, gBlk); R.add(rT, gTxb); R.add(rC, gCoef);
-    HCK(R.alloc(st));
+    if (na) {   // the caller's output buffers
+        const av1mi_noalloc_out &o = *na->o;
+        if (gBlk > o.cap_blocks || gTxb > o.cap_txb || gCoef > o.cap_coeffs) {
+            snprintf(err, errlen, "av1mi_encode_noalloc: the batch has %llu blocks, %llu tx blocks and %llu coefficients, the output "
+                     "buffers hold %llu, %llu and %llu", (unsigned long long)gBlk, (unsigned long long)gTxb, (unsigned long long)gCoef,
+                     (unsigned long long)o.cap_blocks, (unsigned long long)o.cap_txb, (unsigned long long)o.cap_coeffs);
+            return -1;
+        }
+        rB = (RecBlock *)o.blocks; rBT = o.block_txb; rT = (RecTxb *)o.txb; rC = o.coeffs;
+    } else {
+        R.add(rB, gBlk); R.add(rBT, gBlk); R.add(rT, gTxb); R.add(rC, gCoef);
+        HCK(R.alloc(st));
+    }
     HCK(hipMemcpyAsync(dPO, po.data(), (size_t)nSb * sizeof(PackOff), hipMemcpyHostToDevice, st));
     (void)hipEventRecord(enc->ev[5], st);
     launch_pack(B, dPO, rB, rBT, rT, rC, st);
     HCK(hipGetLastError());
     (void)hipEventRecord(enc->ev[6], st);
-    // records (and reconstructions) go to the host through the encoder's pinned staging buffer in one DMA per region,
-    // then into each frame's own buffers on several host threads
-    size_t reconBytes = 0;
-    if (wantRecon)
-        for (int i = 0; i < n; i++) reconBytes += ((size_t)imgs[i].aw * imgs[i].ah + 2 * (size_t)(imgs[i].aw >> imgs[i].ss) * (imgs[i].ah >> imgs[i].ss)) * bps;
-    const size_t need = (dev ? 0 : R.total) + reconBytes;
-    if (need > enc->stageBytes) {
-        HCK(hipStreamSynchronize(st));
-        if (enc->stage) enc->oldStages.push_back(enc->stage);
-        enc->stage = nullptr;
-        const size_t cap = std::max(need + need / 4, 2 * enc->stageBytes);
-        enc->stageBytes = 0;
-        HCK(hipHostMalloc(&enc->stage, cap, hipHostMallocDefault));
-        enc->stageBytes = cap;
-    }
-    char *stg = (char *)enc->stage;
-    if (!dev) HCK(hipMemcpyAsync(stg, R.base, R.total, hipMemcpyDeviceToHost, st));
-    auto hostOf = [&](const void *dptr) { return stg + ((const char *)dptr - (const char *)R.base); };
-    std::vector<std::tuple<void *, const void *, size_t>> jobs;
-    size_t reconOff = dev ? 0 : R.total;
-    for (int i = 0; i < n; i++) {
-        const Img &I = imgs[i];
-        av1mi_frame_out &o = out[i];
-        o.width = I.W; o.height = I.H; o.base_q_idx = I.qindex; o.bit_depth = bd; o.chroma_format = I.ss ? 0 : 1; o.monochrome = I.mono;
-        o.n_blocks = (uint32_t)(imgBlk[i + 1] - imgBlk[i]);
-        o.n_txb = (uint32_t)(imgTxb[i + 1] - imgTxb[i]);
-        o.n_coeffs = imgCoef[i + 1] - imgCoef[i];
-        o.aw = I.aw; o.ah = I.ah; o.tile_cols = I.tileCols; o.tile_rows = I.tileRows;
-        if (!dev) {
-            o.blocks = (av1mi_block *)malloc(std::max<size_t>(o.n_blocks, 1) * 16);
-            o.block_txb = (uint32_t *)malloc(((size_t)o.n_blocks + 1) * 4);
-            o.txb = (av1mi_txb *)malloc(std::max<size_t>(o.n_txb, 1) * 16);
-            o.coeffs = (int16_t *)malloc(std::max<uint64_t>(o.n_coeffs, 1) * 2);
-            jobs.emplace_back(o.blocks, hostOf(rB + imgBlk[i]), (size_t)o.n_blocks * 16);
-            jobs.emplace_back(o.block_txb, hostOf(rBT + imgBlk[i]), (size_t)o.n_blocks * 4);
-            jobs.emplace_back(o.txb, hostOf(rT + imgTxb[i]), (size_t)o.n_txb * 16);
-            jobs.emplace_back(o.coeffs, hostOf(rC + imgCoef[i]), (size_t)o.n_coeffs * 2);
-        }
-        if (wantRecon) {
-            for (int p = 0; p < 3; p++) {
+    if (na) {   // the records are in the caller's buffers; the reconstruction goes to its planes, if it gave any
+        for (int i = 0; i < n; i++) {
+            const Img &I = imgs[i];
+            av1mi_frame_out &o = out[i];
+            o.width = I.W; o.height = I.H; o.base_q_idx = I.qindex; o.bit_depth = bd; o.chroma_format = I.ss ? 0 : 1;
+            o.monochrome = I.mono;
+            o.n_blocks = (uint32_t)(imgBlk[i + 1] - imgBlk[i]);
+            o.n_txb = (uint32_t)(imgTxb[i + 1] - imgTxb[i]);
+            o.n_coeffs = imgCoef[i + 1] - imgCoef[i];
+            o.aw = I.aw; o.ah = I.ah; o.tile_cols = I.tileCols; o.tile_rows = I.tileRows;
+            for (int p = 0; p < 3 && na->o->recon; p++) {
+                void *dst = na->o->recon[3 * (size_t)i + p];
+                if (!dst) continue;
                 const int pw = p ? I.aw >> I.ss : I.aw, ph = p ? I.ah >> I.ss : I.ah;
-                const size_t b = (size_t)pw * ph * bps;
-                o.recon[p] = malloc(b);
-                o.recon_pitch[p] = pw;
-                HCK(hipMemcpyAsync(stg + reconOff, I.rec[p], b, hipMemcpyDeviceToHost, st));
-                jobs.emplace_back(o.recon[p], stg + reconOff, b);
-                reconOff += b;
+                HCK(hipMemcpyAsync(dst, I.rec[p], (size_t)pw * ph * bps, hipMemcpyDeviceToDevice, st));
             }
         }
-    }
-    (void)hipEventRecord(enc->ev[7], st);
-    HCK(hipStreamSynchronize(st));
-    parallel_copy(jobs);
-    if (dev) {   // the packed arena passes to the caller as it is: frames back to back, frame-relative offsets
-        memset(dev, 0, sizeof *dev);
-        dev->n_frames = (uint32_t)n;
-        dev->blk0 = (uint32_t *)malloc(((size_t)n + 1) * 4);
-        dev->txb0 = (uint32_t *)malloc(((size_t)n + 1) * 4);
-        dev->coef0 = (uint64_t *)malloc(((size_t)n + 1) * 8);
-        for (int i = 0; i <= n; i++) { dev->blk0[i] = (uint32_t)imgBlk[i]; dev->txb0[i] = (uint32_t)imgTxb[i]; dev->coef0[i] = imgCoef[i]; }
-        dev->blocks = (av1mi_block *)rB; dev->block_txb_rel = rBT; dev->txb = (av1mi_txb *)rT; dev->coeffs = rC;
-        dev->pool = R.base;
-        R.base = nullptr;
-        R.owned = false;
+        (void)hipEventRecord(enc->ev[7], st);
+        HCK(hipStreamSynchronize(st));
     } else {
-        for (int i = 0; i < n; i++) out[i].block_txb[out[i].n_blocks] = out[i].n_txb;
-        R.release(st);
+        // records (and reconstructions) go to the host through the encoder's pinned staging buffer in one DMA per region,
+        // then into each frame's own buffers on several host threads
+        size_t reconBytes = 0;
+        if (wantRecon)
+            for (int i = 0; i < n; i++) reconBytes += ((size_t)imgs[i].aw * imgs[i].ah + 2 * (size_t)(imgs[i].aw >> imgs[i].ss) * (imgs[i].ah >> imgs[i].ss)) * bps;
+        const size_t need = (dev ? 0 : R.total) + reconBytes;
+        if (need > enc->stageBytes) {
+            HCK(hipStreamSynchronize(st));
+            if (enc->stage) enc->oldStages.push_back(enc->stage);
+            enc->stage = nullptr;
+            const size_t cap = std::max(need + need / 4, 2 * enc->stageBytes);
+            enc->stageBytes = 0;
+            HCK(hipHostMalloc(&enc->stage, cap, hipHostMallocDefault));
+            enc->stageBytes = cap;
+        }
+        char *stg = (char *)enc->stage;
+        if (!dev) HCK(hipMemcpyAsync(stg, R.base, R.total, hipMemcpyDeviceToHost, st));
+        auto hostOf = [&](const void *dptr) { return stg + ((const char *)dptr - (const char *)R.base); };
+        std::vector<std::tuple<void *, const void *, size_t>> jobs;
+        size_t reconOff = dev ? 0 : R.total;
+        for (int i = 0; i < n; i++) {
+            const Img &I = imgs[i];
+            av1mi_frame_out &o = out[i];
+            o.width = I.W; o.height = I.H; o.base_q_idx = I.qindex; o.bit_depth = bd; o.chroma_format = I.ss ? 0 : 1; o.monochrome = I.mono;
+            o.n_blocks = (uint32_t)(imgBlk[i + 1] - imgBlk[i]);
+            o.n_txb = (uint32_t)(imgTxb[i + 1] - imgTxb[i]);
+            o.n_coeffs = imgCoef[i + 1] - imgCoef[i];
+            o.aw = I.aw; o.ah = I.ah; o.tile_cols = I.tileCols; o.tile_rows = I.tileRows;
+            if (!dev) {
+                o.blocks = (av1mi_block *)malloc(std::max<size_t>(o.n_blocks, 1) * 16);
+                o.block_txb = (uint32_t *)malloc(((size_t)o.n_blocks + 1) * 4);
+                o.txb = (av1mi_txb *)malloc(std::max<size_t>(o.n_txb, 1) * 16);
+                o.coeffs = (int16_t *)malloc(std::max<uint64_t>(o.n_coeffs, 1) * 2);
+                jobs.emplace_back(o.blocks, hostOf(rB + imgBlk[i]), (size_t)o.n_blocks * 16);
+                jobs.emplace_back(o.block_txb, hostOf(rBT + imgBlk[i]), (size_t)o.n_blocks * 4);
+                jobs.emplace_back(o.txb, hostOf(rT + imgTxb[i]), (size_t)o.n_txb * 16);
+                jobs.emplace_back(o.coeffs, hostOf(rC + imgCoef[i]), (size_t)o.n_coeffs * 2);
+            }
+            if (wantRecon) {
+                for (int p = 0; p < 3; p++) {
+                    const int pw = p ? I.aw >> I.ss : I.aw, ph = p ? I.ah >> I.ss : I.ah;
+                    const size_t b = (size_t)pw * ph * bps;
+                    o.recon[p] = malloc(b);
+                    o.recon_pitch[p] = pw;
+                    HCK(hipMemcpyAsync(stg + reconOff, I.rec[p], b, hipMemcpyDeviceToHost, st));
+                    jobs.emplace_back(o.recon[p], stg + reconOff, b);
+                    reconOff += b;
+                }
+            }
+        }
+        (void)hipEventRecord(enc->ev[7], st);
+        HCK(hipStreamSynchronize(st));
+        parallel_copy(jobs);
+        if (dev) {   // the packed arena passes to the caller as it is: frames back to back, frame-relative offsets
+            memset(dev, 0, sizeof *dev);
+            dev->n_frames = (uint32_t)n;
+            dev->blk0 = (uint32_t *)malloc(((size_t)n + 1) * 4);
+            dev->txb0 = (uint32_t *)malloc(((size_t)n + 1) * 4);
+            dev->coef0 = (uint64_t *)malloc(((size_t)n + 1) * 8);
+            for (int i = 0; i <= n; i++) { dev->blk0[i] = (uint32_t)imgBlk[i]; dev->txb0[i] = (uint32_t)imgTxb[i]; dev->coef0[i] = imgCoef[i]; }
+            dev->blocks = (av1mi_block *)rB; dev->block_txb_rel = rBT; dev->txb = (av1mi_txb *)rT; dev->coeffs = rC;
+            dev->pool = R.base;
+            R.base = nullptr;
+            R.owned = false;
+        } else {
+            for (int i = 0; i < n; i++) out[i].block_txb[out[i].n_blocks] = out[i].n_txb;
+            R.release(st);
+        }
     }
     A.release(st);
     float ms[7];
@@ -727,13 +789,23 @@ extern "C" {
 
 int av1mi_encode(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_frame_out *out, av1mi_times *t, void *stream, int wantRecon,
                  char *err, int errlen) {
-    return encode_impl(enc, in, n, out, t, stream, wantRecon, nullptr, err, errlen);
+    return encode_impl(enc, in, n, out, t, stream, wantRecon, nullptr, nullptr, err, errlen);
+}
+int av1mi_encode_noalloc(av1mi_encoder *enc, const av1mi_frame_in *in, int n, const av1mi_noalloc_out *o, av1mi_frame_out *out,
+                         av1mi_times *t, void *stream, char *err, int errlen) {
+    if (!o || (n > 0 && (!o->blocks || !o->block_txb || !o->txb || !o->coeffs))) {
+        snprintf(err, errlen, "av1mi_encode_noalloc: no output buffers");
+        return -1;
+    }
+    if (!enc->scratch) { snprintf(err, errlen, "av1mi_encode_noalloc: the device's scratch array is not available"); return -1; }
+    const NoAlloc na{enc->scratch, enc->scratchBytes, o};
+    return encode_impl(enc, in, n, out, t, stream, 0, nullptr, &na, err, errlen);
 }
 int av1mi_encode_dev(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_frame_out *out, av1mi_times *t, void *stream,
                      int wantRecon, av1mi_dev_records *dev, char *err, int errlen) {
     if (!dev) { snprintf(err, errlen, "av1mi_encode_dev: no device record holder"); return -1; }
     memset(dev, 0, sizeof *dev);
-    return encode_impl(enc, in, n, out, t, stream, wantRecon, dev, err, errlen);
+    return encode_impl(enc, in, n, out, t, stream, wantRecon, dev, nullptr, err, errlen);
 }
 void av1mi_dev_records_free(av1mi_dev_records *dev, void *stream) {
     if (!dev) return;
@@ -845,6 +917,9 @@ size_t av1mi_frame_bytes(int W, int H, int bitDepth, int yuv444) {
     return planes + arena + records + network + 64 * 1024;
 }
 int av1mi_dev_free(void *ptr) { return hipFree(ptr) == hipSuccess ? 0 : -1; }
+int av1mi_copy_d2h(void *dst, const void *src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
+}
 int av1mi_copy_h2d(void *dst, const void *src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : -1; }
 int av1mi_device_sync(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : -1; }
 

@@ -167,6 +167,34 @@ int av1mi_encode_dev(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1mi_
                      int want_recon, av1mi_dev_records *dev, char *err, int errlen);
 /* gp_dev_records_free (entropy_dev.h:43): releases dev's device allocation on stream and its host arrays. */
 void av1mi_dev_records_free(av1mi_dev_records *dev, void *stream);
+
+/* ---- encode without device allocation ----
+ * No counterpart in the reference.  For callers that cannot allocate device memory: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  av1mi_encode_noalloc is av1mi_encode_dev with every internal buffer
+ * (analysis records, leaves, superblock slots, maps, the pruning network's activations, the unpacked reconstruction) in
+ * one file-scope device array of AV1MI_NOALLOC_SCRATCH_MIB MiB per device, reserved with the library's code and held
+ * until the process ends; it calls no hipMalloc / hipFree / hipHostMalloc.  The caller passes the outputs: the packed
+ * records in device buffers (frames back to back in batch order; block_txb each block's first tx block relative to its
+ * frame, txb.coeff_off relative to its frame, as av1mi_dev_records), and optionally the reconstruction per frame.
+ * out[i] receives the frame's sizes and counts (its pointers stay NULL).  A batch needing more scratch than the array, or
+ * more records than the capacities, fails with -1 and a message before anything is written to the outputs.
+ * Every call on one device shares that device's array: calls that overlap on one device (several host threads, or
+ * several streams) race and must be serialised by the caller.  The encoder's graph option does not apply here (the
+ * device part runs eagerly on stream). */
+#define AV1MI_NOALLOC_SCRATCH_MIB 160
+typedef struct av1mi_noalloc_out {
+    av1mi_block *blocks;     /* device, cap_blocks entries */
+    uint32_t *block_txb;     /* device, cap_blocks entries */
+    av1mi_txb *txb;          /* device, cap_txb entries */
+    int16_t *coeffs;         /* device, cap_coeffs entries */
+    uint64_t cap_blocks, cap_txb, cap_coeffs;
+    /* NULL, or 3 device plane pointers per frame (NULL entries skipped): the reconstruction, aw x ah luma and
+     * (aw >> ss) x (ah >> ss) chroma samples (uint16_t at 10 bits), rows packed (pitch = width) */
+    void *const *recon;
+} av1mi_noalloc_out;
+int av1mi_encode_noalloc(av1mi_encoder *enc, const av1mi_frame_in *in, int n, const av1mi_noalloc_out *o, av1mi_frame_out *out,
+                         av1mi_times *t, void *stream, char *err, int errlen);
+
 /* gp_frames_to_10bit (pipeline.h:218-225): the 8-bit frames of f become 10-bit device planes in place (Y10 =
  * round(Y8 * 1023 / 255), C10 = round((C8 - 128) * 1023 / 255) + 512), all in one stream-ordered allocation returned in
  * *buf (NULL if no frame was converted; free it with av1mi_dev_free_async after the encode).  The source planes must
@@ -224,6 +252,7 @@ void av1mi_buffer_free(void *p);
 int av1mi_dev_alloc(void **ptr, size_t bytes);
 int av1mi_dev_free(void *ptr);
 int av1mi_copy_h2d(void *dst, const void *src, size_t bytes);
+int av1mi_copy_d2h(void *dst, const void *src, size_t bytes);
 int av1mi_device_sync(void);
 /* Stream-ordered variants and non-blocking streams, for callers that overlap several encodes on one device (the
  * serving loop): allocation / free from the device's stream-ordered pool, asynchronous host-to-device copy. */
