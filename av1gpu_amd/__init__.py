@@ -58,7 +58,7 @@ class _Times(C.Structure):
     _fields_ = [(n, C.c_float) for n in ("setup", "analysis", "dp", "recon", "pack", "download", "total")] + \
                [(n, C.c_uint32) for n in ("steps", "sbs", "blocks", "skipBlocks", "cflBlocks", "rangeFixes")] + \
                [("deviceBytes", C.c_size_t), ("schedule", C.c_uint32), ("queueBlocks", C.c_uint32), ("anStreams", C.c_uint32),
-                ("graph", C.c_uint32), ("tuned", C.c_uint32)]
+                ("graph", C.c_uint32), ("tuned", C.c_uint32), ("flopsAnalysis", C.c_uint64), ("flopsRecon", C.c_uint64)]
 
 
 _lib = None
@@ -97,6 +97,10 @@ def lib():
         L.av1mi_synth_image.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.POINTER(C.c_int)]
         L.av1mi_set_search_export.argtypes = [C.c_void_p, C.c_int]
+        L.av1mi_set_work_count.argtypes = [C.c_void_p, C.c_int]
+        L.av1mi_set_ext_candidates.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_char_p, C.c_int]
+        L.av1mi_set_block_lists.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int]
+        L.av1mi_copy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         L.av1mi_blockres_export.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_size_t]
         L.av1mi_blockres_export.restype = C.c_size_t
         L.av1mi_sbj_export.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]
@@ -210,6 +214,8 @@ class GpTimes:
     anStreams: int = 0
     graph: int = 0
     tuned: int = 0
+    flopsAnalysis: int = 0
+    flopsRecon: int = 0
 
 
 def check_partitions(P: GpParams):
@@ -429,6 +435,30 @@ class GpEncoder:
                 lib().av1mi_frame_out_free(C.byref(outs[i]))
         times = GpTimes(**{k: getattr(t, k) for k, _ in _Times._fields_})
         return res, times
+
+    def set_ext_candidates(self, cand, off, topk):
+        """External screener (GpBatch::extCand / extOff / extTopK): cand {shape: device pointer of av1mi_ext_cand
+        entries}, off: device pointer of 22 * n int64 offsets (n frames per call), topk entries per block; None clears."""
+        arr = (C.c_void_p * 22)()
+        for s, p in (cand or {}).items():
+            arr[s] = p
+        err = C.create_string_buffer(512)
+        if lib().av1mi_set_ext_candidates(self._h, arr if cand else None, off, topk, err, 512) != 0:
+            raise ValueError("set_ext_candidates: " + err.value.decode())
+
+    def set_block_lists(self, lists, counts):
+        """Block lists (GpAnX): lists / counts {shape: device pointer}; None clears."""
+        la, ca = (C.c_void_p * 22)(), (C.c_void_p * 22)()
+        for s, p in (lists or {}).items():
+            la[s] = p
+            ca[s] = counts[s]
+        err = C.create_string_buffer(512)
+        if lib().av1mi_set_block_lists(self._h, la if lists else None, ca if lists else None, err, 512) != 0:
+            raise ValueError("set_block_lists: " + err.value.decode())
+
+    def set_work_count(self, on=True):
+        """Count the forward-transform FLOPs of each later encode (GpTimes.flopsAnalysis / flopsRecon)."""
+        lib().av1mi_set_work_count(self._h, 1 if on else 0)
 
     def set_search_export(self, on=True):
         """Keep the search internals of each later encode (analysis records, superblock costs, leaves)."""

@@ -94,13 +94,29 @@ AV1M_HD constexpr int tx_class(int t) {
     return (t == V_DCT || t == V_ADST || t == V_FLIPADST) ? TX_CLASS_VERT : (t == H_DCT || t == H_ADST || t == H_FLIPADST) ? TX_CLASS_HORIZ : TX_CLASS_2D;
 }
 // 1-D kinds of the row (horizontal) and column (vertical) passes; flips
-AV1M_HD int row_kind(int t) {
+AV1M_HD constexpr int row_kind(int t) {
     return (t == DCT_DCT || t == ADST_DCT || t == FLIPADST_DCT || t == H_DCT) ? K_DCT
          : (t == IDTX || t == V_DCT || t == V_ADST || t == V_FLIPADST) ? K_IDT : K_ADST;
 }
-AV1M_HD int col_kind(int t) {
+AV1M_HD constexpr int col_kind(int t) {
     return (t == DCT_DCT || t == DCT_ADST || t == DCT_FLIPADST || t == V_DCT) ? K_DCT
          : (t == IDTX || t == H_DCT || t == H_ADST || t == H_FLIPADST) ? K_IDT : K_ADST;
+}
+// Algorithmic forward-transform FLOPs of one evaluation of a W x H transform block of type t (work accounting): 2 per
+// fmaf of the column pass (W columns x min(H, 32) kept outputs x H taps) and of the row pass (min(H, 32) rows x
+// min(W, 32) kept outputs x W taps), 1 per multiply of an identity kind's pass and of the row output scale.
+AV1M_HD constexpr uint32_t fwd_flops(int W, int H, int t) {
+    const uint32_t tw = W > 32 ? 32 : W, th = H > 32 ? 32 : H;
+    const uint32_t col = col_kind(t) == K_IDT ? (uint32_t)W * th : 2u * W * th * H;
+    const uint32_t row = row_kind(t) == K_IDT ? th * tw : 2u * th * tw * W;
+    return col + row + th * tw;
+}
+// the same summed over the types of transform set `set` other than DCT_DCT
+AV1M_HD constexpr uint32_t set_flops(int W, int H, int set) {
+    uint32_t f = 0;
+    for (int t = 1; t < 16; t++)
+        if (av1t::Tx_Type_In_Set_Intra[set][t]) f += fwd_flops(W, H, t);
+    return f;
 }
 AV1M_HD int flip_ud(int t) { return t == FLIPADST_DCT || t == FLIPADST_ADST || t == V_FLIPADST || t == FLIPADST_FLIPADST; }
 AV1M_HD int flip_lr(int t) { return t == DCT_FLIPADST || t == ADST_FLIPADST || t == H_FLIPADST || t == FLIPADST_FLIPADST; }
@@ -602,6 +618,25 @@ struct Batch {
     uint32_t *stats;             // [8]: blocks, skip blocks, CfL blocks, range repairs
     const uint32_t *lnMask;      // [nSb][12] learned pruning: expanded nodes (words 0-2), screened luma modes (3), nodes
                                  // keeping their own shapes (4-6), their 4-way shapes (8-10); null = exhaustive
+    unsigned long long *flops;   // [64] or null: forward-transform FLOPs (fwd_flops) of the evaluations, analysis in
+                                 // [0, 32), closed loop in [32, 64) (spread over slots to keep the atomics apart)
+};
+
+// an external screener's candidate (GpExtCand, encoder/src/gp.h:136-143; av1mi_ext_cand)
+struct ExtCand { float J; uint8_t mode; int8_t delta; uint8_t tx, pad; };
+// the analysis hooks of one shape: the external candidates of its blocks (null: screening) and the list of blocks to
+// analyse (null: all); GpBatch::extCand / extOff / extTopK (gp.h:164-171), GpAnX (an_stage.h:23-27)
+struct AnHook {
+    const ExtCand *ext;          // [blocks][extTopK], entry of block (img, local) at (extOff[img] + local) * extTopK
+    const int64_t *extOff;       // [nimg] this shape's first entry per image
+    int extTopK;
+    const uint32_t *list;        // shape-global block indices (shStart order)
+    const uint32_t *count;       // [1] entries of list
+};
+// host side: the hooks of every shape (null: none) and each shape's first record in Batch::blk
+struct AnHooks {
+    const AnHook *hook;
+    const uint32_t *base;
 };
 
 // the device work queue of the persistent closed-loop schedule

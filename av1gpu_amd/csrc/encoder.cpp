@@ -20,8 +20,9 @@
 
 namespace av1m {
 void build_tables(DevTables &T);
-void launch_analysis(const Batch &B, const SearchParams &P, const uint32_t *shTotal, int bd, hipStream_t st);
-void launch_analysis_on(const Batch &B, const SearchParams &P, const uint32_t *shTotal, int bd, const hipStream_t *sts, int ns);
+void launch_analysis(const Batch &B, const SearchParams &P, const uint32_t *shTotal, int bd, const AnHooks &h, hipStream_t st);
+void launch_analysis_on(const Batch &B, const SearchParams &P, const uint32_t *shTotal, int bd, const AnHooks &h, const hipStream_t *sts,
+                        int ns);
 void launch_zero(void *p, size_t bytes, hipStream_t st);
 void launch_dp(const Batch &B, const SearchParams &P, hipStream_t st);
 void launch_recon(const Batch &B, const SearchParams &P, const uint32_t *dSbList, uint32_t n, int bd, hipStream_t st);
@@ -126,6 +127,14 @@ struct av1mi_encoder {
     size_t stageBytes = 0;
     std::vector<void *> oldStages;    // outgrown staging buffers, freed at destroy: hipHostFree would wait for the whole
                                       // device, i.e. for the other encoders' work in a multi-stream server
+    bool countWork = false;           // count the forward-transform FLOPs of the evaluations (av1mi_times.flops*)
+    // analysis hooks (device pointers of the caller): external candidates per shape with their per-(shape, image) offsets,
+    // block lists per shape
+    const ExtCand *ext[NSHAPES] = {};
+    const int64_t *extOff = nullptr;
+    int extTopK = 0;
+    const uint32_t *list[NSHAPES] = {}, *listCount[NSHAPES] = {};
+    bool anyList = false;
     bool dpExport = false;            // keep the DP's per-node relative costs of the last call (training labels)
     std::vector<float> dpRel;
     void *scratch = nullptr;          // this device's copy of the no-allocation scratch array, and its size
@@ -293,11 +302,59 @@ int av1mi_set_shortlist(av1mi_encoder *e, const float *params, float confidence,
     e->ln = nullptr;
     if (!params) return 0;
     if (!(confidence > 0.5f && confidence <= 1.0f)) { snprintf(err, errlen, "confidence %g outside (0.5, 1]", confidence); return -1; }
+    if (e->anyList) { snprintf(err, errlen, "the learned pruning and block lists do not combine"); return -1; }
     e->ln = ln_new(params);
     if (!e->ln) { snprintf(err, errlen, "HIP: could not upload the pruning network"); return -1; }
     e->lnConf = confidence;
     const char *c = getenv("AV1MI_LN_CHUNK");
     e->lnChunk = c && atoi(c) > 0 ? (uint32_t)atoi(c) : 65536u;
+    return 0;
+}
+
+int av1mi_set_ext_candidates(av1mi_encoder *e, const av1mi_ext_cand *const *cand, const int64_t *off, int topK, char *err,
+                             int errlen) {
+    static_assert(sizeof(av1mi_ext_cand) == sizeof(ExtCand) && sizeof(ExtCand) == 8, "GpExtCand is 8 bytes");
+    bool any = false;
+    for (int s = 0; cand && s < NSHAPES; s++) any = any || cand[s];
+    if (any && (!off || topK < 1 || topK > 32)) {
+        snprintf(err, errlen, "external candidates need the offsets and 1..32 candidates per block (topK %d)", topK);
+        return -1;
+    }
+    drop_plan(e);
+    e->gen++;
+    for (int s = 0; s < NSHAPES; s++) e->ext[s] = any ? (const ExtCand *)cand[s] : nullptr;
+    e->extOff = any ? off : nullptr;
+    e->extTopK = any ? topK : 0;
+    return 0;
+}
+
+int av1mi_set_block_lists(av1mi_encoder *e, const uint32_t *const *list, const uint32_t *const *count, char *err, int errlen) {
+    av1mi_params P = e->cfg.P;
+    bool any = false;
+    for (int s = 0; list && s < NSHAPES; s++)
+        if (list[s]) {
+            if (!count || !count[s]) { snprintf(err, errlen, "block list of shape %d without its count", s); return -1; }
+            P.shapeMask &= ~(1 << s);
+            any = true;
+        }
+    // every superblock must keep a partition of finite cost whatever the lists leave out
+    if (any && av1mi_check_partitions(&P, err, errlen)) {
+        const std::string m = std::string("block lists: the shapes without a list must leave every superblock a partition (") + err + ")";
+        snprintf(err, errlen, "%s", m.c_str());
+        return -1;
+    }
+    if (any && e->ln) { snprintf(err, errlen, "block lists and the learned pruning do not combine"); return -1; }
+    drop_plan(e);
+    e->gen++;
+    for (int s = 0; s < NSHAPES; s++) { e->list[s] = any ? list[s] : nullptr; e->listCount[s] = any && list[s] ? count[s] : nullptr; }
+    e->anyList = any;
+    return 0;
+}
+
+int av1mi_set_work_count(av1mi_encoder *e, int on) {
+    drop_plan(e);
+    e->gen++;
+    e->countWork = on != 0;
     return 0;
 }
 
@@ -448,7 +505,9 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
         mapTotal += 5 * mi + 2 * (mi + 2 * c1);
     }
     uint32_t *dStats, *dDone, *dQctl;
+    unsigned long long *dFlops = nullptr;
     A.add(dStats, 8); A.add(dDone, nSb); A.add(dQctl, 2);
+    if (enc->countWork) A.add(dFlops, 64);
     // everything so far starts each call at zero (reconstruction, maps, counters, done flags): one fill kernel; the
     // padding beyond the coded area is never reconstructed, so the returned planes are fully defined
     const size_t zeroBytes = A.total;
@@ -547,6 +606,17 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
     B.leafCount = dLeafCount; B.leaves = dLeaves; B.sbJ = dSbJ; B.sbOut = dSbOut; B.oBlocks = dOB; B.oBlockTxb = dOBT;
     B.oTxb = dOT; B.oCoeffs = dOC; B.stats = dStats; B.txbCap = txbCap; B.coefCap = coefCap; B.dpRel = dDpRel;
     if (enc->ln) B.lnMask = dLnMask;
+    B.flops = dFlops;
+    // analysis hooks of this call: each shape's slice of the external offsets, and each shape's first record
+    AnHook hooks[NSHAPES];
+    uint32_t shBase[NSHAPES];
+    for (int s = 0; s < NSHAPES; s++) {
+        hooks[s] = AnHook{};
+        if (enc->ext[s]) { hooks[s].ext = enc->ext[s]; hooks[s].extOff = enc->extOff + (size_t)s * n; hooks[s].extTopK = enc->extTopK; }
+        if (enc->list[s]) { hooks[s].list = enc->list[s]; hooks[s].count = enc->listCount[s]; }
+        shBase[s] = imgs[0].shOff[s];
+    }
+    const AnHooks anHooks{hooks, shBase};
     // the device part: zeroing, the pruning network (its time counts as analysis), the analysis shapes over anS
     // streams (fork / join by events), the DP, the closed loop.  Eager on the caller's stream, or captured once per batch
     // shape on the capture stream and replayed; no memset nodes, only kernels.
@@ -558,13 +628,13 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
             hipStream_t sts[4] = {s, enc->aux[0], enc->aux[1], enc->aux[2]};
             if (hipEventRecord(enc->evFork, s) != hipSuccess) return -1;
             for (int k = 1; k < anS; k++) if (hipStreamWaitEvent(sts[k], enc->evFork, 0) != hipSuccess) return -1;
-            launch_analysis_on(B, P, shTotal, bd, sts, anS);
+            launch_analysis_on(B, P, shTotal, bd, anHooks, sts, anS);
             for (int k = 1; k < anS; k++) {
                 if (hipEventRecord(enc->evJoin[k - 1], sts[k]) != hipSuccess || hipStreamWaitEvent(s, enc->evJoin[k - 1], 0) != hipSuccess)
                     return -1;
             }
         } else {
-            launch_analysis(B, P, shTotal, bd, s);
+            launch_analysis(B, P, shTotal, bd, anHooks, s);
         }
         if (events) (void)hipEventRecord(enc->ev[2], s);
         launch_dp(B, P, s);
@@ -624,6 +694,8 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
     HCK(hipMemcpyAsync(so.data(), dSbOut, (size_t)nSb * sizeof(SbOut), hipMemcpyDeviceToHost, st));
     HCK(hipMemcpyAsync(stats, dStats, sizeof stats, hipMemcpyDeviceToHost, st));
     HCK(hipMemcpyAsync(qctl, dQctl, sizeof qctl, hipMemcpyDeviceToHost, st));
+    unsigned long long flops[64] = {0};
+    if (dFlops) HCK(hipMemcpyAsync(flops, dFlops, sizeof flops, hipMemcpyDeviceToHost, st));
     if (enc->dpExport) {
         enc->dpRel.resize((size_t)nSb * 255);
         HCK(hipMemcpyAsync(enc->dpRel.data(), dDpRel, enc->dpRel.size() * 4, hipMemcpyDeviceToHost, st));
@@ -781,6 +853,7 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
             for (int k = 1; k < 4; k++) if (tu->ms[k] < tu->ms[tu->best]) tu->best = k;
     }
     tm.blocks = stats[0]; tm.skipBlocks = stats[1]; tm.cflBlocks = stats[2]; tm.rangeFixes = stats[3];
+    for (int k = 0; k < 32; k++) { tm.flopsAnalysis += flops[k]; tm.flopsRecon += flops[32 + k]; }
     if (t) *t = tm;
     return 0;
 }

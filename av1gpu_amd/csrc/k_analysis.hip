@@ -50,9 +50,11 @@ __device__ __forceinline__ int ol_decoded(int px, int py, int x, int y, const Ti
 // Occupancy target 4 waves per SIMD: the compiler then keeps the kernels under 128 VGPRs, and the extra waves hide the
 // LDS and table latency (measured on MI355X: analysis of the 12 MP image 68.5 -> 56.7 ms).  The shapes of 32x32 and
 // up are limited by LDS instead (the compiler notes that it cannot reach the target there; the build silences it).
-template <int W, int H, int BS, typename Px>
+// HOOKS: the instance that reads an external screener's candidates and / or a block list (AnHook); the default instance
+// has neither, so their registers do not weigh on it.
+template <int W, int H, int BS, typename Px, bool HOOKS>
 __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_waves_per_eu(4))) k_analysis(Batch B, SearchParams P,
-                                                                                                      uint32_t nTotal) {
+                                                                                                      uint32_t nTotal, AnHook hk) {
     constexpr int NG = an_groups(W, H), TXS = tx_of(W, H);
     constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
     constexpr int NSUB = (W / 4) * (H / 4), NSX = W / 4;
@@ -62,7 +64,11 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
     const int lane = glane();
     const uint32_t *starts = B.shStart + (size_t)BS * (B.nimg + 1);
     const int set = tx_set_intra(TXS);
-    for (uint32_t gb = blockIdx.x * NG + gid(); gb < nTotal; gb += gridDim.x * NG) {
+    // a block list (k_an_blank has set every record of the shape to "not analysed"): the groups take its entries
+    const uint32_t nItems = (HOOKS && hk.list) ? min(*hk.count, nTotal) : nTotal;
+    for (uint32_t item = blockIdx.x * NG + gid(); item < nItems; item += gridDim.x * NG) {
+        const uint32_t gb = (HOOKS && hk.list) ? hk.list[item] : item;
+        if (HOOKS && gb >= nTotal) continue;   // an entry outside the shape
         int lo = 0, hi = B.nimg - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -123,6 +129,28 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
             if (BS >= BLOCK_8X8 && is_dir(m)) b += C.angle_delta[m - V_PRED][cand_delta(c) + 3];
             return b;
         };
+        const int K = min(P.K, MAXK);
+        int top[MAXK];
+        int nK = 0;
+        // an external screener's candidates, best first, replace the screening (k_analysis.cuh:586-600): entries with a
+        // mode of 13 or more are skipped, deltas kept for directional modes of blocks from 8x8 (outside -3..3: 0); a block
+        // with no usable entry is screened as without the hook
+        if (HOOKS && hk.ext) {
+            const ExtCand *ec = hk.ext + (size_t)(hk.extOff[lo] + li) * hk.extTopK;
+            const ExtCand c = lane < hk.extTopK ? ec[lane] : ExtCand{0.f, 255, 0, 0, 0};
+            const bool ok = c.mode < 13;
+            const uint32_t okm = (uint32_t)(__ballot(ok) >> (threadIdx.x & 32));   // this group's half of the wave
+            const int rank = __popc(okm & ((1u << lane) - 1));
+            if (ok && rank < MAXK) {
+                const int d = (BS >= BLOCK_8X8 && is_dir(c.mode) && c.delta >= -3 && c.delta <= 3) ? c.delta : 0;
+                S.cand[rank] = cand_pack(c.mode, d);
+            }
+            nK = min(min(__popc(okm), MAXK), K);
+#pragma unroll
+            for (int k = 0; k < MAXK; k++) top[k] = k;
+            gsync();
+        }
+        if (!nK) {
         // ---- SATD screening (unfiltered edges) ----
         S.satdI[lane] = 0;
         // the screened modes in mode order (all 13 without a shortlist)
@@ -192,14 +220,12 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
             screen(13, 13 + nd);
             if (lane >= 13 && lane < 13 + nd) js = fmaf(lamS, mbits(S.cand[lane]), (float)S.satdI[lane] * 0.25f);
         }
-        const int K = min(P.K, MAXK);
-        int top[MAXK];
-        int nK = 0;
         for (int k = 0; k < K; k++) {
             top[k] = gargmin(js, lane);
             if (top[k] < 0) break;
             if (lane == top[k]) js = INF;
             nK++;
+        }
         }
         // ---- RD of the top-K (DCT_DCT), then the tx types of the set for the winner ----
         const int hasC = has_chroma(BS, y >> 2, x >> 2, I.mono, I.ss);
@@ -209,7 +235,7 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
         e.bd = bd; e.rnd = P.rnd; e.zbin = P.zbin; e.resEnergy = 0.f; e.C = &C; e.intraDir = 0; e.txType = 0;
         float bestJ = INF;
         int bestK = 0, bestTx = DCT_DCT, resK = -1;
-        const int nSteps = nK + ((P.txSearch && set != TX_SET_DCTONLY) ? 15 : 0);
+        const int nSteps = nK ? nK + ((P.txSearch && set != TX_SET_DCTONLY) ? 15 : 0) : 0;
         for (int st = 0; st < nSteps; st++) {
             const int k = st < nK ? st : bestK, t = st < nK ? DCT_DCT : st - nK + 1;
             if (t != DCT_DCT && !tx_in_set(set, t)) continue;
@@ -251,48 +277,73 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
             for (int k = 0; k < nK; k++)
                 if (k != bestK) out.cand[n++] = (uint8_t)S.cand[top[k]];
             out.ncand = (uint8_t)n;
+            // work accounting: nK evaluations with DCT_DCT, then the set's other types (the analysis's evaluations)
+            constexpr uint32_t FDCT = fwd_flops(W, H, DCT_DCT), FSET = set_flops(W, H, tx_set_intra(TXS));
+            if (B.flops && nK) atomicAdd(&B.flops[gb & 31], (unsigned long long)(nK * FDCT + (nSteps > nK ? FSET : 0u)));
         }
         gsync();
     }
 }
 
+// every record of a shape "not analysed" (J = INF, no candidate): the blocks a block list leaves out
+__global__ void k_an_blank(BlockRes *blk, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        BlockRes r;
+        r.J = INF;
+        for (int k = 0; k < MAXK; k++) r.cand[k] = 0;
+        r.ncand = 0;
+        r.tx = 0;
+        blk[i] = r;
+    }
+}
+
 template <int W, int H, int BS>
-static void launch_an(const Batch &B, const SearchParams &P, uint32_t n, int bd, hipStream_t st) {
+static void launch_an(const Batch &B, const SearchParams &P, uint32_t n, int bd, const AnHooks &h, hipStream_t st) {
     if (!n) return;
     constexpr int NG = an_groups(W, H);
+    const AnHook hk = h.hook ? h.hook[BS] : AnHook{};
+    if (hk.list) hipLaunchKernelGGL(k_an_blank, dim3((n + 255) / 256), dim3(256), 0, st, B.blk + h.base[BS], n);
     uint32_t grid = (n + NG - 1) / NG;
     if (grid > (1u << 30)) grid = 1u << 30;
-    if (bd == 10) hipLaunchKernelGGL((k_analysis<W, H, BS, uint16_t>), dim3(grid), dim3(NG * 32), 0, st, B, P, n);
-    else hipLaunchKernelGGL((k_analysis<W, H, BS, uint8_t>), dim3(grid), dim3(NG * 32), 0, st, B, P, n);
+    const bool hooked = hk.ext || hk.list;
+    if (bd == 10) {
+        if (hooked) hipLaunchKernelGGL((k_analysis<W, H, BS, uint16_t, true>), dim3(grid), dim3(NG * 32), 0, st, B, P, n, hk);
+        else hipLaunchKernelGGL((k_analysis<W, H, BS, uint16_t, false>), dim3(grid), dim3(NG * 32), 0, st, B, P, n, hk);
+    } else {
+        if (hooked) hipLaunchKernelGGL((k_analysis<W, H, BS, uint8_t, true>), dim3(grid), dim3(NG * 32), 0, st, B, P, n, hk);
+        else hipLaunchKernelGGL((k_analysis<W, H, BS, uint8_t, false>), dim3(grid), dim3(NG * 32), 0, st, B, P, n, hk);
+    }
 }
 
 // all shapes of the batch (shTotal[s]: blocks of shape s over the batch; 0 = not analysed), largest first, the i-th
 // shape on sts[i % ns] (the shapes are independent: several streams fill each other's launch tails)
-void launch_analysis_on(const Batch &B, const SearchParams &P, const uint32_t *shTotal, int bd, const hipStream_t *sts, int ns) {
+void launch_analysis_on(const Batch &B, const SearchParams &P, const uint32_t *shTotal, int bd, const AnHooks &h, const hipStream_t *sts,
+                        int ns) {
     int i = 0;
     auto st = [&]() { return sts[(i++) % ns]; };
-    launch_an<64, 64, BLOCK_64X64>(B, P, shTotal[BLOCK_64X64], bd, st());
-    launch_an<64, 32, BLOCK_64X32>(B, P, shTotal[BLOCK_64X32], bd, st());
-    launch_an<32, 64, BLOCK_32X64>(B, P, shTotal[BLOCK_32X64], bd, st());
-    launch_an<64, 16, BLOCK_64X16>(B, P, shTotal[BLOCK_64X16], bd, st());
-    launch_an<16, 64, BLOCK_16X64>(B, P, shTotal[BLOCK_16X64], bd, st());
-    launch_an<32, 32, BLOCK_32X32>(B, P, shTotal[BLOCK_32X32], bd, st());
-    launch_an<32, 16, BLOCK_32X16>(B, P, shTotal[BLOCK_32X16], bd, st());
-    launch_an<16, 32, BLOCK_16X32>(B, P, shTotal[BLOCK_16X32], bd, st());
-    launch_an<32, 8, BLOCK_32X8>(B, P, shTotal[BLOCK_32X8], bd, st());
-    launch_an<8, 32, BLOCK_8X32>(B, P, shTotal[BLOCK_8X32], bd, st());
-    launch_an<16, 16, BLOCK_16X16>(B, P, shTotal[BLOCK_16X16], bd, st());
-    launch_an<16, 8, BLOCK_16X8>(B, P, shTotal[BLOCK_16X8], bd, st());
-    launch_an<8, 16, BLOCK_8X16>(B, P, shTotal[BLOCK_8X16], bd, st());
-    launch_an<16, 4, BLOCK_16X4>(B, P, shTotal[BLOCK_16X4], bd, st());
-    launch_an<4, 16, BLOCK_4X16>(B, P, shTotal[BLOCK_4X16], bd, st());
-    launch_an<8, 8, BLOCK_8X8>(B, P, shTotal[BLOCK_8X8], bd, st());
-    launch_an<8, 4, BLOCK_8X4>(B, P, shTotal[BLOCK_8X4], bd, st());
-    launch_an<4, 8, BLOCK_4X8>(B, P, shTotal[BLOCK_4X8], bd, st());
-    launch_an<4, 4, BLOCK_4X4>(B, P, shTotal[BLOCK_4X4], bd, st());
+    launch_an<64, 64, BLOCK_64X64>(B, P, shTotal[BLOCK_64X64], bd, h, st());
+    launch_an<64, 32, BLOCK_64X32>(B, P, shTotal[BLOCK_64X32], bd, h, st());
+    launch_an<32, 64, BLOCK_32X64>(B, P, shTotal[BLOCK_32X64], bd, h, st());
+    launch_an<64, 16, BLOCK_64X16>(B, P, shTotal[BLOCK_64X16], bd, h, st());
+    launch_an<16, 64, BLOCK_16X64>(B, P, shTotal[BLOCK_16X64], bd, h, st());
+    launch_an<32, 32, BLOCK_32X32>(B, P, shTotal[BLOCK_32X32], bd, h, st());
+    launch_an<32, 16, BLOCK_32X16>(B, P, shTotal[BLOCK_32X16], bd, h, st());
+    launch_an<16, 32, BLOCK_16X32>(B, P, shTotal[BLOCK_16X32], bd, h, st());
+    launch_an<32, 8, BLOCK_32X8>(B, P, shTotal[BLOCK_32X8], bd, h, st());
+    launch_an<8, 32, BLOCK_8X32>(B, P, shTotal[BLOCK_8X32], bd, h, st());
+    launch_an<16, 16, BLOCK_16X16>(B, P, shTotal[BLOCK_16X16], bd, h, st());
+    launch_an<16, 8, BLOCK_16X8>(B, P, shTotal[BLOCK_16X8], bd, h, st());
+    launch_an<8, 16, BLOCK_8X16>(B, P, shTotal[BLOCK_8X16], bd, h, st());
+    launch_an<16, 4, BLOCK_16X4>(B, P, shTotal[BLOCK_16X4], bd, h, st());
+    launch_an<4, 16, BLOCK_4X16>(B, P, shTotal[BLOCK_4X16], bd, h, st());
+    launch_an<8, 8, BLOCK_8X8>(B, P, shTotal[BLOCK_8X8], bd, h, st());
+    launch_an<8, 4, BLOCK_8X4>(B, P, shTotal[BLOCK_8X4], bd, h, st());
+    launch_an<4, 8, BLOCK_4X8>(B, P, shTotal[BLOCK_4X8], bd, h, st());
+    launch_an<4, 4, BLOCK_4X4>(B, P, shTotal[BLOCK_4X4], bd, h, st());
 }
-void launch_analysis(const Batch &B, const SearchParams &P, const uint32_t *shTotal, int bd, hipStream_t st) {
-    launch_analysis_on(B, P, shTotal, bd, &st, 1);
+void launch_analysis(const Batch &B, const SearchParams &P, const uint32_t *shTotal, int bd, const AnHooks &h, hipStream_t st) {
+    launch_analysis_on(B, P, shTotal, bd, h, &st, 1);
 }
 
 // ---------------------------------------------------------------------------------------------------------------

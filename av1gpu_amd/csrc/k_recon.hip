@@ -83,6 +83,8 @@ struct RcShared {
     int satdI[32], uvc[32];
     float uvJ[32];                           // screening costs of the UV candidates
     int nTxb, nCoef, errFlag, nzFlag, eobOut, culOut;
+    int flopsOn;                             // count the forward-transform work (Batch::flops set)
+    unsigned long long flops;                // of the superblock so far
     int am, lm, mode, delta, uvMode, uvDelta, nuv, bsize;
     uint32_t qItem;                          // persistent schedule: the superblock taken
 };
@@ -543,6 +545,12 @@ __device__ void wg_run_tasks(lds<RcShared> &S, const Img &I, const DevTables &T,
         gsync();
     }
     __syncthreads();
+    if (S.flopsOn && threadIdx.x < GW) {   // work accounting: the forward transforms of the tasks
+        int f = 0;
+        for (int t = lane; t < S.nTask; t += GW) f += (int)fwd_flops(W, H, S.taskT[t]);
+        f = gsumi(f);
+        if (lane == 0) S.flops += (unsigned long long)f;
+    }
     if (pickWinner && threadIdx.x < GW) {   // the first minimum over the tasks (at most 64)
         const int n = S.nTask;
         float v = lane < n ? S.taskJ[lane] : INF;
@@ -1089,7 +1097,7 @@ __device__ void recon_sb(lds<RcShared> &S, const Batch &B, const SearchParams &P
         if (yy == (16 >> ss) && xx == -1) v = 0;
         S.bdm[p][yy + 1][xx + 1] = v;
     }
-    if (tid == 0) { S.nTxb = 0; S.nCoef = 0; }
+    if (tid == 0) { S.nTxb = 0; S.nCoef = 0; S.flopsOn = B.flops != nullptr; S.flops = 0; }
     __syncthreads();
     const float lambda = T.lambda[I.bd == 10][I.qindex] * P.lambdaMul;
     const int nLeaves = B.leafCount[sb];
@@ -1156,6 +1164,7 @@ __device__ void recon_sb(lds<RcShared> &S, const Batch &B, const SearchParams &P
         SbOut o;
         o.nBlocks = (uint32_t)nLeaves; o.nTxb = (uint32_t)S.nTxb; o.nCoeffs = (uint32_t)S.nCoef; o.pad = 0;
         B.sbOut[sb] = o;
+        if (S.flopsOn && S.flops) atomicAdd(&B.flops[32 + (sb & 31)], S.flops);
     }
 }
 

@@ -261,7 +261,10 @@ def rd_curve(enc, images, qs, bd=8, yuv444=False):
         t0 = time.perf_counter()
         out, _ = enc.encode(frames)
         secs += time.perf_counter() - t0
-        bits.append(sum(8 * len(A.write_bitstream(o, avif=False)) for o in out))
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(min(16, max(1, len(out)))) as ex:   # the writer releases the GIL (ctypes)
+            sizes = list(ex.map(lambda o: len(A.write_bitstream(o, avif=False)), out))
+        bits.append(sum(8 * n for n in sizes))
         psnr.append(float(np.mean([psnr_y(p[0], o.recon[0], W, H, bd) for (p, W, H), o in zip(images, out)])))
         for f in frames:
             f.close()

@@ -6,18 +6,23 @@ partition DP -> closed-loop reconstruction -> pack -> records on the host) over 
 resident in device memory, the shipped network (av1gpu_amd/shortlist_params.npy) at confidence 0.8.  Beside it:
   plain_single_image   the plain exhaustive search on one 4000 x 3000 image (seed 1000) at qindex 120, the path the
                        ratio vs_plain_single_image divides by;
+  plain_image_by_image the plain exhaustive search over the same 256-image batch, one encode call per image (the
+                       ratio vs_plain_image_by_image divides by it);
   batched_exhaustive   the same batch without the network;
   bd_rate              BD-rate (PSNR-Y, bits of this build's own writer) of the learned search against the exhaustive
-                       one on the batch's images of at most 2 MP (first 16), qindex 60/100/140/180;
+                       one on 12 batch images stratified by size (0.18-24 MP), qindex 60/100/140/180;
   learned_10bit_444    the same batch at 10-bit 4:4:4 (learned and exhaustive, with its BD-rate);
   serving              Poisson arrivals into the continuous-batching server on the batched+learned path, p50 / p99;
-  cpu_baseline         the CPU restatement (oracle/liborc.so) on the whole 4000 x 3000 image (about 10 s).
+  cpu_baseline         the CPU restatement (oracle/liborc.so): warm-up + median of 5 on the 512 x 512 configuration
+                       (1 thread and all threads) and on the whole 4000 x 3000 image (all threads, up to 16).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--batch 256] [--serve 64] [--conf 0.8]
 
 roofline: the dominant kernel of the headline step, the closed loop k_recon_queue (one persistent launch per encode):
 its algorithmic HBM bytes (source planes read once + reconstruction written once + 2 bytes per coefficient + 16 bytes
 per block and tx-block record) over its launch duration, timed with HIP events on the encode stream.
+roofline_valu: the same launch (and the analysis stage) against the FP32 vector peak, from the algorithmic
+forward-transform FLOPs of every RD evaluation, counted by the encoder's work accounting in an untimed encode.
 """
 import argparse
 import json
@@ -30,6 +35,26 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MP/s at qindex 120, batched+learned vs plain, at <=+1% BD-rate; p99 latency"
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TFLOPS = 157.3   # MI355X dense FP32 vector peak
+FLOPS_DEF = ("forward-transform FLOPs of every RD evaluation the search performs (2 per multiply-add of the column and "
+             "row passes, 1 per multiply of identity kinds and of the row scale; 64-point sizes keep 32 outputs), counted "
+             "by the encoder's work accounting in an untimed encode of the same workload")
+
+
+def valu_roofline(kernel, flops, ms):
+    """FP32 VALU roofline of a kernel (or stage) from its algorithmic forward-transform FLOPs per launch and its launch
+    duration."""
+    achieved = flops / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+    return {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 4), "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / VALU_PEAK_TFLOPS, "flops_per_launch": int(flops), "per_launch_ms": round(ms, 4)}
+
+
+def counted_flops(enc, frames):
+    """(analysis, closed loop) forward-transform FLOPs of one encode of frames (an untimed, counting call)."""
+    enc.set_work_count(True)
+    _, t = enc.encode(frames, want_recon=False)
+    enc.set_work_count(False)
+    return t.flopsAnalysis, t.flopsRecon
 
 
 def measured_traffic(kernel, name="counters_batched_learned_256.md"):
@@ -63,6 +88,7 @@ def main():
     ap.add_argument("--height", type=int, default=3000)
     ap.add_argument("--plain-steps", type=int, default=3, help="timed encodes of the plain leg (0 = skip it)")
     ap.add_argument("--no-exhaustive", action="store_true", help="skip the batched exhaustive leg")
+    ap.add_argument("--no-by-image", action="store_true", help="skip the plain image-by-image leg over the batch")
     ap.add_argument("--plain-only", action="store_true", help="run the plain leg alone (profiling)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-bd", action="store_true", help="skip the BD-rate leg")
@@ -106,6 +132,7 @@ def main():
         out = None          # the records are consumed: their host buffers go back before the next call
     A.lib().av1mi_device_sync()
     elapsed = time.perf_counter() - t0
+    fl_an, fl_rc = counted_flops(enc, frames)
     enc.close()
     value = mp * a.steps / elapsed
     avg = lambda k: round(sum(getattr(s, k) for s in stage) / len(stage), 3)
@@ -141,9 +168,16 @@ def main():
                      "traffic_source": "profiles/counters_batched_learned_256.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, "
                                        "factors from a 1 GiB copy)",
                      "per_launch_ms": round(per_launch_ms, 4), "alg_bytes_per_launch": round(ab / launches)},
+        "roofline_valu": {"definition": FLOPS_DEF,
+                          "closed_loop": valu_roofline(kernel, fl_rc / launches, per_launch_ms),
+                          "analysis_stage": valu_roofline("k_analysis (19 shape launches + the pruning network)", fl_an,
+                                                          avg("analysis"))},
         "plain_single_image": plain,
     }
     print("headline done: %.1f MP/s" % value, file=sys.stderr, flush=True)
+    if not a.no_by_image:
+        line["plain_image_by_image"] = plain_by_image_leg(A, frames, mp, a.schedule)
+        line["vs_plain_image_by_image"] = round(value / line["plain_image_by_image"]["value"], 3)
     if not a.no_exhaustive:
         line["batched_exhaustive"] = exhaustive_leg(A, frames, mp, a.schedule, value)
     for f in frames:
@@ -158,7 +192,7 @@ def main():
     if a.serve > 0:
         line["serving"] = serving_leg(A, a.serve, a.rate, (params, a.conf))
     if not a.no_cpu and planes is not None:
-        line["cpu_baseline"] = cpu_baseline(planes, a.width, a.height, 120)
+        line["cpu_baseline"] = cpu_baseline(A, planes, a.width, a.height, 120)
     print(json.dumps(line), flush=True)
 
 
@@ -177,6 +211,7 @@ def plain_leg(A, W, H, qindex, schedule, steps):
         stage.append(t)
     A.lib().av1mi_device_sync()
     elapsed = time.perf_counter() - t0
+    fl_an, fl_rc = counted_flops(enc, [frame])
     enc.close()
     frame.close()
     avg = lambda k: round(sum(getattr(s, k) for s in stage) / len(stage), 3)
@@ -192,9 +227,30 @@ def plain_leg(A, W, H, qindex, schedule, steps):
                       "sbs": last.sbs},
            "roofline": {"bound": "hbm", "kernel": "k_recon_queue" if last.schedule == 0 else "k_recon",
                         "achieved": round(achieved, 4), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                        "per_launch_ms": round(avg("recon") / launches, 4)}}
+                        "per_launch_ms": round(avg("recon") / launches, 4)},
+           "roofline_valu": {"closed_loop": valu_roofline("k_recon_queue" if last.schedule == 0 else "k_recon", fl_rc / launches,
+                                                          avg("recon") / launches),
+                             "analysis_stage": valu_roofline("k_analysis (19 shape launches)", fl_an, avg("analysis"))}}
     print("plain leg done: %.1f MP/s" % res["value"], file=sys.stderr, flush=True)
     return res, planes
+
+
+def plain_by_image_leg(A, frames, mp, schedule):
+    """The plain single-image path over the same batch: the exhaustive search, one encode call per image (each at its
+    own qindex), in batch order, records to the host, after one untimed warm-up call."""
+    enc = A.GpEncoder().init(A.GpConfig(qindex=120, schedule=schedule))
+    enc.encode(frames[:1], want_recon=False)
+    A.lib().av1mi_device_sync()
+    t0 = time.perf_counter()
+    for f in frames:
+        enc.encode([f], want_recon=False)
+    A.lib().av1mi_device_sync()
+    dt = time.perf_counter() - t0
+    enc.close()
+    res = {"workload": "plain exhaustive search, one encode call per image over the headline batch (%d images, %.1f MP)"
+                       % (len(frames), mp), "value": round(mp / dt, 4), "unit": "MP/s", "seconds": round(dt, 3)}
+    print("image-by-image leg done: %.1f MP/s" % res["value"], file=sys.stderr, flush=True)
+    return res
 
 
 def exhaustive_leg(A, frames, mp, schedule, learned_value):
@@ -215,9 +271,15 @@ def exhaustive_leg(A, frames, mp, schedule, learned_value):
     return res
 
 
-def bd_leg(A, learn, params, conf, spec, bplanes, n=16, max_mp=2.0, qs=(60, 100, 140, 180)):
-    """BD-rate of the learned search against the exhaustive one on the first n batch images of at most max_mp."""
-    idx = [i for i, (W, H, _, _) in enumerate(spec) if W * H <= max_mp * 1e6][:n]
+def stratified(spec, n):
+    """n batch images spread evenly over the size order (the smallest and the largest included)."""
+    order = sorted(range(len(spec)), key=lambda i: spec[i][0] * spec[i][1])
+    return sorted(set(order[int(round(k * (len(order) - 1) / (n - 1)))] for k in range(n)))
+
+
+def bd_leg(A, learn, params, conf, spec, bplanes, n=12, qs=(60, 100, 140, 180)):
+    """BD-rate of the learned search against the exhaustive one on n batch images stratified by size (0.18-24 MP)."""
+    idx = stratified(spec, n)
     images = [(bplanes[i], spec[i][0], spec[i][1]) for i in idx]
     curves = {}
     for name, c in (("exhaustive", None), ("learned", conf)):
@@ -231,6 +293,7 @@ def bd_leg(A, learn, params, conf, spec, bplanes, n=16, max_mp=2.0, qs=(60, 100,
                        curves["learned"]["psnr_y"])
     print("bd leg done: %.3f %%" % bd, file=sys.stderr, flush=True)
     return {"bd_rate_psnr_y_pct": round(float(bd), 3), "images": [spec[i][3] for i in idx],
+            "sizes_mp": [round(W * H / 1e6, 2) for _, W, H in images],
             "megapixels": round(sum(W * H for _, W, H in images) / 1e6, 3), "qindex": list(qs), "curves": curves}
 
 
@@ -335,19 +398,46 @@ def serving_leg(A, n, rate, shortlist):
             "workers": 4, "batches": len(srv.batches) - nb0, "learned_confidence": shortlist[1]}
 
 
-def cpu_baseline(planes, W, H, qindex):
-    """The oracle (the CPU restatement of the same exhaustive search) on the whole benchmark image (the plain leg's
-    4000 x 3000 workload, about 10 s of CPU work), all host threads up to 16."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(A, planes, W, H, qindex, runs=5):
+    """The oracle (the CPU restatement of the same exhaustive search, oracle/liborc.so) on this host: one warm-up, then
+    the median of `runs` timed encodes, for the 512 x 512 CPU configuration on one thread and on all threads (up to 16)
+    and for the whole 4000 x 3000 benchmark image on all threads (its value is `value`)."""
+    import statistics
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import refcheck
     threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1))
     o = refcheck.Oracle()
-    t0 = time.perf_counter()
-    o.encode(planes, W, H, qindex=qindex, threads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": round(W * H / 1e6 / dt, 5), "unit": "MP/s", "cores": threads, "kind": "port",
-            "sample": "oracle/liborc.so, the same search on the whole %dx%d benchmark image (seed 1000, qindex %d), %d "
-                      "threads, %.2f s" % (W, H, qindex, threads, dt)}
+
+    def timed(pl, w, h, nt):
+        o.encode(pl, w, h, qindex=qindex, threads=nt)
+        ts = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            o.encode(pl, w, h, qindex=qindex, threads=nt)
+            ts.append(time.perf_counter() - t0)
+        med = statistics.median(ts)
+        return {"threads": nt, "median_s": round(med, 4), "runs_s": [round(t, 4) for t in ts], "mp_per_s": round(w * h / 1e6 / med, 5)}
+
+    small = A.synth_image(1000, 512, 512)
+    legs = {"512x512_1_thread": timed(small, 512, 512, 1), "512x512_all_threads": timed(small, 512, 512, threads),
+            "%dx%d_all_threads" % (W, H): timed(planes, W, H, threads)}
+    big = legs["%dx%d_all_threads" % (W, H)]
+    print("cpu leg done: %.3f MP/s on %d threads" % (big["mp_per_s"], threads), file=sys.stderr, flush=True)
+    return {"value": big["mp_per_s"], "unit": "MP/s", "cores": threads, "kind": "port",
+            "sample": "oracle/liborc.so, the same exhaustive search on the whole %dx%d benchmark image (seed 1000, qindex %d), "
+                      "%d threads, median of %d after a warm-up" % (W, H, qindex, threads, runs),
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "legs": legs}
 
 
 if __name__ == "__main__":

@@ -123,6 +123,10 @@ typedef struct av1mi_times {
     size_t deviceBytes;
     uint32_t schedule, queueBlocks;   /* the closed-loop schedule used (0 queue, 1 steps) and its workgroups */
     uint32_t anStreams, graph, tuned; /* analysis streams used; 1 = replayed / captured graph; 1 = the tuner's choice */
+    /* with av1mi_set_work_count on: the algorithmic forward-transform FLOPs of the analysis (all shapes) and of the closed
+     * loop (2 per fmaf of the column and row passes, 1 per multiply of identity kinds and of the row scale; 64-point
+     * sizes keep 32 outputs); 0 otherwise */
+    uint64_t flopsAnalysis, flopsRecon;
 } av1mi_times;
 
 typedef struct av1mi_encoder av1mi_encoder;
@@ -210,12 +214,38 @@ int av1mi_frames_to_10bit(av1mi_frame_in *f, int n, void **buf, void *stream);
  * AV1MI_LN_CHUNK superblocks at a time (environment, read here; default 65536, ~100 KB of activations each). */
 int av1mi_set_shortlist(av1mi_encoder *enc, const float *params, float confidence, char *err, int errlen);
 size_t av1mi_ln_param_count(void);
+/* External screener (GpBatch::extCand / extOff / extTopK, encoder/src/gp.h:136-143 and 164-171, read by the analysis at
+ * encoder/src/k_analysis.cuh:586-608): for each later encode, the blocks of shape s take their candidates from cand[s]
+ * instead of the SATD screening: device memory, topK (1..32) av1mi_ext_cand per block, best first, the entry of block
+ * `local` (the shape's raster order within its frame) of frame img at (off[s * n + img] + local) * topK, where n is
+ * the number of frames of the call and off a device array of 22 * n offsets.  Entries with a mode of 13 or more are
+ * skipped, the delta is kept for directional modes of blocks from 8x8 (values outside -3..3 count as 0), the first K
+ * usable ones go to the RD; a block with none is screened as without the hook (the reference leaves that case
+ * undefined).  cand NULL or all entries NULL: no hook.  The arrays must stay valid while encodes use them. */
+typedef struct av1mi_ext_cand {
+    float J;
+    uint8_t mode;
+    int8_t delta;
+    uint8_t tx, pad;
+} av1mi_ext_cand;
+int av1mi_set_ext_candidates(av1mi_encoder *enc, const av1mi_ext_cand *const *cand, const int64_t *off, int topK, char *err,
+                             int errlen);
+/* Block lists (GpAnX::list / count, encoder/src/an_stage.h:23-27): for each later encode, shape s analyses only the blocks
+ * list[s] names (device array of shape-global indices: the shape's blocks of frame 0, then frame 1, ..., each frame in
+ * its raster order; count[s]: device pointer to the number of entries); its other blocks are not analysed (J = INF,
+ * no candidate).  Refused when the shapes without a list leave some superblock no partition (av1mi_check_partitions on
+ * them) and together with the learned pruning.  list NULL or all entries NULL: every block is analysed. */
+int av1mi_set_block_lists(av1mi_encoder *enc, const uint32_t *const *list, const uint32_t *const *count, char *err, int errlen);
 /* Training data for the pruning network (no counterpart in the reference): with on != 0, each later av1mi_encode keeps
  * the partition DP's per-node relative cost increases, superblocks in batch order, 255 floats each: (best cost without
  * the node's children - best) / best for the 85 square nodes, then the same without the node's N x N / 2:1 shapes,
  * then without its 4-way shapes.
  * av1mi_dp_export copies up to n floats of the last call and returns how many there are. */
 int av1mi_set_dp_export(av1mi_encoder *enc, int on);
+/* Work accounting (no counterpart in the reference): with on != 0 each later encode counts the forward-transform FLOPs
+ * of every evaluation the analysis and the closed loop perform into av1mi_times.flopsAnalysis / flopsRecon (a few device
+ * atomics per workgroup and superblock; records unchanged). */
+int av1mi_set_work_count(av1mi_encoder *enc, int on);
 size_t av1mi_dp_export(const av1mi_encoder *enc, float *out, size_t n);
 /* Search internals of the last encode, for parity checks (the reference keeps them in device memory only: GpBlockRes,
  * GpLeaf and the per-SB cost of gp.h:112-132).  With on != 0 each later av1mi_encode keeps them:

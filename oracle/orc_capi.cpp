@@ -37,8 +37,25 @@ static Params to_params(const OrcParams *p) {
 // Encode one frame.  planes: 8-bit (bd 8, uint8) or 16-bit (bd 10, uint16) samples, visible size (chroma
 // (W + ssx) >> ssx wide), strides in samples.  Returns a handle (free with orc_free).
 void *orc_encode(const void *y, const void *u, const void *v, const int *strides, int W, int H, int ssx, int mono, int bd,
+                 int qindex, int edgeFilter, const OrcParams *params, int threads);
+// The same with the analysis hooks: ext[s] (NULL or the external candidates of shape s, extTopK per block of the frame's
+// grid in raster order) and analyse[s] (NULL or one byte per block of the grid: 1 = analysed).
+void *orc_encode_hooks(const void *y, const void *u, const void *v, const int *strides, int W, int H, int ssx, int mono, int bd,
+                       int qindex, int edgeFilter, const OrcParams *params, int threads, const void *const *ext, int extTopK,
+                       const uint8_t *const *analyse);
+void *orc_encode(const void *y, const void *u, const void *v, const int *strides, int W, int H, int ssx, int mono, int bd,
                  int qindex, int edgeFilter, const OrcParams *params, int threads) {
+    return orc_encode_hooks(y, u, v, strides, W, H, ssx, mono, bd, qindex, edgeFilter, params, threads, nullptr, 0, nullptr);
+}
+void *orc_encode_hooks(const void *y, const void *u, const void *v, const int *strides, int W, int H, int ssx, int mono, int bd,
+                       int qindex, int edgeFilter, const OrcParams *params, int threads, const void *const *ext, int extTopK,
+                       const uint8_t *const *analyse) {
     Frame *F = new Frame();
+    for (int s = 0; s < 22; s++) {
+        F->ext[s] = ext ? (const ExtCand *)ext[s] : nullptr;
+        F->analyse[s] = analyse ? analyse[s] : nullptr;
+    }
+    F->extTopK = extTopK;
     const void *pl[3] = {y, u, v};
     std::vector<uint16_t> tmp[3];
     const uint16_t *src[3];
@@ -58,12 +75,14 @@ void *orc_encode(const void *y, const void *u, const void *v, const int *strides
 }
 void orc_free(void *h) { delete (Frame *)h; }
 
-// counts: n_blocks, n_txb, n_coeffs, MiCols, MiRows, aw, ah, tileCols, tileRows, stats[0..3]
+// counts: n_blocks, n_txb, n_coeffs, MiCols, MiRows, aw, ah, tileCols, tileRows, stats[0..3], forward-transform FLOPs
+// of the analysis and of the closed loop
 void orc_counts(void *h, int64_t *out) {
     const Frame &F = *(Frame *)h;
     out[0] = (int64_t)F.blocks.size(); out[1] = (int64_t)F.txb.size(); out[2] = (int64_t)F.coeffs.size();
     out[3] = F.MiCols; out[4] = F.MiRows; out[5] = F.aw; out[6] = F.ah; out[7] = F.tileCols; out[8] = F.tileRows;
     for (int i = 0; i < 4; i++) out[9 + i] = F.stats[i];
+    out[13] = (int64_t)F.flops[0].load(); out[14] = (int64_t)F.flops[1].load();
 }
 // record arrays (16-byte blocks / txbs, block_txb[n_blocks + 1], int16 coefficients)
 void orc_record(void *h, void *blocks, uint32_t *blockTxb, void *txb, int16_t *coeffs) {

@@ -308,6 +308,17 @@ float rate_of_levels(const int16_t *q, const EvalIn &e, int *eobOut) {   // dev_
     return lane_sum(bits);
 }
 
+// Work accounting, by its definition: the forward-transform FLOPs of one evaluation are 2 per multiply-add of the
+// column pass (W columns x min(H, 32) kept outputs x H taps) and of the row pass (min(H, 32) rows x min(W, 32) kept
+// outputs x W taps), 1 per multiply where a pass is an identity kind, and 1 per coefficient for the row scale.
+static uint64_t fwd_flops(const EvalIn &e) {
+    const uint64_t W = txw(e.txSz), H = txh(e.txSz), tw = W > 32 ? 32 : W, th = H > 32 ? 32 : H;
+    const uint64_t col = col_kind(e.txType) == K_IDT ? W * th : 2 * W * th * H;
+    const uint64_t row = row_kind(e.txType) == K_IDT ? th * tw : 2 * th * tw * W;
+    return col + row + th * tw;
+}
+static void count_eval(Frame &F, int phase, const EvalIn &e) { F.flops[phase].fetch_add(fwd_flops(e), std::memory_order_relaxed); }
+
 EvalOut eval_txb(const int16_t *res, const EvalIn &e, int16_t *q, float *coefOut) {
     const Tables &T = tables();
     const int t = e.txSz, W = txw(t), H = txh(t), tw = W > 32 ? 32 : W, th = H > 32 ? 32 : H;
@@ -507,8 +518,10 @@ static void analyse_block(Frame &F, const Params &P, int bs, int bx, int by) {
     const Tables &T = tables();
     const int W = 4 * bw4(bs), H = 4 * bh4(bs), lw = av1t::Mi_Width_Log2[bs] + 2, lh = av1t::Mi_Height_Log2[bs] + 2;
     const int x = bx * W, y = by * H;
-    BlockRes &out = F.blk[bs][(size_t)by * F.nbx[bs] + bx];
+    const size_t bi = (size_t)by * F.nbx[bs] + bx;
+    BlockRes &out = F.blk[bs][bi];
     if (x >= F.MiCols * 4 || y >= F.MiRows * 4) { out.J = INF; out.ncand = 0; return; }
+    if (F.analyse[bs] && !F.analyse[bs][bi]) { out.J = INF; out.ncand = 0; return; }   // not in the shape's block list
     const int txSz = tx_of_shape(bs), set = tx_set_intra(txSz), bd = F.bd;
     const TileBox tb = tile_of(F, x, y);
     const int haveL = x > tb.x0, haveA = y > tb.y0;
@@ -537,6 +550,24 @@ static void analyse_block(Frame &F, const Params &P, int bs, int bx, int by) {
     int cands[32];
     float js[32];
     int nc = 13;
+    const int K = std::min(P.K, MAXK);
+    int top[MAXK], nK = 0;
+    // an external screener's candidates, best first (k_analysis.cuh:586-600): entries with a mode of 13 or more are
+    // skipped, deltas kept for directional modes of blocks from 8x8 (outside -3..3: 0); a block with no usable entry is
+    // screened as without the hook
+    if (F.ext[bs]) {
+        const ExtCand *ec = F.ext[bs] + bi * F.extTopK;
+        for (int k = 0; k < F.extTopK && nK < MAXK; k++) {
+            const int m = ec[k].mode;
+            if (m >= 13) continue;
+            const int d = (bs >= BLOCK_8X8 && is_directional(m) && ec[k].delta >= -3 && ec[k].delta <= 3) ? ec[k].delta : 0;
+            cands[nK] = cand_pack(m, d);
+            top[nK] = nK;
+            nK++;
+        }
+        nK = std::min(nK, K);
+    }
+    if (!nK) {
     for (int m = 0; m < 13; m++) { cands[m] = cand_pack(m, 0); js[m] = fmaf(lamS, mbits(cands[m]), screen(cands[m])); }
     if (P.deltas && bs >= BLOCK_8X8) {
         int b0 = -1, b1 = -1;   // the two best directional modes, first minimum on ties
@@ -549,8 +580,6 @@ static void analyse_block(Frame &F, const Params &P, int bs, int bx, int by) {
         }
         nc = 25;
     }
-    const int K = std::min(P.K, MAXK);
-    int top[MAXK], nK = 0;
     bool used[32] = {false};
     for (int k = 0; k < K; k++) {
         int b = -1;
@@ -558,6 +587,7 @@ static void analyse_block(Frame &F, const Params &P, int bs, int bx, int by) {
         if (b < 0) break;
         used[b] = true;
         top[nK++] = b;
+    }
     }
     // RD: top-K with DCT_DCT, then the other tx types of the set for the winner (k_analysis.cuh:641-677)
     const int hasC = has_chroma(bs, y >> 2, x >> 2, F.mono, F.ssx, F.ssy);
@@ -568,7 +598,7 @@ static void analyse_block(Frame &F, const Params &P, int bs, int bx, int by) {
     static thread_local int16_t res[64 * 64], q[32 * 32];
     float bestJ = INF;
     int bestK = 0, bestTx = DCT_DCT, resK = -1;
-    const int nSteps = nK + ((P.txSearch && set != TX_SET_DCTONLY) ? 15 : 0);
+    const int nSteps = nK ? nK + ((P.txSearch && set != TX_SET_DCTONLY) ? 15 : 0) : 0;
     for (int st = 0; st < nSteps; st++) {
         const int k = st < nK ? st : bestK, t = st < nK ? DCT_DCT : st - nK + 1;
         if (t != DCT_DCT && !tx_in_set(set, t)) continue;
@@ -583,6 +613,7 @@ static void analyse_block(Frame &F, const Params &P, int bs, int bx, int by) {
         e.txType = t;
         e.intraDir = cand_mode(c);
         const EvalOut o = eval_txb(res, e, q);
+        count_eval(F, 0, e);
         const float J = fmaf(lambda, (o.R + mbits(c)) + overhead, o.D);
         if (J < bestJ) { bestJ = J; if (st < nK) bestK = k; else bestTx = t; }
     }
@@ -904,6 +935,7 @@ static LumaOut rc_luma(Frame &F, const Params &P, SbState &S, int sb, int bsize,
         ee.txType = t; ee.intraDir = cand_mode(c);
         ee.resEnergy = (W == 64 || H == 64) ? res_energy(res, W * H) : 0.f;
         const EvalOut o = eval_txb(res, ee, q);
+        count_eval(F, 1, ee);
         const float J = fmaf(lambda, o.R + mbits(c), o.D);
         if (J < bestJ) { bestJ = J; bestC = c; bestT = t; bestEob = o.eob; memcpy(bestQ, q, sizeof(int16_t) * tx_coeffs(txSz)); }
     };
@@ -1080,6 +1112,7 @@ static ChromaOut rc_chroma(Frame &F, const Params &P, SbState &S, int sb, int bs
         e.txSz = txSz; e.txType = t.t; e.plane = t.pl; e.dcQ = dcQ[t.pl]; e.acQ = acQ[t.pl]; e.rnd = P.rndC; e.zbin = P.zbinC;
         e.skipCtx = skipCtx[t.pl]; e.dcCtx = dcCtx[t.pl]; e.intraDir = 0; e.bd = bd; e.C = &C; e.resEnergy = 0.f;
         const EvalOut o = eval_txb(res, e, q);
+        count_eval(F, 1, e);
         t.J = fmaf(lambda, o.R, o.D);
         t.eob = o.eob;
         t.q.assign(q, q + tx_coeffs(txSz));
@@ -1139,6 +1172,7 @@ static ChromaOut rc_chroma(Frame &F, const Params &P, SbState &S, int sb, int bs
                 e.txSz = txSz; e.txType = uvTxType; e.plane = pl; e.dcQ = dcQ[pl]; e.acQ = acQ[pl]; e.rnd = P.rndC; e.zbin = P.zbinC;
                 e.skipCtx = sc; e.dcCtx = dc; e.intraDir = 0; e.bd = bd; e.C = &C; e.resEnergy = 0.f;
                 const EvalOut o = eval_txb(res, e, q);
+                count_eval(F, 1, e);
                 for (int i = 0; i < H; i++) for (int j = 0; j < W; j++) dst[i * F.stride[pl] + j] = (uint16_t)pred[i * W + j];
                 if (o.eob > 0) F.stats[3] += inverse_add(F, q, txSz, uvTxType, dcQ[pl], acQ[pl], dst, F.stride[pl]) ? 1 : 0;
                 eobs[pl - 1] |= record_txb(F, S, sb, pl, txSz, uvTxType, x4 + (ox >> 2), y4 + (oy >> 2), q);

@@ -3,6 +3,7 @@
 // explicit (see orc_search.cpp).  Used by tests/ as the parity checker and by bench.py as the CPU baseline.
 #pragma once
 #include <stdint.h>
+#include <atomic>
 #include <vector>
 #include "orc_dsp.h"
 
@@ -76,6 +77,9 @@ struct RecBlock { uint16_t mi_row, mi_col; uint8_t bsize, skip, tx_depth, y_mode
                   int8_t angle_delta_uv, cfl_alpha_u, cfl_alpha_v; uint8_t filter_intra; int8_t cdef_idx; uint8_t sb_qindex; };
 struct RecTxb { uint32_t coeff_off; uint16_t x4, y4; uint8_t plane, tx_size, tx_type, flags; uint16_t eob, reserved; };
 
+// an external screener's candidate (GpExtCand, encoder/src/gp.h:136-143)
+struct ExtCand { float J; uint8_t mode; int8_t delta; uint8_t tx, pad; };
+
 struct Frame {
     int W, H, MiCols, MiRows, sbCols, sbRows, aw, ah, ssx, ssy, mono, bd, qindex, qctx, edgeFilter;
     int tileCols, tileRows, tileColStart[65], tileRowStart[65];
@@ -97,6 +101,12 @@ struct Frame {
     std::vector<RecTxb> txb;
     std::vector<int16_t> coeffs;
     uint32_t stats[8];
+    std::atomic<uint64_t> flops[2]{};   // work accounting: forward-transform FLOPs of the analysis / closed-loop evaluations
+    // analysis hooks (GpBatch::extCand, gp.h:164-171; GpAnX::list, an_stage.h:23-27): per shape, the external candidates of each
+    // block of the frame's grid (raster, extTopK each) replacing the screening, and which blocks are analysed at all
+    const ExtCand *ext[22] = {};
+    int extTopK = 0;
+    const uint8_t *analyse[22] = {};
 };
 
 int tile_layout(int W, int H, int *cols, int *rows, int *colStart, int *rowStart);   // encoder.cpp:56-91 (min legal)

@@ -28,6 +28,9 @@ LEAF_DTYPE = np.dtype([("bsize", "u1"), ("x4", "u1"), ("y4", "u1"), ("pad", "u1"
 assert BLOCK_DTYPE.itemsize == 16 and TXB_DTYPE.itemsize == 16 and BLOCKRES_DTYPE.itemsize == 12
 
 
+EXT_DTYPE = np.dtype([("J", "<f4"), ("mode", "u1"), ("delta", "i1"), ("tx", "u1"), ("pad", "u1")])   # GpExtCand
+
+
 class OrcParams(C.Structure):
     _fields_ = [(n, C.c_float) for n in ("lambdaMul", "satdLambda", "rnd", "zbin", "rndC", "zbinC")] + \
                [(n, C.c_int) for n in ("K", "Krecheck", "txSearch", "txSearchCL", "txAllCL", "deltas", "uvDeltas", "cfl",
@@ -45,6 +48,9 @@ class Oracle:
         L = self.lib
         L.orc_encode.restype = C.c_void_p
         L.orc_encode.argtypes = [C.c_void_p] * 3 + [C.POINTER(C.c_int)] + [C.c_int] * 7 + [C.POINTER(OrcParams), C.c_int]
+        L.orc_encode_hooks.restype = C.c_void_p
+        L.orc_encode_hooks.argtypes = [C.c_void_p] * 3 + [C.POINTER(C.c_int)] + [C.c_int] * 7 + [C.POINTER(OrcParams), C.c_int,
+                                                                                          C.c_void_p, C.c_int, C.c_void_p]
         L.orc_free.argtypes = [C.c_void_p]
         L.orc_counts.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_record.argtypes = [C.c_void_p] * 5
@@ -78,17 +84,35 @@ class Oracle:
             setattr(p, k, v)
         return p
 
-    def encode(self, planes, W, H, ssx=1, mono=0, bd=8, qindex=120, edge_filter=1, params=None, threads=1, detail=False):
+    def encode(self, planes, W, H, ssx=1, mono=0, bd=8, qindex=120, edge_filter=1, params=None, threads=1, detail=False,
+               ext=None, ext_topk=0, analyse=None):
+        """ext: {shape: EXT_DTYPE array [blocks of the frame's grid (raster), ext_topk]} external candidates;
+        analyse: {shape: uint8 array [blocks of the grid]} 1 = analysed (the analysis hooks)."""
         strides = (C.c_int * 3)(*[p.shape[1] for p in planes])
         pl = [np.ascontiguousarray(p) for p in planes]
-        h = self.lib.orc_encode(_ptr(pl[0]), _ptr(pl[1]), _ptr(pl[2]), strides, W, H, ssx, mono, bd, qindex, edge_filter,
-                                C.byref(params or self.params()), threads)
+        keep = []
+        ext_p = an_p = None
+        if ext:
+            ext_p = (C.c_void_p * 22)()
+            for sh, a in ext.items():
+                a = np.ascontiguousarray(a, EXT_DTYPE)
+                keep.append(a)
+                ext_p[sh] = a.ctypes.data
+        if analyse:
+            an_p = (C.c_void_p * 22)()
+            for sh, a in analyse.items():
+                a = np.ascontiguousarray(a, np.uint8)
+                keep.append(a)
+                an_p[sh] = a.ctypes.data
+        h = self.lib.orc_encode_hooks(_ptr(pl[0]), _ptr(pl[1]), _ptr(pl[2]), strides, W, H, ssx, mono, bd, qindex, edge_filter,
+                                      C.byref(params or self.params()), threads, ext_p, ext_topk, an_p)
         try:
-            cnt = np.zeros(13, np.int64)
+            cnt = np.zeros(15, np.int64)
             self.lib.orc_counts(h, _ptr(cnt))
             nb, nt, nc, mic, mir, aw, ah = (int(x) for x in cnt[:7])
             out = dict(blocks=np.zeros(nb, BLOCK_DTYPE), block_txb=np.zeros(nb + 1, np.uint32), txb=np.zeros(nt, TXB_DTYPE),
-                       coeffs=np.zeros(nc, np.int16), MiCols=mic, MiRows=mir, aw=aw, ah=ah, stats=cnt[9:13].copy())
+                       coeffs=np.zeros(nc, np.int16), MiCols=mic, MiRows=mir, aw=aw, ah=ah, stats=cnt[9:13].copy(),
+                       flops=(int(cnt[13]), int(cnt[14])))
             self.lib.orc_record(h, _ptr(out["blocks"]), _ptr(out["block_txb"]), _ptr(out["txb"]), _ptr(out["coeffs"]))
             rec = []
             for p in range(3):
