@@ -29,6 +29,7 @@ _PARAM_FLOATS = ("lambdaMul", "satdLambda", "rnd", "zbin", "rndC", "zbinC")
 _PARAM_INTS = ("K", "Krecheck", "txSearch", "txSearchCL", "txAllCL", "deltas", "uvDeltas", "cfl", "uvK", "shapeMask",
                "partMask8", "partMaskN", "maxPart")
 _PARAM_FLOATS2 = ("uvOverheadBits", "splitBias", "lambdaCMul")
+_POSTFILTER_INTS = ("cdef", "cdefDampingMinus3", "cdefYPri", "cdefYSec", "cdefUVPri", "cdefUVSec")
 
 
 class _Params(C.Structure):
@@ -38,7 +39,24 @@ class _Params(C.Structure):
 
 class _Config(C.Structure):
     _fields_ = [("qindex", C.c_int), ("edgeFilter", C.c_int), ("P", _Params), ("schedule", C.c_int), ("queueBlocks", C.c_int),
-                ("anStreams", C.c_int), ("graph", C.c_int), ("tune", C.c_int)]
+                ("anStreams", C.c_int), ("graph", C.c_int), ("tune", C.c_int), ("lfMode", C.c_int), ("lfMul", C.c_float),
+                ("lfLevel", C.c_int)] + [(n, C.c_int) for n in _POSTFILTER_INTS]
+
+
+class _FrameParams(C.Structure):
+    """av1mi_frame_params = av1g_frame_params (decisions.h:269-313), 88 bytes."""
+    _fields_ = [("version", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32)] + \
+               [(n, C.c_uint8) for n in ("monochrome", "base_q_idx", "tx_mode_select", "reduced_tx_set")] + \
+               [(n, C.c_int8) for n in ("tile_cols_log2", "tile_rows_log2", "delta_q_y_dc", "delta_q_u_dc", "delta_q_u_ac",
+                                        "delta_q_v_dc", "delta_q_v_ac")] + \
+               [(n, C.c_uint8) for n in ("enable_filter_intra", "enable_intra_edge_filter", "delta_q_present", "delta_q_res",
+                                         "using_qmatrix", "qm_y", "qm_u", "qm_v")] + \
+               [("loop_filter_level", C.c_uint8 * 4), ("loop_filter_sharpness", C.c_uint8), ("enable_cdef", C.c_uint8),
+                ("cdef_damping_minus_3", C.c_uint8), ("cdef_bits", C.c_uint8), ("cdef_y_pri", C.c_uint8 * 8),
+                ("cdef_y_sec", C.c_uint8 * 8), ("cdef_uv_pri", C.c_uint8 * 8), ("cdef_uv_sec", C.c_uint8 * 8)] + \
+               [(n, C.c_uint8) for n in ("allow_screen_content_tools", "chroma_format", "bit_depth", "color_description",
+                                         "color_primaries", "transfer_characteristics", "matrix_coefficients")] + \
+               [("reserved", C.c_uint8 * 9)]
 
 
 class _FrameIn(C.Structure):
@@ -51,14 +69,15 @@ class _FrameOut(C.Structure):
                 ("chroma_format", C.c_uint32), ("monochrome", C.c_uint32), ("n_blocks", C.c_uint32), ("n_txb", C.c_uint32),
                 ("n_coeffs", C.c_uint64), ("blocks", C.c_void_p), ("block_txb", C.c_void_p), ("txb", C.c_void_p),
                 ("coeffs", C.c_void_p), ("recon", C.c_void_p * 3), ("recon_pitch", C.c_int * 3), ("aw", C.c_int),
-                ("ah", C.c_int), ("tile_cols", C.c_int), ("tile_rows", C.c_int)]
+                ("ah", C.c_int), ("tile_cols", C.c_int), ("tile_rows", C.c_int), ("fp", _FrameParams)]
 
 
 class _Times(C.Structure):
     _fields_ = [(n, C.c_float) for n in ("setup", "analysis", "dp", "recon", "pack", "download", "total")] + \
                [(n, C.c_uint32) for n in ("steps", "sbs", "blocks", "skipBlocks", "cflBlocks", "rangeFixes")] + \
                [("deviceBytes", C.c_size_t), ("schedule", C.c_uint32), ("queueBlocks", C.c_uint32), ("anStreams", C.c_uint32),
-                ("graph", C.c_uint32), ("tuned", C.c_uint32), ("flopsAnalysis", C.c_uint64), ("flopsRecon", C.c_uint64)]
+                ("graph", C.c_uint32), ("tuned", C.c_uint32), ("flopsAnalysis", C.c_uint64), ("flopsRecon", C.c_uint64),
+                ("postfilter", C.c_float)]
 
 
 _lib = None
@@ -158,6 +177,17 @@ class GpConfig:
     anStreams: int = 1     # analysis shape launches spread over 1..4 streams
     graph: int = 0         # capture the device part per batch shape as a hipGraph and replay it
     tune: int = 0          # the first 4 calls of a batch shape try anStreams 1/4 x schedule 0/1, then keep the fastest
+    # post-filters (pipeline.h:60-66): lfMode 0 off / 1 level from each frame's qindex (times lfMul) / 2 fixed lfLevel;
+    # cdef 1 = one strength set (damping cdefDampingMinus3 + 3, primary 0..15, secondary 0/1/2/4)
+    lfMode: int = 0
+    lfMul: float = 1.0
+    lfLevel: int = 0
+    cdef: int = 0
+    cdefDampingMinus3: int = 0
+    cdefYPri: int = 0
+    cdefYSec: int = 0
+    cdefUVPri: int = 0
+    cdefUVSec: int = 0
 
 
 @dataclass
@@ -191,6 +221,7 @@ class GpFrameOut:
     tile_cols: int
     tile_rows: int
     chroma_format: int = 0       # 0 = 4:2:0 (or monochrome), 1 = 4:4:4
+    fp: object = None            # the frame's decisions (av1mi_frame_params: loop filter levels, CDEF, ...)
 
 
 @dataclass
@@ -216,6 +247,7 @@ class GpTimes:
     tuned: int = 0
     flopsAnalysis: int = 0
     flopsRecon: int = 0
+    postfilter: float = 0
 
 
 def check_partitions(P: GpParams):
@@ -266,6 +298,9 @@ def write_bitstream(rec, edge_filter=1, avif=True, monochrome=0):
     o.n_blocks, o.n_txb, o.n_coeffs = len(blocks), len(txb), len(co)
     o.blocks, o.block_txb, o.txb, o.coeffs = blocks.ctypes.data, bt.ctypes.data, txb.ctypes.data, co.ctypes.data
     o.tile_cols, o.tile_rows = g("tile_cols"), g("tile_rows")
+    fp = rec.get("fp") if isinstance(rec, dict) else getattr(rec, "fp", None)
+    if fp is not None:
+        C.memmove(C.byref(o.fp), C.byref(fp), C.sizeof(_FrameParams))
     data, size = C.c_void_p(), C.c_size_t()
     err = C.create_string_buffer(512)
     fn = lib().av1mi_write_avif if avif else lib().av1mi_write_obus
@@ -377,7 +412,8 @@ class GpEncoder:
         if self._h:
             lib().av1mi_destroy(self._h)
             self._h = None
-        c = _Config(cfg.qindex, cfg.edgeFilter, cfg.P._c(), cfg.schedule, cfg.queueBlocks, cfg.anStreams, cfg.graph, cfg.tune)
+        c = _Config(cfg.qindex, cfg.edgeFilter, cfg.P._c(), cfg.schedule, cfg.queueBlocks, cfg.anStreams, cfg.graph, cfg.tune,
+                    cfg.lfMode, cfg.lfMul, cfg.lfLevel, *[getattr(cfg, n) for n in _POSTFILTER_INTS])
         h = C.c_void_p()
         err = C.create_string_buffer(512)
         if lib().av1mi_create(C.byref(c), C.byref(h), err, 512) != 0:
@@ -428,8 +464,10 @@ class GpEncoder:
                         h = o.ah >> (1 if p and o.chroma_format != 1 else 0)
                         w = o.recon_pitch[p]
                         rec.append(_owned(o.recon[p], w * h, np.uint16 if bps == 2 else np.uint8, own).reshape(h, w))
+                fp = _FrameParams()
+                C.memmove(C.byref(fp), C.byref(o.fp), C.sizeof(_FrameParams))
                 res.append(GpFrameOut(o.width, o.height, o.base_q_idx, o.bit_depth, blocks, bt, txb, co, rec, o.tile_cols,
-                                      o.tile_rows, o.chroma_format))
+                                      o.tile_rows, o.chroma_format, fp))
         finally:
             for i in range(n):
                 lib().av1mi_frame_out_free(C.byref(outs[i]))

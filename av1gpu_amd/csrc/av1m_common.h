@@ -585,6 +585,10 @@ struct Img {
     int pitch[3];                             // reconstruction pitches, samples
     uint8_t *mY, *mUV, *mSkip, *mMiSize, *mTx; // 4x4 maps (MiRows x MiCols)
     uint8_t *cLvl[3], *cDc[3];                // coefficient-context maps per plane (plane units)
+    // post-filters: effective loop filter levels (Y vertical, Y horizontal, U, V; 0 off) and sharpness; CDEF with one
+    // strength set (secondary strengths 0, 1, 2, 4), the deblocked frame filtered into cdefOut (same geometry as rec)
+    uint8_t lfLevel[4], lfSharp, cdefOn, cdefDamping, cdefYPri, cdefYSec, cdefUVPri, cdefUVSec;
+    void *cdefOut[3];
 };
 struct BlockRes { float J; uint8_t cand[MAXK]; uint8_t ncand; uint8_t tx; };
 struct Leaf { uint8_t bsize, x4, y4, pad; };

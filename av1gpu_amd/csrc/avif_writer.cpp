@@ -185,6 +185,32 @@ public:
         tiles = tile_info(MiCols, MiRows);
         if ((int)tiles.colStart.size() - 1 != R.tile_cols || (int)tiles.rowStart.size() - 1 != R.tile_rows)
             throw WriteError("the record's tile layout is not the minimum uniform one");
+        // post-filter parameters (loop_filter_params, cdef_params) from the frame's decisions, checked as the reference's
+        // av1g_validate does (av1core/src/encoder.cpp:1495-1524)
+        if (R.fp.version) {
+            const av1mi_frame_params &F = R.fp;
+            for (int k = 0; k < 4; k++) {
+                if (F.loop_filter_level[k] > 63) throw WriteError("loop_filter_level > 63");
+                lf[k] = F.loop_filter_level[k];
+            }
+            if (mono) lf[2] = lf[3] = 0;
+            if (!lf[0] && !lf[1] && (lf[2] || lf[3])) throw WriteError("chroma loop filter levels need a nonzero luma level");
+            if (F.loop_filter_sharpness > 7) throw WriteError("loop_filter_sharpness > 7");
+            lfSharp = F.loop_filter_sharpness;
+            if (F.enable_cdef > 1 || F.cdef_damping_minus_3 > 3 || F.cdef_bits > 3) throw WriteError("bad CDEF parameters");
+            cdefOn = F.enable_cdef;
+            if (cdefOn) {
+                cdefDamp3 = F.cdef_damping_minus_3;
+                cdefBits = F.cdef_bits;
+                auto secOk = [](int v) { return v == 0 || v == 1 || v == 2 || v == 4; };
+                for (int i = 0; i < (1 << cdefBits); i++) {
+                    if (F.cdef_y_pri[i] > 15 || F.cdef_uv_pri[i] > 15 || !secOk(F.cdef_y_sec[i]) || !secOk(F.cdef_uv_sec[i]))
+                        throw WriteError("bad CDEF strength");
+                    cdefYPri[i] = F.cdef_y_pri[i]; cdefYSec[i] = F.cdef_y_sec[i];
+                    cdefUVPri[i] = F.cdef_uv_pri[i]; cdefUVSec[i] = F.cdef_uv_sec[i];
+                }
+            }
+        }
         const size_t mi = (size_t)MiCols * MiRows;
         mSize.assign(mi, 0); mSkip.assign(mi, 0); mYMode.assign(mi, 0);
         for (int p = 0; p < 3; p++) {
@@ -226,6 +252,9 @@ public:
 private:
     const av1mi_frame_out &R;
     int edge, W, H, MiCols, MiRows, mono, bd, qctx, ss = 1;
+    int lf[4] = {0, 0, 0, 0}, lfSharp = 0, cdefOn = 0, cdefDamp3 = 0, cdefBits = 0;
+    int cdefYPri[8] = {}, cdefYSec[8] = {}, cdefUVPri[8] = {}, cdefUVSec[8] = {};
+    int sbCdef = -1;   // cdef_idx of the current superblock (-1: none read yet; spec clear_cdef / read_cdef)
     size_t seqLen = 0;
     Tiles tiles;
     std::vector<uint8_t> mSize, mSkip, mYMode;
@@ -261,7 +290,7 @@ private:
         s.put(0, 1);                 // enable_filter_intra
         s.put(edge ? 1 : 0, 1);      // enable_intra_edge_filter
         s.put(0, 1);                 // enable_superres
-        s.put(0, 1);                 // enable_cdef
+        s.put(cdefOn, 1);            // enable_cdef
         s.put(0, 1);                 // enable_restoration
         s.put(bd == 10, 1);          // high_bitdepth
         if (profile() != 1) s.put(mono, 1);   // mono_chrome (absent in profile 1)
@@ -304,9 +333,19 @@ private:
         f.put(0, 1);                 // using_qmatrix
         f.put(0, 1);                 // segmentation_enabled
         f.put(0, 1);                 // delta_q_present
-        f.put(0, 6); f.put(0, 6);    // loop_filter_level[0], [1]
-        f.put(0, 3);                 // loop_filter_sharpness
+        f.put(lf[0], 6); f.put(lf[1], 6);    // loop_filter_params: loop_filter_level[0], [1]
+        if (!mono && (lf[0] || lf[1])) { f.put(lf[2], 6); f.put(lf[3], 6); }
+        f.put(lfSharp, 3);           // loop_filter_sharpness
         f.put(0, 1);                 // loop_filter_delta_enabled
+        if (cdefOn) {                // cdef_params
+            f.put(cdefDamp3, 2);
+            f.put(cdefBits, 2);
+            for (int i = 0; i < (1 << cdefBits); i++) {
+                f.put(cdefYPri[i], 4);
+                f.put(cdefYSec[i] == 4 ? 3 : cdefYSec[i], 2);
+                if (!mono) { f.put(cdefUVPri[i], 4); f.put(cdefUVSec[i] == 4 ? 3 : cdefUVSec[i], 2); }
+            }
+        }
         f.put(0, 1);                 // tx_mode_select = 0: TX_MODE_LARGEST
         f.put(0, 1);                 // reduced_tx_set
         f.align();
@@ -323,7 +362,10 @@ private:
         for (int p = 0; p < 3; p++) { std::fill(aLev[p].begin(), aLev[p].end(), 0); std::fill(aDc[p].begin(), aDc[p].end(), 0); }
         for (int r = tr0; r < tr1; r += 16) {
             for (int p = 0; p < 3; p++) { std::fill(lLev[p].begin(), lLev[p].end(), 0); std::fill(lDc[p].begin(), lDc[p].end(), 0); }
-            for (int c = tc0; c < tc1; c += 16) partition(ec, r, c, BLOCK_64X64, blk);
+            for (int c = tc0; c < tc1; c += 16) {
+                sbCdef = -1;   // clear_cdef
+                partition(ec, r, c, BLOCK_64X64, blk);
+            }
         }
         return ec.finish();
     }
@@ -439,6 +481,16 @@ private:
         // intra_frame_mode_info: skip, y mode, angle delta, uv mode (+ CfL), uv angle delta
         const int sctx = (availU ? at(mSkip, r - 1, c) : 0) + (availL ? at(mSkip, r, c - 1) : 0);
         ec.adapt(b.skip ? 1 : 0, cdf.skip[sctx], 2);
+        // read_cdef: the superblock's index with its first coded block (cdef_bits literal bits), the same in the others
+        if (cdefOn ? (b.cdef_idx < 0 || b.cdef_idx >= (1 << cdefBits)) : b.cdef_idx != 0) throw WriteError("cdef_idx out of range");
+        if (cdefOn && !b.skip) {
+            if (sbCdef < 0) {
+                ec.literal((uint32_t)b.cdef_idx, cdefBits);
+                sbCdef = b.cdef_idx;
+            } else if (b.cdef_idx != sbCdef) {
+                throw WriteError("cdef_idx differs from its superblock's");
+            }
+        }
         const int am = av1t::Intra_Mode_Context[availU ? at(mYMode, r - 1, c) : DC_PRED];
         const int lm = av1t::Intra_Mode_Context[availL ? at(mYMode, r, c - 1) : DC_PRED];
         if (b.y_mode > PAETH_PRED) throw WriteError("bad y_mode");

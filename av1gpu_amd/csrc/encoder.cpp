@@ -8,6 +8,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <math.h>
 #include <algorithm>
 #include <map>
 #include <atomic>
@@ -39,6 +40,8 @@ size_t recon_smem_bytes();
 void launch_pack(const Batch &B, const PackOff *dOff, RecBlock *oB, uint32_t *oBT, RecTxb *oT, int16_t *oC, hipStream_t st);
 void launch_to10(const uint8_t *src, int sp, uint16_t *dst, int dp, int w, int h, int chroma, hipStream_t st);
 hipError_t noalloc_scratch(void **ptr, size_t *bytes);
+void launch_deblock(const Batch &B, int pass, const uint64_t *dStart, uint64_t units, int bd, hipStream_t st);
+void launch_cdef(const Batch &B, int bd, hipStream_t st);
 }  // namespace av1m
 
 using namespace av1m;
@@ -117,7 +120,7 @@ struct av1mi_encoder {
     av1mi_config cfg;
     int device = 0;
     DevTables *dT = nullptr;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[10] = {};
     int cus = 0, slots[2] = {0, 0};   // compute units; resident closed-loop workgroups per CU (8-bit, 10-bit)
     uint64_t clockHz = 100000000;     // wall_clock64 rate
     LnNet *ln = nullptr;              // learned pruning network (null: exhaustive search)
@@ -260,6 +263,17 @@ int av1mi_create(const av1mi_config *cfg, av1mi_encoder **out, char *err, int er
     if (he != hipSuccess) { snprintf(err, errlen, "HIP: %s (table upload)", hipGetErrorString(he)); delete e; return -1; }
     if (recon_prepare(8) || recon_prepare(10)) { snprintf(err, errlen, "closed-loop kernel needs %zu bytes of LDS", recon_smem_bytes()); delete e; return -1; }
     if (c.schedule < 0 || c.schedule > 1 || c.queueBlocks < 0) { snprintf(err, errlen, "bad schedule %d / queueBlocks %d", c.schedule, c.queueBlocks); delete e; return -1; }
+    const auto secOk = [](int v) { return v == 0 || v == 1 || v == 2 || v == 4; };
+    if (c.lfMode < 0 || c.lfMode > 2 || (c.lfMode == 2 && (c.lfLevel < 0 || c.lfLevel > 63)) || !(c.lfMul >= 0.f && c.lfMul < 1e6f) ||
+        c.cdef < 0 || c.cdef > 1 ||
+        (c.cdef && (c.cdefDampingMinus3 < 0 || c.cdefDampingMinus3 > 3 || c.cdefYPri < 0 || c.cdefYPri > 15 || c.cdefUVPri < 0 ||
+                    c.cdefUVPri > 15 || !secOk(c.cdefYSec) || !secOk(c.cdefUVSec)))) {
+        snprintf(err, errlen, "bad post-filter options: lfMode %d (0..2), lfLevel %d (0..63), lfMul %g, cdef %d, damping-3 %d (0..3), "
+                 "strengths Y %d/%d UV %d/%d (primary 0..15, secondary 0, 1, 2, 4)", c.lfMode, c.lfLevel, c.lfMul, c.cdef,
+                 c.cdefDampingMinus3, c.cdefYPri, c.cdefYSec, c.cdefUVPri, c.cdefUVSec);
+        delete e;
+        return -1;
+    }
     if (c.anStreams < 0 || c.anStreams > 4 || c.graph < 0 || c.graph > 1 || c.tune < 0 || c.tune > 1) {
         snprintf(err, errlen, "bad anStreams %d / graph %d / tune %d", c.anStreams, c.graph, c.tune);
         delete e;
@@ -420,6 +434,44 @@ void av1mi_get_config(const av1mi_encoder *e, av1mi_config *cfg) { *cfg = e->cfg
 
 }  // extern "C"
 
+// The post-filter decisions of a frame (pipeline.cpp:960-986): the loop filter level of lfMode 1 is libaom's
+// LPF_PICK_FROM_Q fit for a key frame at the frame's ac quantizer (8-bit: Round2(acq * 17563 - 421574, 18); 10-bit:
+// Round2(acq * 20723 + 4060632, 20) - 4) times lfMul, rounded to nearest (ties to even) and clamped to 0..63; the same
+// level on every plane (none on chroma of monochrome frames), sharpness 0; CDEF one strength set.
+static void frame_postfilter(const av1mi_config &c, Img &I) {
+    memset(I.lfLevel, 0, sizeof I.lfLevel);
+    I.lfSharp = 0;
+    if (c.lfMode) {
+        int lv = c.lfLevel;
+        const long acq = ac_q(I.qindex, I.bd);
+        if (c.lfMode == 1) {
+            const long v = I.bd == 10 ? ((acq * 20723 + 4060632 + (1 << 19)) >> 20) - 4 : (acq * 17563 - 421574 + (1 << 17)) >> 18;
+            lv = (int)lrintf((float)v * c.lfMul);
+        }
+        lv = std::max(0, std::min(63, lv));
+        I.lfLevel[0] = I.lfLevel[1] = (uint8_t)lv;
+        I.lfLevel[2] = I.lfLevel[3] = (uint8_t)(I.mono ? 0 : lv);
+    }
+    I.cdefOn = (uint8_t)(c.cdef ? 1 : 0);
+    I.cdefDamping = (uint8_t)(c.cdefDampingMinus3 + 3);
+    I.cdefYPri = (uint8_t)c.cdefYPri; I.cdefYSec = (uint8_t)c.cdefYSec;
+    I.cdefUVPri = (uint8_t)c.cdefUVPri; I.cdefUVSec = (uint8_t)c.cdefUVSec;
+}
+
+// out.fp: the frame parameters of av1mi_frame_params_of with the frame's post-filter decisions
+static void frame_params_fill(av1mi_frame_out &o, const Img &I) {
+    av1mi_frame_params_of(&o, I.edgeFilter, -1, &o.fp);
+    for (int k = 0; k < 4; k++) o.fp.loop_filter_level[k] = I.lfLevel[k];
+    o.fp.loop_filter_sharpness = I.lfSharp;
+    o.fp.enable_cdef = I.cdefOn;
+    if (I.cdefOn) {
+        o.fp.cdef_damping_minus_3 = (uint8_t)(I.cdefDamping - 3);
+        o.fp.cdef_bits = 0;
+        o.fp.cdef_y_pri[0] = I.cdefYPri; o.fp.cdef_y_sec[0] = I.cdefYSec;
+        o.fp.cdef_uv_pri[0] = I.cdefUVPri; o.fp.cdef_uv_sec[0] = I.cdefUVSec;
+    }
+}
+
 // The no-allocation variant of an encode: the arena laid over the device's scratch array, the packed records written
 // into the caller's buffers.
 struct NoAlloc {
@@ -469,6 +521,7 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
         if (I.qindex < 1 || I.qindex > 255) { snprintf(err, errlen, "frame %d: qindex %d outside 1..255", i, I.qindex); return -1; }
         I.qctx = I.qindex <= 20 ? 0 : I.qindex <= 60 ? 1 : I.qindex <= 120 ? 2 : 3;
         I.edgeFilter = enc->cfg.edgeFilter ? 1 : 0;
+        frame_postfilter(enc->cfg, I);
         tile_layout(f.W, f.H, I.tileCols, I.tileRows, I.tileColStart, I.tileRowStart);
         for (int p = 0; p < 3; p++) { I.src[p] = f.dsrc[p]; I.spitch[p] = f.spitch[p]; }
         I.sbBase = nSb;
@@ -508,6 +561,16 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
     unsigned long long *dFlops = nullptr;
     A.add(dStats, 8); A.add(dDone, nSb); A.add(dQctl, 2);
     if (enc->countWork) A.add(dFlops, 64);
+    // the CDEF output planes (the reconstruction's geometry), zeroed like it
+    std::vector<uint8_t *> cdefMem(n, nullptr);
+    bool anyLf = false, anyCdef = false;
+    for (int i = 0; i < n; i++) {
+        const Img &I = imgs[i];
+        anyLf = anyLf || I.lfLevel[0] || I.lfLevel[1];
+        if (!I.cdefOn) continue;
+        anyCdef = true;
+        A.add(cdefMem[i], ((size_t)I.aw * I.ah + 2 * (size_t)(I.aw >> I.ss) * (I.ah >> I.ss)) * bps);
+    }
     // everything so far starts each call at zero (reconstruction, maps, counters, done flags): one fill kernel; the
     // padding beyond the coded area is never reconstructed, so the returned planes are fully defined
     const size_t zeroBytes = A.total;
@@ -540,6 +603,22 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
     uint32_t *dSteps, *dLnMask = nullptr;
     PackOff *dPO;
     A.add(dSteps, stepFlat.size()); A.add(dPO, nSb);
+    // deblocking work units per image (pass 0: each plane's rows x 4-sample columns; pass 1: 4-sample rows x columns)
+    uint64_t *dDbStart = nullptr;
+    std::vector<uint64_t> dbStart(2 * ((size_t)n + 1), 0);
+    if (anyLf) {
+        for (int pass = 0; pass < 2; pass++)
+            for (int i = 0; i < n; i++) {
+                const Img &I = imgs[i];
+                uint64_t u = 0;
+                for (int p = 0; p < (I.mono ? 1 : 3); p++) {
+                    const uint64_t rows = (uint64_t)(I.MiRows * 4) >> (p ? I.ss : 0), cols = (uint64_t)(I.MiCols * 4) >> (p ? I.ss : 0);
+                    u += pass == 0 ? rows * (cols >> 2) : (rows >> 2) * cols;
+                }
+                dbStart[pass * ((size_t)n + 1) + i + 1] = dbStart[pass * ((size_t)n + 1) + i] + u;
+            }
+        A.add(dDbStart, dbStart.size());
+    }
     uint8_t *dLnAct = nullptr;
     uint32_t lnN = std::min(nSb, enc->lnChunk);
     const size_t fixedBytes = A.total + (enc->ln ? (((size_t)nSb * 12 * 4 + 255) & ~(size_t)255) : 0) +
@@ -599,6 +678,8 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
         uint8_t *c = m + 5 * mi;
         I.cLvl[0] = c; I.cLvl[1] = c + mi; I.cLvl[2] = c + mi + c1;
         I.cDc[0] = c + mi + 2 * c1; I.cDc[1] = c + 2 * mi + 2 * c1; I.cDc[2] = c + 2 * mi + 3 * c1;
+        for (int p = 0; p < 3; p++) I.cdefOut[p] = nullptr;
+        if (cdefMem[i]) { I.cdefOut[0] = cdefMem[i]; I.cdefOut[1] = cdefMem[i] + ysz; I.cdefOut[2] = cdefMem[i] + ysz + csz; }
     }
     Batch B;
     memset(&B, 0, sizeof B);
@@ -688,6 +769,17 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
     (void)hipEventRecord(enc->ev[4], st);
     tm.steps = (uint32_t)steps.size();
     tm.sbs = nSb;
+    // post-filters on the finished reconstruction: deblocking (vertical, then horizontal edges), then CDEF into its own
+    // planes (GpTimes::postfilter)
+    (void)hipEventRecord(enc->ev[8], st);
+    if (anyLf) {
+        HCK(hipMemcpyAsync(dDbStart, dbStart.data(), dbStart.size() * 8, hipMemcpyHostToDevice, st));
+        for (int pass = 0; pass < 2; pass++)
+            launch_deblock(B, pass, dDbStart + pass * ((size_t)n + 1), dbStart[pass * ((size_t)n + 1) + n], bd, st);
+    }
+    if (anyCdef) launch_cdef(B, bd, st);
+    HCK(hipGetLastError());
+    (void)hipEventRecord(enc->ev[9], st);
     // ---- pack: prefix sums in decode order (tiles raster, SBs raster inside each tile) ----
     std::vector<SbOut> so(nSb);
     uint32_t stats[8], qctl[2];
@@ -763,11 +855,12 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
             o.n_txb = (uint32_t)(imgTxb[i + 1] - imgTxb[i]);
             o.n_coeffs = imgCoef[i + 1] - imgCoef[i];
             o.aw = I.aw; o.ah = I.ah; o.tile_cols = I.tileCols; o.tile_rows = I.tileRows;
+            frame_params_fill(o, I);
             for (int p = 0; p < 3 && na->o->recon; p++) {
                 void *dst = na->o->recon[3 * (size_t)i + p];
                 if (!dst) continue;
                 const int pw = p ? I.aw >> I.ss : I.aw, ph = p ? I.ah >> I.ss : I.ah;
-                HCK(hipMemcpyAsync(dst, I.rec[p], (size_t)pw * ph * bps, hipMemcpyDeviceToDevice, st));
+                HCK(hipMemcpyAsync(dst, I.cdefOn ? I.cdefOut[p] : I.rec[p], (size_t)pw * ph * bps, hipMemcpyDeviceToDevice, st));
             }
         }
         (void)hipEventRecord(enc->ev[7], st);
@@ -801,6 +894,7 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
             o.n_txb = (uint32_t)(imgTxb[i + 1] - imgTxb[i]);
             o.n_coeffs = imgCoef[i + 1] - imgCoef[i];
             o.aw = I.aw; o.ah = I.ah; o.tile_cols = I.tileCols; o.tile_rows = I.tileRows;
+            frame_params_fill(o, I);
             if (!dev) {
                 o.blocks = (av1mi_block *)malloc(std::max<size_t>(o.n_blocks, 1) * 16);
                 o.block_txb = (uint32_t *)malloc(((size_t)o.n_blocks + 1) * 4);
@@ -817,7 +911,7 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
                     const size_t b = (size_t)pw * ph * bps;
                     o.recon[p] = malloc(b);
                     o.recon_pitch[p] = pw;
-                    HCK(hipMemcpyAsync(stg + reconOff, I.rec[p], b, hipMemcpyDeviceToHost, st));
+                    HCK(hipMemcpyAsync(stg + reconOff, I.cdefOn ? I.cdefOut[p] : I.rec[p], b, hipMemcpyDeviceToHost, st));
                     jobs.emplace_back(o.recon[p], stg + reconOff, b);
                     reconOff += b;
                 }
@@ -845,7 +939,9 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
     A.release(st);
     float ms[7];
     for (int k = 0; k < 7; k++) (void)hipEventElapsedTime(&ms[k], enc->ev[k], enc->ev[k + 1]);
-    tm.setup = ms[0]; tm.analysis = ms[1]; tm.dp = ms[2]; tm.recon = ms[3]; tm.pack = ms[5]; tm.download = ms[6] + ms[4];
+    (void)hipEventElapsedTime(&tm.postfilter, enc->ev[8], enc->ev[9]);
+    tm.setup = ms[0]; tm.analysis = ms[1]; tm.dp = ms[2]; tm.recon = ms[3]; tm.pack = ms[5];
+    tm.download = ms[6] + std::max(0.f, ms[4] - tm.postfilter);
     (void)hipEventElapsedTime(&tm.total, enc->ev[0], enc->ev[7]);
     if (cand >= 0) {   // autotuner: the device part's time of this candidate; after the 4th, keep the fastest
         tu->ms[cand] = ms[1] + ms[2] + ms[3];

@@ -16,8 +16,6 @@ int refuse_config(const GpConfig &c, std::string &err) {
         {c.qm != -1 || c.qmUV != -1, "qm / qmUV (quantizer matrices)"},
         {c.deltaq != 0 || c.deltaqStrength != 0, "deltaq (per-superblock qindex)"},
         {c.dqUDc || c.dqUAc || c.dqVDc || c.dqVAc, "dqUDc / dqUAc / dqVDc / dqVAc (chroma delta-q)"},
-        {c.lfMode != 0, "lfMode (deblocking)"},
-        {c.cdef != 0, "cdef (CDEF)"},
         {c.tuneIq || c.vb || c.fLf >= 0 || c.fCdef || c.ssimLambda, "tuneIq / vb / fLf / fCdef / ssimLambda (tune=iq tools)"},
         {c.tileMinMp > 0, "tileMinMp (requested tiling; this core codes minimum-legal tiles)"},
         {c.lossless != 0, "lossless"},
@@ -43,6 +41,7 @@ av1mi_frame_out GpFrameOut::record() {
     o.n_blocks = (uint32_t)blocks.size(); o.n_txb = (uint32_t)txb.size(); o.n_coeffs = coeffs.size();
     o.blocks = blocks.data(); o.block_txb = block_txb.data(); o.txb = txb.data(); o.coeffs = coeffs.data();
     o.tile_cols = tile_cols; o.tile_rows = tile_rows;
+    o.fp = fp;
     return o;
 }
 
@@ -60,6 +59,9 @@ int GpEncoder::init(const GpConfig &cfg, std::string &err) {
     c.qindex = cfg.qindex;
     c.edgeFilter = cfg.edgeFilter;
     c.schedule = cfg.schedule; c.queueBlocks = cfg.queueBlocks; c.anStreams = cfg.anStreams; c.graph = cfg.graph; c.tune = cfg.tune;
+    c.lfMode = cfg.lfMode; c.lfMul = cfg.lfMul; c.lfLevel = cfg.lfLevel;
+    c.cdef = cfg.cdef; c.cdefDampingMinus3 = cfg.cdefDampingMinus3; c.cdefYPri = cfg.cdefYPri; c.cdefYSec = cfg.cdefYSec;
+    c.cdefUVPri = cfg.cdefUVPri; c.cdefUVSec = cfg.cdefUVSec;
     char e[512] = {0};
     if (av1mi_create(&c, &h_, e, sizeof e)) { err = e; h_ = nullptr; return -1; }
     cfg_ = cfg;
@@ -92,7 +94,11 @@ int GpEncoder::encode(const std::vector<GpFrameIn> &in, std::vector<GpFrameOut> 
     for (int i = 0; i < n; i++) {
         av1mi_frame_out &o = fo[i];
         GpFrameOut &g = out[i];
-        av1mi_frame_params_of(&o, cfg_.edgeFilter, in[i].colorMatrix, &g.fp);
+        g.fp = o.fp;   // the encoder's frame decisions, with the frame's colour description
+        av1mi_frame_params col;
+        av1mi_frame_params_of(&o, cfg_.edgeFilter, in[i].colorMatrix, &col);
+        g.fp.color_description = col.color_description; g.fp.color_primaries = col.color_primaries;
+        g.fp.transfer_characteristics = col.transfer_characteristics; g.fp.matrix_coefficients = col.matrix_coefficients;
         g.tile_cols = o.tile_cols; g.tile_rows = o.tile_rows;
         if (!dev) {
             g.blocks.assign(o.blocks, o.blocks + o.n_blocks);

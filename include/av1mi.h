@@ -87,6 +87,14 @@ typedef struct av1mi_config {
     int anStreams;
     int graph;
     int tune;
+    /* post-filters (GpConfig, pipeline.h:60-66; frame levels as pipeline.cpp:960-986), decoder-exact on the GPU after
+     * the closed loop; the returned reconstruction is the filtered frame.  lfMode: 0 off, 1 libaom's LPF_PICK_FROM_Q
+     * level of each frame's qindex times lfMul, 2 the fixed lfLevel (0..63); cdef: 1 = one CDEF strength set for every
+     * frame (cdef_bits 0): damping cdefDampingMinus3 + 3 (0..3), primary strengths 0..15, secondary 0, 1, 2 or 4. */
+    int lfMode;
+    float lfMul;
+    int lfLevel;
+    int cdef, cdefDampingMinus3, cdefYPri, cdefYSec, cdefUVPri, cdefUVSec;
 } av1mi_config;
 
 /* GpFrameIn (pipeline.h:149-163): device planes padded (edge replicated) to aw x ah, multiples of 64; chroma
@@ -114,6 +122,9 @@ typedef struct av1mi_frame_out {
     int recon_pitch[3];
     int aw, ah;
     int tile_cols, tile_rows;
+    /* the frame's decisions (GpFrameOut::fp, pipeline.h:166; set as pipeline.cpp:938-1007 with the default colour
+     * description): sizes, format, base_q_idx, tiles, enable_intra_edge_filter, loop filter levels, CDEF */
+    av1mi_frame_params fp;
 } av1mi_frame_out;
 
 /* GpTimes (pipeline.h:180-189): milliseconds from HIP events, and counts */
@@ -127,6 +138,7 @@ typedef struct av1mi_times {
      * loop (2 per fmaf of the column and row passes, 1 per multiply of identity kinds and of the row scale; 64-point
      * sizes keep 32 outputs); 0 otherwise */
     uint64_t flopsAnalysis, flopsRecon;
+    float postfilter;   /* ms of deblocking + CDEF (GpTimes::postfilter) */
 } av1mi_times;
 
 typedef struct av1mi_encoder av1mi_encoder;
@@ -271,7 +283,8 @@ int av1mi_ln_train_step(av1mi_learner *l, uint32_t first, uint32_t n, float lr, 
 
 /* The bitstream of a record (the reference's host writer, av1core/src/encoder.cpp:1072-1659 with avif.cpp):
  * av1mi_write_obus gives the sequence header and frame OBUs of the still picture (default initial CDFs adapted per
- * symbol, no post-filters),
+ * symbol; the loop filter and CDEF parameters of rec->fp when its version is set, each superblock's cdef_idx from its
+ * first coded block),
  * av1mi_write_avif the same wrapped in an AVIF file.  edge_filter: enable_intra_edge_filter of the encode.  *data is
  * malloc'd (release with av1mi_buffer_free).  0, or -1 with a message when the record is not codable. */
 int av1mi_write_obus(const av1mi_frame_out *rec, int edge_filter, void **data, size_t *size, char *err, int errlen);

@@ -91,7 +91,7 @@ void frame_params_to(const av1mi_frame_params &s, FrameParamsT &d) {
 template <class GpTimesT>
 void times_to(const av1mi_times &s, GpTimesT &t) {
     t.setup = s.setup; t.analysis = s.analysis; t.dp = s.dp; t.recon = s.recon; t.pack = s.pack; t.download = s.download;
-    t.total = s.total; t.postfilter = 0;
+    t.total = s.total; t.postfilter = s.postfilter;
     t.steps = s.steps; t.sbs = s.sbs; t.blocks = s.blocks; t.skipBlocks = s.skipBlocks; t.cflBlocks = s.cflBlocks;
     t.rangeFixes = s.rangeFixes; t.txSplit = 0; t.dpRepaired = 0; t.deviceBytes = s.deviceBytes;
 }

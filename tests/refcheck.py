@@ -281,7 +281,9 @@ class Ref:
                        txb=np.frombuffer(C.string_at(r.txb, nt * 16), TXB_DTYPE).copy(),
                        coeffs=np.frombuffer(C.string_at(r.coeffs, nc * 2), np.int16).copy(),
                        base_q_idx=r.fp.base_q_idx, width=r.fp.width, height=r.fp.height, bit_depth=r.fp.bit_depth,
-                       edge_filter=r.fp.enable_intra_edge_filter, n_tiles=p.contents.n_tiles)
+                       edge_filter=r.fp.enable_intra_edge_filter, n_tiles=p.contents.n_tiles,
+                       post={f: (list(getattr(r.fp, f)) if hasattr(getattr(r.fp, f), "__len__") else getattr(r.fp, f))
+                             for f in self.PF_FIELDS})
             return out
         finally:
             self.lib.av1g_parsed_free(p)
@@ -300,14 +302,27 @@ class Ref:
         fp.bit_depth = bd
         return fp
 
-    def encode(self, rec, W, H, qindex, ssx=1, mono=0, bd=8, edge_filter=1, avif=False, layout_check=True):
-        """Decoder-exact reconstruction (and AVIF bytes) of a record; returns (recon planes cropped, avif bytes, info)."""
+    PF_FIELDS = ("loop_filter_level", "loop_filter_sharpness", "enable_cdef", "cdef_damping_minus_3", "cdef_bits", "cdef_y_pri",
+                 "cdef_y_sec", "cdef_uv_pri", "cdef_uv_sec")
+
+    def encode(self, rec, W, H, qindex, ssx=1, mono=0, bd=8, edge_filter=1, avif=False, layout_check=True, post=None):
+        """Decoder-exact reconstruction (and AVIF bytes) of a record; returns (recon planes cropped, avif bytes, info).
+        post: frame parameters whose loop filter / CDEF fields the frame takes (the reconstruction is then the
+        post-filtered one, as the reference's decoder-exact backend applies them)."""
         blocks = np.ascontiguousarray(rec["blocks"])
         block_txb = np.ascontiguousarray(rec["block_txb"]).astype(np.uint32)
         txb = np.ascontiguousarray(rec["txb"]).copy()
         coeffs = np.ascontiguousarray(rec["coeffs"]).copy()
         r = DecisionRecord()
         r.fp = self.frame_params(W, H, qindex, ssx, mono, bd, edge_filter)
+        if post is not None:
+            for f in self.PF_FIELDS:
+                v = getattr(post, f)
+                if hasattr(v, "__len__"):
+                    for k in range(len(v)):
+                        getattr(r.fp, f)[k] = v[k]
+                else:
+                    setattr(r.fp, f, v)
         r.n_blocks = len(blocks)
         if layout_check:   # av1g_layout must reproduce the engine's tx-block layout exactly
             bt2 = np.zeros(len(blocks) + 1, np.uint32)
