@@ -483,6 +483,75 @@ __device__ EvalOut half_eval(RP res, FP tmp, QP q, const E &e, const TT &T) {
     return half_qrate<W, H>(tmp, TS, q, e, T);
 }
 
+// 2-D forward DCT_DCT of a W x H residual with a side of 32 or 64 by the 32 lanes of the group, the residual produced
+// on the fly by res(i, j).  Column pass: lane = column (two columns, j and j + 32, for W = 64), one fmaf chain per kept
+// output over k ascending with the matrix coefficient a wave-uniform scalar operand; row pass: lane = row.  The
+// coefficients (scaled) land in tmp (row i at tmp + i * (W + 1), tw x th kept).  Returns the residual energy sum over the
+// raster order (W or H = 64: the energy the dropped coefficients carry), else 0.
+template <int W, int H, typename Res>
+__device__ float grp_fwd_dct(lds<float> *tmp, const DevTables &T, float scale, Res res) {
+    constexpr int TW = W > 32 ? 32 : W, TH = H > 32 ? 32 : H, TS = W + 1;
+    constexpr int LW = W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
+    constexpr bool E64 = W == 64 || H == 64;
+    constexpr int NC = W == 64 ? 2 : 1;
+    const int lane = glane();
+    cst<DevTables> *TU = uni(&T);
+    cst<float> *FcT = TU->fwdT + TU->fwdOff[K_DCT][LH];
+    cst<float> *FrT = TU->fwdT + TU->fwdOff[K_DCT][LW];
+    float en = 0.f, eo = 0.f;
+    if (lane < W) {
+        float acc[NC][TH];
+#pragma unroll
+        for (int i = 0; i < TH; i++) { acc[0][i] = 0.f; if constexpr (NC == 2) acc[NC - 1][i] = 0.f; }
+#pragma unroll 2
+        for (int k = 0; k < H; k++) {
+            const int r0 = res(k, lane), r1 = NC == 2 ? res(k, lane + 32) : 0;
+            if constexpr (E64) {
+                // raster-order virtual lane l holds elements l, l + 32, ...: W = 64 alternates columns l and l + 32 per
+                // row; W = 32 is column l; W = 16 (16x64) splits column l & 15 into even (lane l) and odd rows (l + 16)
+                if constexpr (W == 64) { en = en + (float)(r0 * r0); en = en + (float)(r1 * r1); }
+                else if constexpr (W == 32) en = en + (float)(r0 * r0);
+                else { if (k & 1) eo = eo + (float)(r0 * r0); else en = en + (float)(r0 * r0); }
+            }
+            const float x0 = (float)r0, x1 = (float)r1;
+            cst<float> *f = FcT + k * H;
+#pragma unroll
+            for (int i = 0; i < TH; i++) {
+                const float fi = f[i];
+                acc[0][i] = fmaf(fi, x0, acc[0][i]);
+                if constexpr (NC == 2) acc[NC - 1][i] = fmaf(fi, x1, acc[NC - 1][i]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < TH; i++) {
+            tmp[i * TS + lane] = acc[0][i];
+            if constexpr (NC == 2) tmp[i * TS + lane + 32] = acc[NC - 1][i];
+        }
+    }
+    if constexpr (E64 && W == 16) {
+        const float o = __shfl_xor(eo, 16, 32);
+        if (lane >= 16) en = o;
+    }
+    gsync();
+    if (lane < TH) {
+        float acc[TW];
+#pragma unroll
+        for (int o = 0; o < TW; o++) acc[o] = 0.f;
+        lds<float> *row = tmp + lane * TS;
+#pragma unroll 2
+        for (int k = 0; k < W; k++) {
+            const float x = row[k];
+            cst<float> *f = FrT + k * W;
+#pragma unroll
+            for (int o = 0; o < TW; o++) acc[o] = fmaf(f[o], x, acc[o]);
+        }
+#pragma unroll
+        for (int o = 0; o < TW; o++) row[o] = acc[o] * scale;
+    }
+    gsync();
+    return E64 ? gsum(en) : 0.f;
+}
+
 // sum of r^2 over a W x H residual in raster order (32 virtual lanes)
 template <int W, int H, typename RP>
 __device__ float grp_res_energy(RP res) {

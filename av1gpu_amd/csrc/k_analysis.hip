@@ -8,13 +8,24 @@
 //
 // k_dp: one thread per 64x64 superblock: bottom-up DP over the 85 square nodes and 10 partition types with the
 // averaged partition bits, then the leaves in decode order.  Reference: encoder/src/k_dp.cuh:72-297.
+#include <type_traits>
 #include "dev_eval.h"
 
 namespace av1m {
 
+// Shapes at least 32 wide try DCT_DCT only: their evaluation takes the forward DCT with the residual made on the fly
+// from the prediction (grp_fwd_dct: wave-uniform matrix operands), so no residual array sits in LDS.
+template <int W, int H>
+constexpr bool an_direct() { return W >= 32; }
+template <int W, int H>
+struct EvalScrDirect {
+    static constexpr int TS = W + 1, TH = H > 32 ? 32 : H;
+    float tmp[TH * TS];
+    int16_t q[(W > 32 ? 32 : W) * TH];
+};
 template <int W, int H>
 struct AnScr {
-    EvalScr<W, H> ev;
+    std::conditional_t<an_direct<W, H>(), EvalScrDirect<W, H>, EvalScr<W, H>> ev;
     uint16_t ea[EDGE_LEN], el[EDGE_LEN];   // source edges
     uint16_t pa[EDGE_LEN], pl[EDGE_LEN];   // edge-filtered copies of the candidate under evaluation
     int satdI[32];
@@ -240,6 +251,30 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
             const int k = st < nK ? st : bestK, t = st < nK ? DCT_DCT : st - nK + 1;
             if (t != DCT_DCT && !tx_in_set(set, t)) continue;
             const int c = S.cand[top[k]], m = cand_mode(c), pa = pangle(m, cand_delta(c));
+            if constexpr (an_direct<W, H>()) {   // DCT_DCT only (st < nK): prediction and forward DCT in one pass
+                const uint16_t *EA = A, *EL = L;
+                int upA = 0, upL = 0;
+                if (I.edgeFilter && is_dir(m)) {
+                    for (int i = lane; i < EDGE_LEN; i += GW) { S.pa[i] = S.ea[i]; S.pl[i] = S.el[i]; }
+                    gsync();
+                    grp_edge_prepare(pa, LW, LH, 0, haveL, haveA, x, y, maxX, maxY, S.pa + EDGE_OFF, S.pl + EDGE_OFF, bd, upA, upL);
+                    EA = S.pa + EDGE_OFF;
+                    EL = S.pl + EDGE_OFF;
+                }
+                auto res = [&](int i, int j) -> int {
+                    const int pr = m == DC_PRED ? dcv : pred_px(m, pa, i, j, LW, LH, EA, EL, upA, upL);
+                    return (int)src[i * sp + j] - pr;
+                };
+                lds<float> *tl = (lds<float> *)(lds<char> *)(char *)S.ev.tmp;
+                e.resEnergy = grp_fwd_dct<W, H>(tl, T, T.txScale[TXS], res);
+                e.txType = DCT_DCT;
+                e.intraDir = m;
+                const EvalOut o = grp_qrate<W, H>(S.ev.tmp, W + 1, S.ev.q, e, T);
+                const float J = fmaf(lambda, (o.R + mbits(c)) + overhead, o.D);
+                if (J < bestJ) { bestJ = J; bestK = k; }
+                gsync();
+                continue;
+            } else {
             if (k != resK) {
                 const uint16_t *EA = A, *EL = L;
                 int upA = 0, upL = 0;
@@ -268,6 +303,7 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
                 if (st < nK) bestK = k; else bestTx = t;
             }
             gsync();
+            }
         }
         if (lane == 0) {
             out.J = nK ? bestJ : INF;
