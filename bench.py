@@ -12,7 +12,8 @@ resident in device memory, the shipped network (av1gpu_amd/shortlist_params.npy)
   bd_rate              BD-rate (PSNR-Y, bits of this build's own writer) of the learned search against the exhaustive
                        one on 12 batch images stratified by size (0.18-24 MP), qindex 60/100/140/180;
   learned_10bit_444    the same batch at 10-bit 4:4:4 (learned and exhaustive, with its BD-rate);
-  serving              Poisson arrivals into the continuous-batching server on the batched+learned path, p50 / p99;
+  serving              Poisson arrivals into the continuous-batching server on the batched+learned path, p50 / p99, at
+                       50 req/s (serving_rates: 50, 200 and 400 req/s);
   cpu_baseline         the CPU restatement (oracle/liborc.so): warm-up + median of 5 on the 512 x 512 configuration
                        (1 thread and all threads) and on the whole 4000 x 3000 image (all threads, up to 16).
 
@@ -96,7 +97,7 @@ def main():
     ap.add_argument("--no-444", action="store_true", help="skip the 10-bit 4:4:4 batched+learned leg")
     ap.add_argument("--no-modes", action="store_true", help="skip the launch-mode leg (streams / hipGraph / autotuner)")
     ap.add_argument("--serve", type=int, default=64, help="requests of the serving leg (0 = skip it)")
-    ap.add_argument("--rate", type=float, default=50.0, help="serving leg: Poisson arrival rate, requests per second")
+    ap.add_argument("--rate", default="50,200,400", help="serving leg: Poisson arrival rates, requests per second (comma list)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only (the path shards by image / tile with no collective); "
@@ -190,7 +191,11 @@ def main():
     if not a.no_444:
         line["learned_10bit_444"] = leg_444(A, learn, params, a.conf, a.batch, a.schedule)
     if a.serve > 0:
-        line["serving"] = serving_leg(A, a.serve, a.rate, (params, a.conf))
+        rates = [float(r) for r in str(a.rate).split(",") if r]
+        legs = [serving_leg(A, a.serve, r, (params, a.conf)) for r in rates]
+        line["serving"] = legs[0]
+        line["serving_rates"] = [{k: v[k] for k in ("requests", "value", "p50_ms", "p99_ms", "batches")} | {"rate": r}
+                                 for r, v in zip(rates, legs)]
     if not a.no_cpu and planes is not None:
         line["cpu_baseline"] = cpu_baseline(A, planes, a.width, a.height, 120)
     print(json.dumps(line), flush=True)
