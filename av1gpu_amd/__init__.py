@@ -40,7 +40,7 @@ class _Params(C.Structure):
 class _Config(C.Structure):
     _fields_ = [("qindex", C.c_int), ("edgeFilter", C.c_int), ("P", _Params), ("schedule", C.c_int), ("queueBlocks", C.c_int),
                 ("anStreams", C.c_int), ("graph", C.c_int), ("tune", C.c_int), ("lfMode", C.c_int), ("lfMul", C.c_float),
-                ("lfLevel", C.c_int)] + [(n, C.c_int) for n in _POSTFILTER_INTS]
+                ("lfLevel", C.c_int)] + [(n, C.c_int) for n in _POSTFILTER_INTS] + [("queueOcc", C.c_int)]
 
 
 class _FrameParams(C.Structure):
@@ -188,6 +188,7 @@ class GpConfig:
     cdefYSec: int = 0
     cdefUVPri: int = 0
     cdefUVSec: int = 0
+    queueOcc: int = 0      # closed-loop workgroups per CU: 0 by the batch's parallelism, 1 or 2 (same results)
 
 
 @dataclass
@@ -413,7 +414,7 @@ class GpEncoder:
             lib().av1mi_destroy(self._h)
             self._h = None
         c = _Config(cfg.qindex, cfg.edgeFilter, cfg.P._c(), cfg.schedule, cfg.queueBlocks, cfg.anStreams, cfg.graph, cfg.tune,
-                    cfg.lfMode, cfg.lfMul, cfg.lfLevel, *[getattr(cfg, n) for n in _POSTFILTER_INTS])
+                    cfg.lfMode, cfg.lfMul, cfg.lfLevel, *[getattr(cfg, n) for n in _POSTFILTER_INTS], cfg.queueOcc)
         h = C.c_void_p()
         err = C.create_string_buffer(512)
         if lib().av1mi_create(C.byref(c), C.byref(h), err, 512) != 0:

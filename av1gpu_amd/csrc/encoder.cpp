@@ -27,8 +27,8 @@ void launch_analysis_on(const Batch &B, const SearchParams &P, const uint32_t *s
 void launch_zero(void *p, size_t bytes, hipStream_t st);
 void launch_dp(const Batch &B, const SearchParams &P, hipStream_t st);
 void launch_recon(const Batch &B, const SearchParams &P, const uint32_t *dSbList, uint32_t n, int bd, hipStream_t st);
-void launch_recon_queue(const Batch &B, const SearchParams &P, const RcQueue &Q, uint32_t blocks, int bd, hipStream_t st);
-int recon_queue_slots(int bd);
+void launch_recon_queue(const Batch &B, const SearchParams &P, const RcQueue &Q, uint32_t blocks, int bd, int wpe, hipStream_t st);
+int recon_queue_slots(int bd, int wpe);
 struct LnNet;
 LnNet *ln_new(const float *params);
 void ln_delete(LnNet *N);
@@ -121,7 +121,7 @@ struct av1mi_encoder {
     int device = 0;
     DevTables *dT = nullptr;
     hipEvent_t ev[10] = {};
-    int cus = 0, slots[2] = {0, 0};   // compute units; resident closed-loop workgroups per CU (8-bit, 10-bit)
+    int cus = 0, slots[2][2] = {{0, 0}, {0, 0}};   // compute units; resident closed-loop workgroups per CU [10-bit][variant]
     uint64_t clockHz = 100000000;     // wall_clock64 rate
     LnNet *ln = nullptr;              // learned pruning network (null: exhaustive search)
     float lnConf = 0.8f;
@@ -262,7 +262,7 @@ int av1mi_create(const av1mi_config *cfg, av1mi_encoder **out, char *err, int er
     free(h);
     if (he != hipSuccess) { snprintf(err, errlen, "HIP: %s (table upload)", hipGetErrorString(he)); delete e; return -1; }
     if (recon_prepare(8) || recon_prepare(10)) { snprintf(err, errlen, "closed-loop kernel needs %zu bytes of LDS", recon_smem_bytes()); delete e; return -1; }
-    if (c.schedule < 0 || c.schedule > 1 || c.queueBlocks < 0) { snprintf(err, errlen, "bad schedule %d / queueBlocks %d", c.schedule, c.queueBlocks); delete e; return -1; }
+    if (c.schedule < 0 || c.schedule > 1 || c.queueBlocks < 0 || c.queueOcc < 0 || c.queueOcc > 2) { snprintf(err, errlen, "bad schedule %d / queueBlocks %d", c.schedule, c.queueBlocks); delete e; return -1; }
     const auto secOk = [](int v) { return v == 0 || v == 1 || v == 2 || v == 4; };
     if (c.lfMode < 0 || c.lfMode > 2 || (c.lfMode == 2 && (c.lfLevel < 0 || c.lfLevel > 63)) || !(c.lfMul >= 0.f && c.lfMul < 1e6f) ||
         c.cdef < 0 || c.cdef > 1 ||
@@ -288,8 +288,8 @@ int av1mi_create(const av1mi_config *cfg, av1mi_encoder **out, char *err, int er
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, e->device) == hipSuccess) e->cus = v;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeWallClockRate, e->device) == hipSuccess && v > 0) e->clockHz = (uint64_t)v * 1000;
-    e->slots[0] = recon_queue_slots(8);
-    e->slots[1] = recon_queue_slots(10);
+    for (int b = 0; b < 2; b++)
+        for (int v = 0; v < 2; v++) e->slots[b][v] = recon_queue_slots(b ? 10 : 8, v + 1);
     for (auto &v : e->ev)
         if ((he = hipEventCreate(&v)) != hipSuccess) { snprintf(err, errlen, "HIP: %s (event)", hipGetErrorString(he)); av1mi_destroy(e); return -1; }
     // default-priority non-blocking streams for the analysis branches and the capture; timing-free events joining them
@@ -641,7 +641,11 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
         anS = (k & 1) ? 4 : 1;
         sched = k >> 1;
     }
-    const int slots = enc->slots[bd == 10] > 0 ? enc->slots[bd == 10] : 1;
+    // closed-loop variant: two workgroups per CU when the batch keeps more superblocks ready per wavefront step than the
+    // device has CUs (throughput), else one with the larger register budget (the latency of a superblock chain)
+    const int wpe = enc->cfg.queueOcc ? enc->cfg.queueOcc
+                  : (uint64_t)nSb >= (uint64_t)std::max(1, enc->cus) * std::max<size_t>(1, steps.size()) ? 2 : 1;
+    const int slots = enc->slots[bd == 10][wpe - 1] > 0 ? enc->slots[bd == 10][wpe - 1] : 1;
     const uint32_t qBlocks = enc->cfg.queueBlocks > 0 ? (uint32_t)enc->cfg.queueBlocks : (uint32_t)std::max(1, enc->cus * slots);
     // ---- device memory: stream-ordered per call, or the captured plan's (graph mode) ----
     bool capture = false;
@@ -655,7 +659,7 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
         A.bind(na->mem);
     } else if (graph) {
         std::vector<uint64_t> key = sig;
-        key.insert(key.end(), {(uint64_t)anS, (uint64_t)sched, (uint64_t)qBlocks, (uint64_t)A.total});
+        key.insert(key.end(), {(uint64_t)anS, (uint64_t)sched, (uint64_t)qBlocks, (uint64_t)A.total, (uint64_t)wpe});
         if (enc->plan.key != key) {
             drop_plan(enc);
             HCK(hipMalloc(&enc->plan.mem, A.total));
@@ -726,7 +730,7 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
             RcQueue Q;
             Q.order = dSteps; Q.n = nSb; Q.head = dQctl; Q.abort = dQctl + 1; Q.done = dDone;
             Q.spinTicks = enc->clockHz * 30;   // no legitimate wait comes near 30 s
-            launch_recon_queue(B, P, Q, std::min<uint32_t>(qBlocks, nSb), bd, s);
+            launch_recon_queue(B, P, Q, std::min<uint32_t>(qBlocks, nSb), bd, wpe, s);
         }
         return hipGetLastError() == hipSuccess ? 0 : -1;
     };

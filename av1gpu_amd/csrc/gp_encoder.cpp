@@ -59,6 +59,7 @@ int GpEncoder::init(const GpConfig &cfg, std::string &err) {
     c.qindex = cfg.qindex;
     c.edgeFilter = cfg.edgeFilter;
     c.schedule = cfg.schedule; c.queueBlocks = cfg.queueBlocks; c.anStreams = cfg.anStreams; c.graph = cfg.graph; c.tune = cfg.tune;
+    c.queueOcc = cfg.queueOcc;
     c.lfMode = cfg.lfMode; c.lfMul = cfg.lfMul; c.lfLevel = cfg.lfLevel;
     c.cdef = cfg.cdef; c.cdefDampingMinus3 = cfg.cdefDampingMinus3; c.cdefYPri = cfg.cdefYPri; c.cdefYSec = cfg.cdefYSec;
     c.cdefUVPri = cfg.cdefUVPri; c.cdefUVSec = cfg.cdefUVSec;

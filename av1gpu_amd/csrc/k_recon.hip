@@ -16,8 +16,15 @@
 namespace av1m {
 
 constexpr int RC_NT = 256, RC_NG = 8;
+// transform teams for tx blocks with a side of 32 or 64 (groups 0 .. nTeam - 1): RC_NTEAM of them inside RcShared keep
+// the workgroup's LDS under half a compute unit's, so two workgroups share a CU; the one-workgroup-per-CU instance adds
+// RC_NTEAM more after RcShared (every group a team)
+constexpr int RC_NTEAM = 4;
 
-constexpr int RC_SMALL_TASKS = 48, RC_SMALL_RES = 16;
+constexpr int RC_SMALL_RES = 16;
+// levels of every task of a tx block up to 16x16, packed by the block's size (task t at t * W * H): at most 6 candidates
+// x 7 tx types of 128 coefficients, 6 x 5 of 256 (16x16: intra set 2), or 2 planes x 7 chroma candidates of 256
+constexpr int RC_LEV_TOTAL = 7680;
 // per-group scratch of the evaluations of transform blocks up to 16x16
 struct RcGroup {
     float tmp[16 * 17];
@@ -28,7 +35,7 @@ struct RcGroup {
 struct RcSmall {
     RcGroup g[RC_NG];
     int16_t res[RC_SMALL_RES][256];
-    int16_t lev[RC_SMALL_TASKS][256];
+    int16_t lev[RC_LEV_TOTAL];
 };
 // scratch of one group evaluating a transform block with a side of 32 or 64 (DCT_DCT only)
 struct RcTeam {
@@ -43,28 +50,28 @@ struct RcTabs {
     int fwdOff[3][7];
     float txScale[19];
     float txg[19][16];
-    uint16_t dscan[DSCAN_TOTAL];   // default scans (offsets dscan_off)
-    uint16_t iscan[DSCAN_TOTAL];   // their inverses: scan index of each position
     uint8_t cbOff[19][25];         // Coeff_Base_Ctx_Offset
+    const uint16_t *iscan;         // the inverse default scans (device tables; read through the caches)
+    const CostTab *cost;           // the cost tables of the image's quantizer context (device tables)
 };
-using RcEvalIn = EvalInT<const lds<CostTab> *>;
+using RcEvalIn = EvalInT<const CostTab *>;
 template <int TSZ>
-__device__ __forceinline__ const lds<uint16_t> *dscan_ptr(const lds<RcTabs> &T) { return T.dscan + dscan_off(av1t::Adjusted_Tx_Size[TSZ]); }
+__device__ __forceinline__ const uint16_t *dscan_ptr(const lds<RcTabs> &) { return scan_of(TSZ, DCT_DCT); }
 template <int TSZ>
 __device__ __forceinline__ const lds<uint8_t> *cboff_ptr(const lds<RcTabs> &T) { return T.cbOff[TSZ]; }
 template <int TSZ>
-__device__ __forceinline__ const lds<uint16_t> *iscan_ptr(const lds<RcTabs> &T) { return T.iscan + dscan_off(av1t::Adjusted_Tx_Size[TSZ]); }
+__device__ __forceinline__ const uint16_t *iscan_ptr(const lds<RcTabs> &T) { return T.iscan + dscan_off(av1t::Adjusted_Tx_Size[TSZ]); }
 
 union RcScr {
     RcSmall s;
-    RcTeam t[RC_NG];
+    RcTeam t[RC_NTEAM];
 };
 struct RcShared {
     RcScr u;
-    CostTab cost;                            // the cost tables of the image's quantizer context
     RcTabs tab;
     uint16_t srcb[64 * 64];                  // source block of the current leaf: luma W x H, or U at 0 and V at 2048
-    int32_t inv[64 * 64];                    // inverse transform residual
+    int16_t inv[64 * 64];                    // inverse transform residual (the row pass clamps to 16 bits; the column
+                                             // outputs of a block passing the range checks are within 12)
     uint16_t ea[3][EDGE_LEN], el[3][EDGE_LEN];   // unfiltered edges of the current tx block per plane
     int dcv[3], ft[3], hvL[3], hvA[3], px[3], py[3], mx[3], my[3];
     int skipCtx[3], dcCtx[3];
@@ -77,17 +84,26 @@ struct RcShared {
     float gBestJ[RC_NG];
     int gBestT[RC_NG];
     int win, winEob[3];
-    int cflL[32 * 32];
+    int16_t cflL[32 * 32];                   // CfL luma (at most 8 x 1023)
     int cflAvg, cflAlpha[2];
     unsigned long long acc64[8];
     int satdI[32], uvc[32];
     float uvJ[32];                           // screening costs of the UV candidates
     int nTxb, nCoef, errFlag, nzFlag, eobOut, culOut;
+    int nTeam;                               // transform teams: RC_NTEAM, or 2 * RC_NTEAM with the extra ones after RcShared
     int flopsOn;                             // count the forward-transform work (Batch::flops set)
     unsigned long long flops;                // of the superblock so far
     int am, lm, mode, delta, uvMode, uvDelta, nuv, bsize;
     uint32_t qItem;                          // persistent schedule: the superblock taken
 };
+
+// scratch of transform team g: the first RC_NTEAM share the union with the small-block scratch, the others follow RcShared
+__device__ __forceinline__ lds<RcTeam> &team_scr(lds<RcShared> &S, int g) {
+    return g < RC_NTEAM ? S.u.t[g] : ((lds<RcTeam> *)(lds<char> *)(&S + 1))[g - RC_NTEAM];
+}
+__device__ __forceinline__ const lds<RcTeam> &team_scr(const lds<RcShared> &S, int g) {
+    return g < RC_NTEAM ? S.u.t[g] : ((const lds<RcTeam> *)(const lds<char> *)(&S + 1))[g - RC_NTEAM];
+}
 
 template <typename Px> __device__ __forceinline__ Px *rec_plane(const Img &I, int p) { return (Px *)I.rec[p]; }
 template <typename Px> __device__ __forceinline__ const Px *src_plane(const Img &I, int p) { return (const Px *)I.src[p]; }
@@ -200,7 +216,7 @@ __device__ EvalOut grp_task(lds<RcShared> &S, lds<RcGroup> &X, const Img &I, int
     e.skipCtx = S.skipCtx[p]; e.dcCtx = S.dcCtx[p]; e.intraDir = p == 0 ? cand_mode(c) : 0; e.bd = I.bd;
     e.rnd = p ? P.rndC : P.rnd; e.zbin = p ? P.zbinC : P.zbin;
     e.resEnergy = 0.f;
-    e.C = &S.cost;
+    e.C = S.tab.cost;
     return grp_eval<W, H>(res, X.tmp, lev, e, S.tab);
 }
 // the same task by half the group (blocks of up to 64 coefficients), with tmp its own part of the group's scratch
@@ -213,7 +229,7 @@ __device__ EvalOut half_task(lds<RcShared> &S, lds<float> *tmp, const Img &I, in
     e.skipCtx = S.skipCtx[p]; e.dcCtx = S.dcCtx[p]; e.intraDir = p == 0 ? cand_mode(c) : 0; e.bd = I.bd;
     e.rnd = p ? P.rndC : P.rnd; e.zbin = p ? P.zbinC : P.zbin;
     e.resEnergy = 0.f;
-    e.C = &S.cost;
+    e.C = S.tab.cost;
     return half_eval<W, H>(res, tmp, lev, e, S.tab);
 }
 
@@ -250,14 +266,14 @@ __device__ EvalOut grp_task_big(lds<RcShared> &S, lds<RcTeam> &X, const Img &I, 
     e.dcQ = dc_q(I.qindex, bd); e.acQ = ac_q(I.qindex, bd);
     e.skipCtx = S.skipCtx[p]; e.dcCtx = S.dcCtx[p]; e.intraDir = p == 0 ? cand_mode(c) : 0; e.bd = bd;
     e.rnd = p ? P.rndC : P.rnd; e.zbin = p ? P.zbinC : P.zbin;
-    e.C = &S.cost;
+    e.C = S.tab.cost;
     return grp_qrate<W, H>(X.tmp, W + 1, X.q, e, S.tab);
 }
 
 // ---- exact inverse transform + reconstruction by the workgroup (spec 7.13.3, 7.12.3 step 3) ----
 template <int LW>
 __device__ __forceinline__ uint32_t inv_row(int i, int W, int th, int tw, const lds<int16_t> *q, int dcQ, int acQ, int dsh, int bd, bool rect, int rk,
-                            int rowShift, lds<int32_t> *res) {
+                            int rowShift, lds<int16_t> *res) {
     constexpr int N = 1 << LW;
     int32_t Tv[N];
 #pragma unroll
@@ -270,11 +286,11 @@ __device__ __forceinline__ uint32_t inv_row(int i, int W, int th, int tw, const 
     Inv::run<LW>(Tv, rk, bd + 8, err);
     const int colR = bd + 6 > 16 ? bd + 6 : 16;
 #pragma unroll
-    for (int j = 0; j < N; j++) res[i * W + j] = clip3(-(1 << (colR - 1)), (1 << (colR - 1)) - 1, (int32_t)round2l(Tv[j], rowShift));
+    for (int j = 0; j < N; j++) res[i * W + j] = (int16_t)clip3(-(1 << (colR - 1)), (1 << (colR - 1)) - 1, (int32_t)round2l(Tv[j], rowShift));
     return err;
 }
 template <int LH>
-__device__ __forceinline__ uint32_t inv_col(int j, int W, int ck, int bd, lds<int32_t> *res) {
+__device__ __forceinline__ uint32_t inv_col(int j, int W, int ck, int bd, lds<int16_t> *res) {
     constexpr int N = 1 << LH;
     int32_t Tv[N];
 #pragma unroll
@@ -282,7 +298,7 @@ __device__ __forceinline__ uint32_t inv_col(int j, int W, int ck, int bd, lds<in
     uint32_t err = 0;
     Inv::run<LH>(Tv, ck, bd + 6 > 16 ? bd + 6 : 16, err);
 #pragma unroll
-    for (int i = 0; i < N; i++) res[i * W + j] = (int32_t)round2l(Tv[i], 4);
+    for (int i = 0; i < N; i++) res[i * W + j] = (int16_t)round2l(Tv[i], 4);
     return err;
 }
 
@@ -369,7 +385,7 @@ __device__ int wg_record_txb(lds<RcShared> &S, const Batch &B, uint32_t sb, cons
         // eob: 1 + the largest scan index of a nonzero level (inverse default scan; rows for class VERT, columns for
         // HORIZ); cul: the sum of |level| before the eob, i.e. over every level (later ones are 0), capped at 63
         const int cls = tx_class(txType), tw = tx_w(txSz) > 32 ? 32 : tx_w(txSz), th = tx_h(txSz) > 32 ? 32 : tx_h(txSz);
-        const lds<uint16_t> *isc = S.tab.iscan + dscan_off(av1t::Adjusted_Tx_Size[txSz]);
+        const uint16_t *isc = S.tab.iscan + dscan_off(av1t::Adjusted_Tx_Size[txSz]);
         int last = 0, cul = 0;
         for (int i = tid; i < n; i += 32) {
             const int v = q[i];
@@ -423,8 +439,13 @@ __device__ void wg_run_tasks(lds<RcShared> &S, const Img &I, const DevTables &T,
                              bool pickWinner) {
     constexpr bool big = W > 16 || H > 16;
     constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
+    if constexpr (!big) {   // the packed levels hold every task: 6 luma candidates x the set's types, or 14 chroma tasks
+        constexpr int set = tx_set_intra(tx_of(W, H));
+        constexpr int nt = set == TX_SET_INTRA_1 ? 7 : set == TX_SET_INTRA_2 ? 5 : 1;
+        static_assert((6 * nt > 14 ? 6 * nt : 14) * W * H <= RC_LEV_TOTAL, "the packed levels cannot hold every task");
+    }
     const int g = gid(), lane = glane();
-    const lds<CostTab> &C = S.cost;
+    const CostTab &C = *S.tab.cost;
     // blocks of up to 64 coefficients: 16 half groups, two tasks per group at once; luma residuals are ready, chroma
     // tasks first make theirs (one task per group, as below), then the halves evaluate
     bool done = false;
@@ -439,7 +460,7 @@ __device__ void wg_run_tasks(lds<RcShared> &S, const Img &I, const DevTables &T,
         const int h = lane >> 4;
         for (int t = 2 * g + h; t < S.nTask; t += 2 * RC_NG) {
             const int c = S.taskC[t], tt = S.taskT[t], p = S.taskP[t];
-            const EvalOut o = half_task<W, H>(S, S.u.s.g[g].tmp + h * 136, I, p, c, tt, S.u.s.res[lumaBits ? S.taskK[t] : t], S.u.s.lev[t], P);
+            const EvalOut o = half_task<W, H>(S, S.u.s.g[g].tmp + h * 136, I, p, c, tt, S.u.s.res[lumaBits ? S.taskK[t] : t], S.u.s.lev + t * (W * H), P);
             float J;
             if (lumaBits) {
                 const int m = cand_mode(c);
@@ -453,15 +474,16 @@ __device__ void wg_run_tasks(lds<RcShared> &S, const Img &I, const DevTables &T,
             gsync();
         }
     }
-    if (!done) for (int t = g; t < S.nTask; t += RC_NG) {
+    const int NGT = big ? S.nTeam : RC_NG;   // big tasks: the teams (groups 0 .. nTeam - 1)
+    if (!done && g < NGT) for (int t = g; t < S.nTask; t += NGT) {
         const int c = S.taskC[t], tt = S.taskT[t], p = S.taskP[t];
         EvalOut o;
         if constexpr (big) {
-            o = grp_task_big<W, H>(S, S.u.t[g], I, T, p, c, stage_src(S, p), W, P);
+            o = grp_task_big<W, H>(S, team_scr(S, g), I, T, p, c, stage_src(S, p), W, P);
         } else {
             if (!lumaBits)   // chroma: this task's own residual
                 grp_residual(S, I.edgeFilter ? S.u.s.g[g].ea : nullptr, S.u.s.g[g].el, p, c, LW, LH, stage_src(S, p), W, S.u.s.res[t], I.bd);
-            o = grp_task<W, H>(S, S.u.s.g[g], I, p, c, tt, S.u.s.res[lumaBits ? S.taskK[t] : t], S.u.s.lev[t], P);
+            o = grp_task<W, H>(S, S.u.s.g[g], I, p, c, tt, S.u.s.res[lumaBits ? S.taskK[t] : t], S.u.s.lev + t * (W * H), P);
         }
         float J;
         if (lumaBits) {
@@ -507,16 +529,16 @@ __device__ void wg_luma_residuals(lds<RcShared> &S, const Img &I) {
 // is t, or it evaluates t again
 template <int W, int H>
 __device__ lds<int16_t> *wg_final_levels(lds<RcShared> &S, const Img &I, const DevTables &T, const SearchParams &P, int t, int &team) {
-    team = t % RC_NG;
+    team = (W > 16 || H > 16) ? t % S.nTeam : t % RC_NG;
     if constexpr (W > 16 || H > 16) {
-        if (t + RC_NG < S.nTask) {   // overwritten by a later task of the same team
+        if (t + S.nTeam < S.nTask) {   // overwritten by a later task of the same team
             const int p = S.taskP[t];
-            if (gid() == team) grp_task_big<W, H>(S, S.u.t[team], I, T, p, S.taskC[t], stage_src(S, p), W, P);
+            if (gid() == team) grp_task_big<W, H>(S, team_scr(S, team), I, T, p, S.taskC[t], stage_src(S, p), W, P);
             __syncthreads();
         }
-        return S.u.t[team].q;
+        return team_scr(S, team).q;
     } else {
-        return S.u.s.lev[t];
+        return S.u.s.lev + t * (W * H);
     }
 }
 
@@ -526,7 +548,7 @@ template <int W, int H, typename Px>
 __device__ void wg_final_pred(const lds<RcShared> &S, const Img &I, int p, int t, int k, int team, Px *dst, int pitch) {
     constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
     if constexpr (W > 16 || H > 16) {
-        wg_write_pred<Px>(S, I.edgeFilter ? S.u.t[team].ea : nullptr, S.u.t[team].el, team, p, S.taskC[t], LW, LH, dst, pitch, I.bd);
+        wg_write_pred<Px>(S, I.edgeFilter ? team_scr(S, team).ea : nullptr, team_scr(S, team).el, team, p, S.taskC[t], LW, LH, dst, pitch, I.bd);
     } else {
         const lds<uint16_t> *src = stage_src(S, p);
         const lds<int16_t> *res = S.u.s.res[k];
@@ -619,7 +641,7 @@ __device__ void rc_chroma(lds<RcShared> &S, const Batch &B, const SearchParams &
                           int availUC, int availLC, int tx0, int ty0, float lambda) {
     constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
     const DevTables &T = *B.T;
-    const lds<CostTab> &C = S.cost;
+    const CostTab &C = *S.tab.cost;
     const int tid = threadIdx.x, ss = I.ss, bd = I.bd, txSz = tx_of(W, H);
     const int cx = (mi_col >> ss) * 4, cy = (mi_row >> ss) * 4, x4 = cx >> 2, y4 = cy >> 2;
     const int sbr4 = (mi_row & 15) >> ss, sbc4 = (mi_col & 15) >> ss;
@@ -984,9 +1006,7 @@ __device__ void rc_chroma(lds<RcShared> &S, const Batch &B, const SearchParams &
 // stage the cost tables of quantizer context qctx (and, when first, the small forward matrices) in LDS
 __device__ void stage_tables(lds<RcShared> &S, const DevTables &T, int qctx, bool matrices) {
     const int tid = threadIdx.x;
-    const float *gc = (const float *)&T.cost[qctx];
-    lds<float> *lc = (lds<float> *)&S.cost;
-    for (int k = tid; k < (int)(sizeof(CostTab) / 4); k += RC_NT) lc[k] = gc[k];
+    if (tid == 0) { S.tab.cost = &T.cost[qctx]; S.tab.iscan = T.iscan; }
     if (matrices) {
         for (int k = tid; k < 3 * 336; k += RC_NT) {
             const int kind = k / 336, r = k % 336, ln = r < 16 ? 2 : r < 80 ? 3 : 4, e = r - (ln == 2 ? 0 : ln == 3 ? 16 : 80);
@@ -996,12 +1016,6 @@ __device__ void stage_tables(lds<RcShared> &S, const DevTables &T, int qctx, boo
         if (tid < 21) S.tab.fwdOff[tid / 7][tid % 7] = (tid / 7) * 336 + ((tid % 7) == 3 ? 16 : (tid % 7) == 4 ? 80 : 0);
         if (tid < 19) S.tab.txScale[tid] = T.txScale[tid];
         for (int k = tid; k < 19 * 16; k += RC_NT) S.tab.txg[k / 16][k % 16] = T.txg[k / 16][k % 16];
-        for (int t = 0; t < 19; t++) {
-            if (av1t::Tx_Width[t] == 64 || av1t::Tx_Height[t] == 64) continue;
-            const uint16_t *src = scan_of(t, DCT_DCT);
-            const int n = av1t::Tx_Width[t] * av1t::Tx_Height[t], off = dscan_off(t);
-            for (int k = tid; k < n; k += RC_NT) { S.tab.dscan[off + k] = src[k]; S.tab.iscan[off + src[k]] = (uint16_t)k; }
-        }
         for (int k = tid; k < 19 * 25; k += RC_NT) S.tab.cbOff[k / 25][k % 25] = (&av1t::Coeff_Base_Ctx_Offset[0][0][0])[k];
     }
 }
@@ -1099,16 +1113,19 @@ __device__ void recon_sb(lds<RcShared> &S, const Batch &B, const SearchParams &P
     }
 }
 
-// One workgroup per compute unit (the LDS footprint allows no more), so the compiler is told to spend registers on
-// latency (batched LDS loads) instead of keeping them free for occupancy it cannot have.
-#define AV1M_RC_KERNEL __global__ void __launch_bounds__(RC_NT) __attribute__((amdgpu_waves_per_eu(1, 1)))
+// RcShared allows two workgroups per compute unit.  WPE 2 holds the registers to 256 so both fit (more superblocks in
+// flight: batches); WPE 1 is one workgroup per CU, which spends the registers and the rest of the LDS on the latency of
+// one superblock (fewer spills, eight transform teams: a single image, whose closed loop is bound by its chain of
+// superblocks).  The same results either way.
+#define AV1M_RC_KERNEL(WPE) __global__ void __launch_bounds__(RC_NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 
 // one launch per wavefront step: workgroup b takes superblock sbList[b]
 template <typename Px>
-AV1M_RC_KERNEL k_recon(Batch B, SearchParams P, const uint32_t *sbList) {
+AV1M_RC_KERNEL(2) k_recon(Batch B, SearchParams P, const uint32_t *sbList) {
     extern __shared__ __align__(16) char smraw[];
     lds<RcShared> &S = *(lds<RcShared> *)(lds<char> *)smraw;
     const uint32_t sb = sbList[blockIdx.x];
+    if (threadIdx.x == 0) S.nTeam = RC_NTEAM;
     stage_tables(S, *B.T, B.img[B.sbImg[sb]].qctx, true);
     __syncthreads();
     recon_sb<Px>(S, B, P, sb);
@@ -1134,12 +1151,13 @@ __device__ bool wait_done(const uint32_t *f, const RcQueue &Q) {
 // neighbours inside its tile are done.  Items are taken in an order where every dependency was taken earlier by a
 // running workgroup, so the waits always end; a wait that exceeds the time limit raises the abort flag, which makes
 // every workgroup stop taking items and return.
-template <typename Px>
-AV1M_RC_KERNEL k_recon_queue(Batch B, SearchParams P, RcQueue Q) {
+template <typename Px, int WPE>
+AV1M_RC_KERNEL(WPE) k_recon_queue(Batch B, SearchParams P, RcQueue Q) {
     extern __shared__ __align__(16) char smraw[];
     lds<RcShared> &S = *(lds<RcShared> *)(lds<char> *)smraw;
     const int tid = threadIdx.x;
     int staged = -1;
+    if (tid == 0) S.nTeam = WPE == 1 ? 2 * RC_NTEAM : RC_NTEAM;
     for (;;) {
         if (tid == 0) {
             uint32_t it = Q.n;
@@ -1180,36 +1198,53 @@ AV1M_RC_KERNEL k_recon_queue(Batch B, SearchParams P, RcQueue Q) {
     }
 }
 
-static_assert(sizeof(RcShared) <= 160 * 1024, "the closed loop's LDS exceeds a compute unit's 160 KiB");
-size_t recon_smem_bytes() { return sizeof(RcShared); }
+static_assert(sizeof(RcShared) <= 80 * 1024, "the closed loop's LDS exceeds half a compute unit's 160 KiB");
+static_assert(alignof(RcShared) >= alignof(RcTeam) && sizeof(RcShared) + RC_NTEAM * sizeof(RcTeam) <= 160 * 1024,
+              "the eight-team instance's LDS exceeds a compute unit's");
+// dynamic LDS of the instance at WPE workgroups per CU
+constexpr size_t rc_smem(int wpe) { return sizeof(RcShared) + (wpe == 1 ? RC_NTEAM * sizeof(RcTeam) : 0); }
+size_t recon_smem_bytes() { return rc_smem(1); }
 
 int recon_prepare(int bd) {
-    const int sz = (int)sizeof(RcShared);
-    const hipError_t e = bd == 10 ? hipFuncSetAttribute((const void *)k_recon<uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize, sz)
-                                  : hipFuncSetAttribute((const void *)k_recon<uint8_t>, hipFuncAttributeMaxDynamicSharedMemorySize, sz);
-    const hipError_t f = bd == 10 ? hipFuncSetAttribute((const void *)k_recon_queue<uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize, sz)
-                                  : hipFuncSetAttribute((const void *)k_recon_queue<uint8_t>, hipFuncAttributeMaxDynamicSharedMemorySize, sz);
+    const int s2 = (int)rc_smem(2), s1 = (int)rc_smem(1);
+    const hipError_t e = bd == 10 ? hipFuncSetAttribute((const void *)k_recon<uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize, s2)
+                                  : hipFuncSetAttribute((const void *)k_recon<uint8_t>, hipFuncAttributeMaxDynamicSharedMemorySize, s2);
+    hipError_t f = bd == 10 ? hipFuncSetAttribute((const void *)k_recon_queue<uint16_t, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, s1)
+                            : hipFuncSetAttribute((const void *)k_recon_queue<uint8_t, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, s1);
+    if (f == hipSuccess)
+        f = bd == 10 ? hipFuncSetAttribute((const void *)k_recon_queue<uint16_t, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, s2)
+                     : hipFuncSetAttribute((const void *)k_recon_queue<uint8_t, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, s2);
     return e == hipSuccess && f == hipSuccess ? 0 : -1;
 }
 
 void launch_recon(const Batch &B, const SearchParams &P, const uint32_t *dSbList, uint32_t n, int bd, hipStream_t st) {
     if (!n) return;
-    if (bd == 10) hipLaunchKernelGGL(k_recon<uint16_t>, dim3(n), dim3(RC_NT), sizeof(RcShared), st, B, P, dSbList);
-    else hipLaunchKernelGGL(k_recon<uint8_t>, dim3(n), dim3(RC_NT), sizeof(RcShared), st, B, P, dSbList);
+    if (bd == 10) hipLaunchKernelGGL(k_recon<uint16_t>, dim3(n), dim3(RC_NT), rc_smem(2), st, B, P, dSbList);
+    else hipLaunchKernelGGL(k_recon<uint8_t>, dim3(n), dim3(RC_NT), rc_smem(2), st, B, P, dSbList);
 }
 
 // resident workgroups of the persistent kernel per compute unit
-int recon_queue_slots(int bd) {
+int recon_queue_slots(int bd, int wpe) {
     int n = 0;
-    const hipError_t e = bd == 10 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_recon_queue<uint16_t>, RC_NT, sizeof(RcShared))
-                                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_recon_queue<uint8_t>, RC_NT, sizeof(RcShared));
+    hipError_t e;
+    if (wpe == 2)
+        e = bd == 10 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_recon_queue<uint16_t, 2>, RC_NT, rc_smem(2))
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_recon_queue<uint8_t, 2>, RC_NT, rc_smem(2));
+    else
+        e = bd == 10 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_recon_queue<uint16_t, 1>, RC_NT, rc_smem(1))
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_recon_queue<uint8_t, 1>, RC_NT, rc_smem(1));
     return e == hipSuccess ? n : 0;
 }
 
-void launch_recon_queue(const Batch &B, const SearchParams &P, const RcQueue &Q, uint32_t blocks, int bd, hipStream_t st) {
+void launch_recon_queue(const Batch &B, const SearchParams &P, const RcQueue &Q, uint32_t blocks, int bd, int wpe, hipStream_t st) {
     if (!Q.n || !blocks) return;
-    if (bd == 10) hipLaunchKernelGGL(k_recon_queue<uint16_t>, dim3(blocks), dim3(RC_NT), sizeof(RcShared), st, B, P, Q);
-    else hipLaunchKernelGGL(k_recon_queue<uint8_t>, dim3(blocks), dim3(RC_NT), sizeof(RcShared), st, B, P, Q);
+    if (wpe == 2) {
+        if (bd == 10) hipLaunchKernelGGL((k_recon_queue<uint16_t, 2>), dim3(blocks), dim3(RC_NT), rc_smem(2), st, B, P, Q);
+        else hipLaunchKernelGGL((k_recon_queue<uint8_t, 2>), dim3(blocks), dim3(RC_NT), rc_smem(2), st, B, P, Q);
+    } else {
+        if (bd == 10) hipLaunchKernelGGL((k_recon_queue<uint16_t, 1>), dim3(blocks), dim3(RC_NT), rc_smem(1), st, B, P, Q);
+        else hipLaunchKernelGGL((k_recon_queue<uint8_t, 1>), dim3(blocks), dim3(RC_NT), rc_smem(1), st, B, P, Q);
+    }
 }
 
 }  // namespace av1m

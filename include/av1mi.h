@@ -95,6 +95,10 @@ typedef struct av1mi_config {
     float lfMul;
     int lfLevel;
     int cdef, cdefDampingMinus3, cdefYPri, cdefYSec, cdefUVPri, cdefUVSec;
+    /* closed-loop workgroups per compute unit of the persistent schedule (no counterpart in the reference): 0 = by the
+     * batch's parallelism (2 when it keeps more superblocks ready per wavefront step than the device has CUs), 1 or 2.
+     * Results do not depend on it. */
+    int queueOcc;
 } av1mi_config;
 
 /* GpFrameIn (pipeline.h:149-163): device planes padded (edge replicated) to aw x ah, multiples of 64; chroma

@@ -73,7 +73,7 @@ struct GpConfig {
     int yuv444 = 0, depth = 8;
     int lossless = 0, losslessCfl = 1, llTileArea = 262144, ycgcoRe = 0, alpha = 1, alphaQindex = -1;
     int palette = 1;   /* 0 off, 1 frames that pass screen-content detection (photos never do: no palettes), 2 refused */
-    int schedule = 0, queueBlocks = 0, anStreams = 1, graph = 0, tune = 0;
+    int schedule = 0, queueBlocks = 0, anStreams = 1, graph = 0, tune = 0, queueOcc = 0;
 };
 
 /* GpFrameIn (pipeline.h:149-163) plus a per-frame qindex (0: the config's) */

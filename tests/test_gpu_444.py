@@ -41,15 +41,20 @@ def test_gpu_444_equals_oracle(enc, oracle, W, H, seed, q, bd):
 
 
 def test_gpu_444_batch_and_decodes(enc, ref):
-    """A 4:4:4 batch of mixed sizes and qindex: each record equals its image alone; the reference's backend and dav1d
-    decode each to the GPU reconstruction."""
+    """A 4:4:4 batch of mixed sizes and qindex: each record equals its image alone and the two-workgroups-per-CU closed
+    loop's; the reference's backend and dav1d decode each to the GPU reconstruction."""
     specs = [(264, 200, 21, 120), (96, 64, 22, 255), (200, 136, 23, 80)]
     frames = [A.DeviceFrame(A.synth_image(s, W, H, ssx=0), W, H, qindex=q, yuv444=True) for W, H, s, q in specs]
     alone = [enc.encode([f])[0][0] for f in frames]
     together = enc.encode(frames)[0]
-    for a, b, (W, H, s, q) in zip(alone, together, specs):
+    e2 = A.GpEncoder().init(A.GpConfig(qindex=120, queueOcc=2))   # the two-workgroups-per-CU closed loop
+    other = e2.encode(frames)[0]
+    e2.close()
+    for a, b, c, (W, H, s, q) in zip(alone, together, other, specs):
         for k in ("blocks", "txb", "coeffs"):
-            assert np.array_equal(getattr(a, k), getattr(b, k)), k
+            assert np.array_equal(getattr(a, k), getattr(b, k)) and np.array_equal(getattr(a, k), getattr(c, k)), k
+        for p in range(3):
+            assert np.array_equal(b.recon[p], c.recon[p])
         dec, _, _ = ref.encode(dict(blocks=b.blocks, block_txb=b.block_txb, txb=b.txb, coeffs=b.coeffs), W, H, q, ssx=0)
         for p in range(3):
             assert np.array_equal(b.recon[p][:dec[p].shape[0], :dec[p].shape[1]], dec[p])

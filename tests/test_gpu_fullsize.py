@@ -3,8 +3,8 @@
 * the 4000 x 3000 benchmark image (2 tile rows, 2961 superblocks) equals the oracle on every record array and the
   whole reconstruction;
 * frames wider than 4096 samples (2 and 4 tile columns) equal the oracle under every closed-loop schedule (the
-  persistent queue with every slot and with 3 workgroups, and one launch per wavefront step), which exercises the
-  queue's waits at tile edges;
+  persistent queue's one- and two-workgroups-per-CU instances, with every slot and with 3 workgroups, and one launch
+  per wavefront step), which exercises the queue's waits at tile edges;
 * a 21 MP frame of 4 tile rows decodes (reference backend, dav1d) to the GPU reconstruction;
 * a size-stratified subset of the mixed 0.18-24 MP batch, including its largest images at qindex 80 and 160, encoded
   in one call on the batched+learned path and exhaustively: each record equals its image encoded alone and decodes to
@@ -86,9 +86,9 @@ def test_gpu_tile_columns_equal_oracle_in_every_schedule(oracle, W, H, seed, col
     planes = A.synth_image(seed, W, H)
     r = oracle.encode(planes, W, H, qindex=120, threads=THREADS)
     frame = A.DeviceFrame(planes, W, H, qindex=120)
-    for sched, qb in ((0, 0), (0, 3), (1, 0)):
-        print("  schedule", sched, "queueBlocks", qb, flush=True)
-        e = A.GpEncoder().init(A.GpConfig(qindex=120, schedule=sched, queueBlocks=qb))
+    for sched, qb, occ in ((0, 0, 1), (0, 0, 2), (0, 3, 2), (1, 0, 0)):
+        print("  schedule", sched, "queueBlocks", qb, "queueOcc", occ, flush=True)
+        e = A.GpEncoder().init(A.GpConfig(qindex=120, schedule=sched, queueBlocks=qb, queueOcc=occ))
         g, t = e.encode([frame])
         e.close()
         assert g[0].tile_cols == cols and t.schedule == sched

@@ -107,14 +107,15 @@ def test_gpu_full_size_record_decodes(enc, ref):
 
 def test_gpu_schedules_agree(enc):
     """The persistent queue (every resident slot, and only 3 workgroups, which makes most superblocks wait on their
-    neighbours) and one launch per wavefront step give identical records for a batch of mixed sizes and qindex."""
+    neighbours; its one- and two-workgroups-per-CU instances) and one launch per wavefront step give identical records
+    for a batch of mixed sizes and qindex."""
     specs = [(1000, 700, 1080, 120), (300, 200, 1081, 80), (64, 64, 1082, 160), (520, 130, 1083, 120)]
     frames = [A.DeviceFrame(A.synth_image(s, W, H), W, H, qindex=q) for W, H, s, q in specs]
     base, tq = enc.encode(frames)
     assert tq.schedule == 0 and tq.queueBlocks > 0
-    for sched, qb in ((1, 0), (0, 3)):
-        e = A.GpEncoder().init(A.GpConfig(qindex=120, schedule=sched, queueBlocks=qb))
-        print("schedule", sched, "queueBlocks", qb, flush=True)
+    for sched, qb, occ in ((1, 0, 0), (0, 3, 0), (0, 0, 1), (0, 0, 2), (0, 5, 2)):
+        e = A.GpEncoder().init(A.GpConfig(qindex=120, schedule=sched, queueBlocks=qb, queueOcc=occ))
+        print("schedule", sched, "queueBlocks", qb, "queueOcc", occ, flush=True)
         out, t = e.encode(frames)
         assert t.schedule == sched
         for a, b in zip(base, out):
