@@ -58,13 +58,18 @@ __device__ __forceinline__ int ol_decoded(int px, int py, int x, int y, const Ti
     return zorder16(((px - sbx) >> 2) & 15, ((py - sby) >> 2) & 15) < zorder16(((x - sbx) >> 2) & 15, ((y - sby) >> 2) & 15);
 }
 
-// Occupancy target 4 waves per SIMD: the compiler then keeps the kernels under 128 VGPRs, and the extra waves hide the
-// LDS and table latency (measured on MI355X: analysis of the 12 MP image 68.5 -> 56.7 ms).  The shapes of 32x32 and
-// up are limited by LDS instead (the compiler notes that it cannot reach the target there; the build silences it).
+// Occupancy target per shape, in waves per SIMD.  At 4 the compiler keeps a kernel under 128 VGPRs and the extra waves
+// hide the LDS and table latency, which outweighs the spills that costs the small shapes (measured on MI355X, kernel
+// trace of the 256-image batch at targets 2 .. 6: 4 is the fastest for most shapes and 3 or 5 for the few below,
+// analysis 1119 -> 1096 ms).  The shapes of 32x32 and up are limited by LDS instead (the compiler notes that it cannot
+// reach the target there; the build silences it).
+constexpr int an_wpe(int W, int H) {
+    return (W == 16 && H == 32) || (W == 32 && H == 64) ? 3 : (W == 8 && H == 16) || (W == 16 && H == 8) ? 5 : 4;
+}
 // HOOKS: the instance that reads an external screener's candidates and / or a block list (AnHook); the default instance
 // has neither, so their registers do not weigh on it.
 template <int W, int H, int BS, typename Px, bool HOOKS>
-__global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_waves_per_eu(4))) k_analysis(Batch B, SearchParams P,
+__global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_waves_per_eu(an_wpe(W, H)))) k_analysis(Batch B, SearchParams P,
                                                                                                       uint32_t nTotal, AnHook hk) {
     constexpr int NG = an_groups(W, H), TXS = tx_of(W, H);
     constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;

@@ -429,13 +429,15 @@ __device__ int wg_record_txb(lds<RcShared> &S, const Batch &B, uint32_t sb, cons
 // source of plane p in the staging buffer
 __device__ __forceinline__ const lds<uint16_t> *stage_src(const lds<RcShared> &S, int p) { return S.srcb + (p == 2 ? 2048 : 0); }
 
+// Inlined into each per-shape leaf function (rc_luma / rc_chroma): as a call of its own it would run on the ~140
+// registers outside the callee-saved ones and spill; inlined, the leaf function's one frame covers it.
 // run S.taskC/T/P[0 .. nTask) on the 8 groups (the source block staged in S.srcb, row stride W); per task
 // J = fmaf(lambda, R + extra, D) with extra = luma mode bits (plane 0) or 0; S.taskJ / taskE.  Then S.win = the first
 // minimum over the tasks.  Blocks up to 16x16: luma tasks read the residual of their candidate (S.taskK), made by
 // wg_luma_residuals; chroma tasks make their own in slot t; levels of task t go to S.u.s.lev[t].  Larger blocks:
 // team g keeps the levels and prepared edges of its last task.
 template <int W, int H>
-__device__ void wg_run_tasks(lds<RcShared> &S, const Img &I, const DevTables &T, const SearchParams &P, float lambda, bool lumaBits,
+__device__ __forceinline__ void wg_run_tasks(lds<RcShared> &S, const Img &I, const DevTables &T, const SearchParams &P, float lambda, bool lumaBits,
                              bool pickWinner) {
     constexpr bool big = W > 16 || H > 16;
     constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
@@ -528,7 +530,7 @@ __device__ void wg_luma_residuals(lds<RcShared> &S, const Img &I) {
 // the levels of the winning task t, and (larger blocks) the team holding them with its prepared edges: its last task
 // is t, or it evaluates t again
 template <int W, int H>
-__device__ lds<int16_t> *wg_final_levels(lds<RcShared> &S, const Img &I, const DevTables &T, const SearchParams &P, int t, int &team) {
+__device__ __forceinline__ lds<int16_t> *wg_final_levels(lds<RcShared> &S, const Img &I, const DevTables &T, const SearchParams &P, int t, int &team) {
     team = (W > 16 || H > 16) ? t % S.nTeam : t % RC_NG;
     if constexpr (W > 16 || H > 16) {
         if (t + S.nTeam < S.nTask) {   // overwritten by a later task of the same team
@@ -1119,12 +1121,30 @@ __device__ void recon_sb(lds<RcShared> &S, const Batch &B, const SearchParams &P
 // superblocks).  The same results either way.
 #define AV1M_RC_KERNEL(WPE) __global__ void __launch_bounds__(RC_NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 
+// The kernels' one parameter, read in place in the kernarg segment (it is the segment's first argument): handing a
+// reference to a by-value kernel parameter to the out-of-line superblock code would make the compiler copy it into
+// every lane's scratch, and every read of a field a per-lane scratch load.
+struct RcStepArgs {
+    Batch B;
+    SearchParams P;
+    const uint32_t *sbList;
+};
+struct RcQueueArgs {
+    Batch B;
+    SearchParams P;
+    RcQueue Q;
+};
+template <typename A> __device__ __forceinline__ const A &kernel_args() { return *(const A *)__builtin_amdgcn_kernarg_segment_ptr(); }
+
 // one launch per wavefront step: workgroup b takes superblock sbList[b]
 template <typename Px>
-AV1M_RC_KERNEL(2) k_recon(Batch B, SearchParams P, const uint32_t *sbList) {
+AV1M_RC_KERNEL(2) k_recon(RcStepArgs) {
     extern __shared__ __align__(16) char smraw[];
     lds<RcShared> &S = *(lds<RcShared> *)(lds<char> *)smraw;
-    const uint32_t sb = sbList[blockIdx.x];
+    const RcStepArgs &A = kernel_args<RcStepArgs>();
+    const Batch &B = A.B;
+    const SearchParams &P = A.P;
+    const uint32_t sb = A.sbList[blockIdx.x];
     if (threadIdx.x == 0) S.nTeam = RC_NTEAM;
     stage_tables(S, *B.T, B.img[B.sbImg[sb]].qctx, true);
     __syncthreads();
@@ -1152,9 +1172,13 @@ __device__ bool wait_done(const uint32_t *f, const RcQueue &Q) {
 // running workgroup, so the waits always end; a wait that exceeds the time limit raises the abort flag, which makes
 // every workgroup stop taking items and return.
 template <typename Px, int WPE>
-AV1M_RC_KERNEL(WPE) k_recon_queue(Batch B, SearchParams P, RcQueue Q) {
+AV1M_RC_KERNEL(WPE) k_recon_queue(RcQueueArgs) {
     extern __shared__ __align__(16) char smraw[];
     lds<RcShared> &S = *(lds<RcShared> *)(lds<char> *)smraw;
+    const RcQueueArgs &A = kernel_args<RcQueueArgs>();
+    const Batch &B = A.B;
+    const SearchParams &P = A.P;
+    const RcQueue &Q = A.Q;
     const int tid = threadIdx.x;
     int staged = -1;
     if (tid == 0) S.nTeam = WPE == 1 ? 2 * RC_NTEAM : RC_NTEAM;
@@ -1219,8 +1243,9 @@ int recon_prepare(int bd) {
 
 void launch_recon(const Batch &B, const SearchParams &P, const uint32_t *dSbList, uint32_t n, int bd, hipStream_t st) {
     if (!n) return;
-    if (bd == 10) hipLaunchKernelGGL(k_recon<uint16_t>, dim3(n), dim3(RC_NT), rc_smem(2), st, B, P, dSbList);
-    else hipLaunchKernelGGL(k_recon<uint8_t>, dim3(n), dim3(RC_NT), rc_smem(2), st, B, P, dSbList);
+    const RcStepArgs a{B, P, dSbList};
+    if (bd == 10) hipLaunchKernelGGL(k_recon<uint16_t>, dim3(n), dim3(RC_NT), rc_smem(2), st, a);
+    else hipLaunchKernelGGL(k_recon<uint8_t>, dim3(n), dim3(RC_NT), rc_smem(2), st, a);
 }
 
 // resident workgroups of the persistent kernel per compute unit
@@ -1238,12 +1263,13 @@ int recon_queue_slots(int bd, int wpe) {
 
 void launch_recon_queue(const Batch &B, const SearchParams &P, const RcQueue &Q, uint32_t blocks, int bd, int wpe, hipStream_t st) {
     if (!Q.n || !blocks) return;
+    const RcQueueArgs a{B, P, Q};
     if (wpe == 2) {
-        if (bd == 10) hipLaunchKernelGGL((k_recon_queue<uint16_t, 2>), dim3(blocks), dim3(RC_NT), rc_smem(2), st, B, P, Q);
-        else hipLaunchKernelGGL((k_recon_queue<uint8_t, 2>), dim3(blocks), dim3(RC_NT), rc_smem(2), st, B, P, Q);
+        if (bd == 10) hipLaunchKernelGGL((k_recon_queue<uint16_t, 2>), dim3(blocks), dim3(RC_NT), rc_smem(2), st, a);
+        else hipLaunchKernelGGL((k_recon_queue<uint8_t, 2>), dim3(blocks), dim3(RC_NT), rc_smem(2), st, a);
     } else {
-        if (bd == 10) hipLaunchKernelGGL((k_recon_queue<uint16_t, 1>), dim3(blocks), dim3(RC_NT), rc_smem(1), st, B, P, Q);
-        else hipLaunchKernelGGL((k_recon_queue<uint8_t, 1>), dim3(blocks), dim3(RC_NT), rc_smem(1), st, B, P, Q);
+        if (bd == 10) hipLaunchKernelGGL((k_recon_queue<uint16_t, 1>), dim3(blocks), dim3(RC_NT), rc_smem(1), st, a);
+        else hipLaunchKernelGGL((k_recon_queue<uint8_t, 1>), dim3(blocks), dim3(RC_NT), rc_smem(1), st, a);
     }
 }
 
