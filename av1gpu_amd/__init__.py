@@ -77,7 +77,7 @@ class _Times(C.Structure):
                [(n, C.c_uint32) for n in ("steps", "sbs", "blocks", "skipBlocks", "cflBlocks", "rangeFixes")] + \
                [("deviceBytes", C.c_size_t), ("schedule", C.c_uint32), ("queueBlocks", C.c_uint32), ("anStreams", C.c_uint32),
                 ("graph", C.c_uint32), ("tuned", C.c_uint32), ("flopsAnalysis", C.c_uint64), ("flopsRecon", C.c_uint64),
-                ("postfilter", C.c_float)]
+                ("postfilter", C.c_float), ("queueOcc", C.c_uint32)]
 
 
 _lib = None
@@ -176,7 +176,7 @@ class GpConfig:
     queueBlocks: int = 0   # persistent workgroups, 0 = every resident slot
     anStreams: int = 1     # analysis shape launches spread over 1..4 streams
     graph: int = 0         # capture the device part per batch shape as a hipGraph and replay it
-    tune: int = 0          # the first 4 calls of a batch shape try anStreams 1/4 x schedule 0/1, then keep the fastest
+    tune: int = 0          # the first 6 calls of a batch shape try anStreams 1/4 x closed loop (queue at 1 or 2 WG/CU, steps)
     # post-filters (pipeline.h:60-66): lfMode 0 off / 1 level from each frame's qindex (times lfMul) / 2 fixed lfLevel;
     # cdef 1 = one strength set (damping cdefDampingMinus3 + 3, primary 0..15, secondary 0/1/2/4)
     lfMode: int = 0
@@ -249,6 +249,7 @@ class GpTimes:
     flopsAnalysis: int = 0
     flopsRecon: int = 0
     postfilter: float = 0
+    queueOcc: int = 0
 
 
 def check_partitions(P: GpParams):

@@ -157,9 +157,12 @@ struct av1mi_encoder {
         void *mem = nullptr;
         hipGraphExec_t exec = nullptr;
     } plan;
-    struct Tune {                     // autotuner state of one batch shape: candidates tried, their device times
+    // autotuner state of one batch shape: candidates tried, their device times.  Candidate k: analysis streams 1 (k
+    // even) or 4; closed loop k >> 1 = 0 persistent queue at one workgroup per CU, 1 at two, 2 one launch per step
+    static constexpr int NTUNE = 6;
+    struct Tune {
         int next = 0, best = 0;
-        float ms[4] = {0, 0, 0, 0};
+        float ms[NTUNE] = {};
     };
     std::map<std::vector<uint64_t>, Tune> tunes;
 };
@@ -633,18 +636,18 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
     for (int i = 0; i < n; i++)
         sig.insert(sig.end(), {(uint64_t)imgs[i].W, (uint64_t)imgs[i].H, (uint64_t)imgs[i].aw, (uint64_t)imgs[i].ah, (uint64_t)imgs[i].ss,
                                (uint64_t)imgs[i].mono});
-    int anS = std::max(1, enc->cfg.anStreams), sched = enc->cfg.schedule, cand = -1;
+    int anS = std::max(1, enc->cfg.anStreams), sched = enc->cfg.schedule, cand = -1, occ = enc->cfg.queueOcc;
     av1mi_encoder::Tune *tu = nullptr;
     if (enc->cfg.tune) {
         tu = &enc->tunes[sig];
-        const int k = tu->next < 4 ? (cand = tu->next) : tu->best;
+        const int k = tu->next < av1mi_encoder::NTUNE ? (cand = tu->next) : tu->best;
         anS = (k & 1) ? 4 : 1;
-        sched = k >> 1;
+        sched = (k >> 1) == 2 ? 1 : 0;
+        occ = (k >> 1) == 2 ? 0 : (k >> 1) + 1;
     }
-    // closed-loop variant: two workgroups per CU when the batch keeps more superblocks ready per wavefront step than the
+    // closed-loop instance: two workgroups per CU when the batch keeps more superblocks ready per wavefront step than the
     // device has CUs (throughput), else one with the larger register budget (the latency of a superblock chain)
-    const int wpe = enc->cfg.queueOcc ? enc->cfg.queueOcc
-                  : (uint64_t)nSb >= (uint64_t)std::max(1, enc->cus) * std::max<size_t>(1, steps.size()) ? 2 : 1;
+    const int wpe = occ ? occ : (uint64_t)nSb >= (uint64_t)std::max(1, enc->cus) * std::max<size_t>(1, steps.size()) ? 2 : 1;
     const int slots = enc->slots[bd == 10][wpe - 1] > 0 ? enc->slots[bd == 10][wpe - 1] : 1;
     const uint32_t qBlocks = enc->cfg.queueBlocks > 0 ? (uint32_t)enc->cfg.queueBlocks : (uint32_t)std::max(1, enc->cus * slots);
     // ---- device memory: stream-ordered per call, or the captured plan's (graph mode) ----
@@ -770,6 +773,7 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
     tm.anStreams = (uint32_t)anS;
     tm.graph = (uint32_t)graph;
     tm.tuned = tu && cand < 0 ? 1u : 0u;
+    tm.queueOcc = sched == 1 ? 0u : (uint32_t)wpe;
     (void)hipEventRecord(enc->ev[4], st);
     tm.steps = (uint32_t)steps.size();
     tm.sbs = nSb;
@@ -947,10 +951,10 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
     tm.setup = ms[0]; tm.analysis = ms[1]; tm.dp = ms[2]; tm.recon = ms[3]; tm.pack = ms[5];
     tm.download = ms[6] + std::max(0.f, ms[4] - tm.postfilter);
     (void)hipEventElapsedTime(&tm.total, enc->ev[0], enc->ev[7]);
-    if (cand >= 0) {   // autotuner: the device part's time of this candidate; after the 4th, keep the fastest
+    if (cand >= 0) {   // autotuner: the device part's time of this candidate; after the last, keep the fastest
         tu->ms[cand] = ms[1] + ms[2] + ms[3];
-        if (++tu->next == 4)
-            for (int k = 1; k < 4; k++) if (tu->ms[k] < tu->ms[tu->best]) tu->best = k;
+        if (++tu->next == av1mi_encoder::NTUNE)
+            for (int k = 1; k < av1mi_encoder::NTUNE; k++) if (tu->ms[k] < tu->ms[tu->best]) tu->best = k;
     }
     tm.blocks = stats[0]; tm.skipBlocks = stats[1]; tm.cflBlocks = stats[2]; tm.rangeFixes = stats[3];
     for (int k = 0; k < 32; k++) { tm.flopsAnalysis += flops[k]; tm.flopsRecon += flops[32 + k]; }

@@ -304,14 +304,14 @@ def bd_leg(A, learn, params, conf, spec, bplanes, n=12, qs=(60, 100, 140, 180)):
 
 def modes_leg(A, learn, params, conf, n=16, W=512, H=512, calls=10):
     """Launch modes on a workload where launches matter: batches of n small images (the same batch shape every call,
-    learned pruning on): eager on one stream, analysis over 4 streams, hipGraph replay, and the autotuner (its 4 trial
+    learned pruning on): eager on one stream, analysis over 4 streams, hipGraph replay, and the autotuner (its 6 trial
     calls in the warm-up).  Each mode: untimed calls until warm, then `calls` timed calls."""
     frames = [A.DeviceFrame(A.synth_image(3000 + i, W, H), W, H, qindex=120) for i in range(n)]
     mp = n * W * H / 1e6
     res = {"workload": "%d images of %dx%d 8-bit 4:2:0 (seeds 3000..%d), qindex 120, learned pruning, the same batch every "
                        "call" % (n, W, H, 3000 + n - 1), "megapixels_per_call": mp}
     for name, kw, warm in (("eager_1_stream", {}, 2), ("eager_4_streams", {"anStreams": 4}, 2),
-                           ("graph_4_streams", {"anStreams": 4, "graph": 1}, 2), ("tuned_graph", {"tune": 1, "graph": 1}, 5)):
+                           ("graph_4_streams", {"anStreams": 4, "graph": 1}, 2), ("tuned_graph", {"tune": 1, "graph": 1}, 7)):
         enc = A.GpEncoder().init(A.GpConfig(qindex=120, **kw))
         learn.set_shortlist(enc, params, conf)
         for _ in range(warm):
@@ -324,7 +324,7 @@ def modes_leg(A, learn, params, conf, n=16, W=512, H=512, calls=10):
         dt = (time.perf_counter() - t0) / calls
         res[name] = {"ms_per_call": round(dt * 1e3, 3), "value": round(mp / dt, 3), "unit": "MP/s",
                      "device_ms": round(t.analysis + t.dp + t.recon, 3), "anStreams": t.anStreams, "schedule": t.schedule,
-                     "graph": t.graph, "tuned": t.tuned}
+                     "queueOcc": t.queueOcc, "graph": t.graph, "tuned": t.tuned}
         enc.close()
     for f in frames:
         f.close()

@@ -82,8 +82,8 @@ typedef struct av1mi_config {
     /* anStreams: the 19 analysis shape launches spread over 1..4 streams (0 = 1).  graph: capture the device part
      * of an encode (zeroing, pruning network, analysis, DP, closed loop) once per batch shape (image sizes, bit depth,
      * chroma format) as a hipGraph and replay it; per-image qindex and plane pointers may change between calls.
-     * tune: the first 4 calls of each batch shape try anStreams 1 / 4 x schedule 0 / 1 and keep the fastest.  None of
-     * these changes a record. */
+     * tune: the first 6 calls of each batch shape try anStreams 1 / 4 x closed loop (persistent queue at one or two
+     * workgroups per CU, or one launch per step) and keep the fastest.  None of these changes a record. */
     int anStreams;
     int graph;
     int tune;
@@ -143,6 +143,7 @@ typedef struct av1mi_times {
      * sizes keep 32 outputs); 0 otherwise */
     uint64_t flopsAnalysis, flopsRecon;
     float postfilter;   /* ms of deblocking + CDEF (GpTimes::postfilter) */
+    uint32_t queueOcc;  /* persistent closed loop: the instance used, 1 or 2 workgroups per CU (0: per-step schedule) */
 } av1mi_times;
 
 typedef struct av1mi_encoder av1mi_encoder;

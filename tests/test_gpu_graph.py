@@ -86,20 +86,21 @@ def test_graph_with_learned_pruning():
 
 
 @pytest.mark.parametrize("graph", [0, 1])
-def test_autotuner_tries_four_candidates_then_keeps_one(graph):
+def test_autotuner_tries_six_candidates_then_keeps_one(graph):
     f = _frames()
     ref = _eager(f)
     e = A.GpEncoder().init(A.GpConfig(qindex=120, tune=1, graph=graph))
     seen = set()
-    for call in range(6):
+    for call in range(8):
         out, t = e.encode(f)
         _same(ref, out)
-        if call < 4:
+        key = (t.anStreams, t.schedule, t.queueOcc)
+        if call < 6:
             assert t.tuned == 0
-            seen.add((t.anStreams, t.schedule))
+            seen.add(key)
         else:
-            assert t.tuned == 1 and (t.anStreams, t.schedule) in seen
-    assert seen == {(1, 0), (4, 0), (1, 1), (4, 1)}
+            assert t.tuned == 1 and key in seen
+    assert seen == {(s, 0, occ) for s in (1, 4) for occ in (1, 2)} | {(1, 1, 0), (4, 1, 0)}
     e.close()
     for x in f:
         x.close()
