@@ -9,7 +9,7 @@
 #include <hip/hip_runtime.h>
 #define AV1M_HD __host__ __device__ __forceinline__
 #else
-#define AV1M_HD static inline
+#define AV1M_HD inline
 #endif
 
 namespace av1m {
@@ -505,6 +505,68 @@ AV1M_HD int pred_px(int mode, int pa, int i, int j, int lw, int lh, const P *A, 
     const int a = A[j], l = L[i], tl = A[-1], base = a + l - tl;   // PAETH
     const int pL = base > l ? base - l : l - base, pT = base > a ? base - a : a - base, pTL = base > tl ? base - tl : tl - base;
     return (pL <= pT && pL <= pTL) ? l : pT <= pTL ? a : tl;
+}
+// CH predicted samples of column j, rows i0, i0 + di, ..., i0 + (CH - 1) di, of any mode (DC_PRED: dcv): pred_px's
+// cases with the mode and zone tests made once for the CH samples, so a group whose lanes share the mode branches once
+// per run of samples instead of once per sample
+template <int CH, typename P>
+AV1M_HD void pred_col(int mode, int pa, int dcv, int i0, int di, int j, int lw, int lh, const P *A, const P *L, int upA, int upL,
+                      int (&o)[CH]) {
+    const int w = 1 << lw, h = 1 << lh;
+    if (mode == DC_PRED) {
+        for (int c = 0; c < CH; c++) o[c] = dcv;
+    } else if (is_dir(mode)) {
+        if (pa == 90) {
+            const int v = A[j];
+            for (int c = 0; c < CH; c++) o[c] = v;
+        } else if (pa == 180) {
+            for (int c = 0; c < CH; c++) o[c] = L[i0 + c * di];
+        } else if (pa < 90) {
+            const int dx = av1t::Dr_Intra_Derivative[pa], mb = (w + h - 1) << upA, am = A[mb];
+            for (int c = 0; c < CH; c++) {
+                const int idx = (i0 + c * di + 1) * dx;
+                const int base = (idx >> (6 - upA)) + (j << upA), sh = ((idx << upA) >> 1) & 0x1F;
+                o[c] = base < mb ? (A[base] * (32 - sh) + A[base + 1] * sh + 16) >> 5 : am;
+            }
+        } else if (pa < 180) {
+            const int dx = av1t::Dr_Intra_Derivative[180 - pa], dy = av1t::Dr_Intra_Derivative[pa - 90];
+            for (int c = 0; c < CH; c++) {
+                const int i = i0 + c * di;
+                const int ia = (j << 6) - (i + 1) * dx;
+                if ((ia >> (6 - upA)) >= -(1 << upA)) {
+                    const int base = ia >> (6 - upA), sh = ((ia * (1 << upA)) >> 1) & 0x1F;
+                    o[c] = (A[base] * (32 - sh) + A[base + 1] * sh + 16) >> 5;
+                } else {
+                    const int il = (i << 6) - (j + 1) * dy;
+                    const int base = il >> (6 - upL), sh = ((il * (1 << upL)) >> 1) & 0x1F;
+                    o[c] = (L[base] * (32 - sh) + L[base + 1] * sh + 16) >> 5;
+                }
+            }
+        } else {
+            const int dy = av1t::Dr_Intra_Derivative[270 - pa], idx = (j + 1) * dy, sh = ((idx << upL) >> 1) & 0x1F;
+            for (int c = 0; c < CH; c++) {
+                const int base = (idx >> (6 - upL)) + ((i0 + c * di) << upL);
+                o[c] = (L[base] * (32 - sh) + L[base + 1] * sh + 16) >> 5;
+            }
+        }
+    } else if (mode == PAETH_PRED) {
+        const int a = A[j], tl = A[-1];
+        for (int c = 0; c < CH; c++) {
+            const int l = L[i0 + c * di], base = a + l - tl;
+            const int pL = base > l ? base - l : l - base, pT = base > a ? base - a : a - base, pTL = base > tl ? base - tl : tl - base;
+            o[c] = (pL <= pT && pL <= pTL) ? l : pT <= pTL ? a : tl;
+        }
+    } else {
+        // the SMOOTH modes: (x + 128) >> 8 = (2x + 256) >> 9, so SMOOTH_V / SMOOTH_H are SMOOTH with the other
+        // direction's pair replaced by a second copy of their own
+        const uint8_t *wx = sm_weights(lw), *wy = sm_weights(lh);
+        const int a = A[j], lb = L[h - 1], ar = A[w - 1], wj = wx[j];
+        for (int c = 0; c < CH; c++) {
+            const int i = i0 + c * di, wi = wy[i], l = L[i];
+            const int vv = wi * a + (256 - wi) * lb, hh = wj * l + (256 - wj) * ar;
+            o[c] = (mode == SMOOTH_PRED ? vv + hh : mode == SMOOTH_V_PRED ? 2 * vv : 2 * hh) + 256 >> 9;
+        }
+    }
 }
 AV1M_HD int edge_strength(int w, int h, int ft, int delta) {
     const int d = delta < 0 ? -delta : delta, s = w + h;

@@ -231,7 +231,7 @@ struct EvalScr {
 };
 
 template <int W, int H, typename FP, typename QP, typename E, typename TT>
-__device__ EvalOut grp_qrate(FP tmp, int ts, QP q, const E &e, const TT &T);
+__device__ __forceinline__ EvalOut grp_qrate(FP tmp, int ts, QP q, const E &e, const TT &T);
 
 // Forward transform + quantization + distortion + rate of the residual res (W x H, row-major), by the 32 lanes of
 // the calling group, with tmp (th x (W + 1) floats) as scratch.  Levels into q (natural order, tw x th).  All lanes
@@ -302,7 +302,7 @@ __device__ EvalOut grp_eval(RP res, FP tmp, QP q, const E &e, const TT &T) {
 // Quantization, distortion and rate of the coefficients of a W x H tx block held in tmp (row i of the kept tw x th
 // region at tmp + i * ts), levels into q (natural order, tw x th).  All lanes of the group return the same values.
 template <int W, int H, typename FP, typename QP, typename E, typename TT>
-__device__ EvalOut grp_qrate(FP tmp, int ts, QP q, const E &e, const TT &T) {
+__device__ __forceinline__ EvalOut grp_qrate(FP tmp, int ts, QP q, const E &e, const TT &T) {
     constexpr int TW = W > 32 ? 32 : W, TH = H > 32 ? 32 : H;
     constexpr int LTW = TW == 4 ? 2 : TW == 8 ? 3 : TW == 16 ? 4 : 5;
     constexpr int N = TW * TH;
@@ -484,16 +484,17 @@ __device__ EvalOut half_eval(RP res, FP tmp, QP q, const E &e, const TT &T) {
 }
 
 // 2-D forward DCT_DCT of a W x H residual with a side of 32 or 64 by the 32 lanes of the group, the residual produced
-// on the fly by res(i, j).  Column pass: lane = column (two columns, j and j + 32, for W = 64), one fmaf chain per kept
+// on the fly: res(k0, j, r) fills r[0 .. CH) with rows k0 .. k0 + CH - 1 of column j.  Column pass: lane = column (two columns, j and j + 32, for W = 64), one fmaf chain per kept
 // output over k ascending with the matrix coefficient a wave-uniform scalar operand; row pass: lane = row.  The
 // coefficients (scaled) land in tmp (row i at tmp + i * (W + 1), tw x th kept).  Returns the residual energy sum over the
 // raster order (W or H = 64: the energy the dropped coefficients carry), else 0.
 template <int W, int H, typename Res>
-__device__ float grp_fwd_dct(lds<float> *tmp, const DevTables &T, float scale, Res res) {
+__device__ __forceinline__ float grp_fwd_dct(lds<float> *tmp, const DevTables &T, float scale, Res res) {
     constexpr int TW = W > 32 ? 32 : W, TH = H > 32 ? 32 : H, TS = W + 1;
     constexpr int LW = W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
     constexpr bool E64 = W == 64 || H == 64;
     constexpr int NC = W == 64 ? 2 : 1;
+    constexpr int CH = H < 8 ? H : 8;   // residual rows made per call of res (its mode tests once per run)
     const int lane = glane();
     cst<DevTables> *TU = uni(&T);
     cst<float> *FcT = TU->fwdT + TU->fwdOff[K_DCT][LH];
@@ -503,23 +504,29 @@ __device__ float grp_fwd_dct(lds<float> *tmp, const DevTables &T, float scale, R
         float acc[NC][TH];
 #pragma unroll
         for (int i = 0; i < TH; i++) { acc[0][i] = 0.f; if constexpr (NC == 2) acc[NC - 1][i] = 0.f; }
-#pragma unroll 2
-        for (int k = 0; k < H; k++) {
-            const int r0 = res(k, lane), r1 = NC == 2 ? res(k, lane + 32) : 0;
-            if constexpr (E64) {
-                // raster-order virtual lane l holds elements l, l + 32, ...: W = 64 alternates columns l and l + 32 per
-                // row; W = 32 is column l; W = 16 (16x64) splits column l & 15 into even (lane l) and odd rows (l + 16)
-                if constexpr (W == 64) { en = en + (float)(r0 * r0); en = en + (float)(r1 * r1); }
-                else if constexpr (W == 32) en = en + (float)(r0 * r0);
-                else { if (k & 1) eo = eo + (float)(r0 * r0); else en = en + (float)(r0 * r0); }
-            }
-            const float x0 = (float)r0, x1 = (float)r1;
-            cst<float> *f = FcT + k * H;
+        for (int k0 = 0; k0 < H; k0 += CH) {
+            int ra[CH], rb[CH];
+            res(k0, lane, ra);
+            if constexpr (NC == 2) res(k0, lane + 32, rb);
 #pragma unroll
-            for (int i = 0; i < TH; i++) {
-                const float fi = f[i];
-                acc[0][i] = fmaf(fi, x0, acc[0][i]);
-                if constexpr (NC == 2) acc[NC - 1][i] = fmaf(fi, x1, acc[NC - 1][i]);
+            for (int kk = 0; kk < CH; kk++) {
+                const int k = k0 + kk, r0 = ra[kk], r1 = NC == 2 ? rb[kk] : 0;
+                if constexpr (E64) {
+                    // raster-order virtual lane l holds elements l, l + 32, ...: W = 64 alternates columns l and l + 32
+                    // per row; W = 32 is column l; W = 16 (16x64) splits column l & 15 into even (lane l) and odd rows
+                    // (l + 16)
+                    if constexpr (W == 64) { en = en + (float)(r0 * r0); en = en + (float)(r1 * r1); }
+                    else if constexpr (W == 32) en = en + (float)(r0 * r0);
+                    else { if (k & 1) eo = eo + (float)(r0 * r0); else en = en + (float)(r0 * r0); }
+                }
+                const float x0 = (float)r0, x1 = (float)r1;
+                cst<float> *f = FcT + k * H;
+#pragma unroll
+                for (int i = 0; i < TH; i++) {
+                    const float fi = f[i];
+                    acc[0][i] = fmaf(fi, x0, acc[0][i]);
+                    if constexpr (NC == 2) acc[NC - 1][i] = fmaf(fi, x1, acc[NC - 1][i]);
+                }
             }
         }
 #pragma unroll

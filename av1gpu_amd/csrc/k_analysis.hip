@@ -266,9 +266,11 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
                     EA = S.pa + EDGE_OFF;
                     EL = S.pl + EDGE_OFF;
                 }
-                auto res = [&](int i, int j) -> int {
-                    const int pr = m == DC_PRED ? dcv : pred_px(m, pa, i, j, LW, LH, EA, EL, upA, upL);
-                    return (int)src[i * sp + j] - pr;
+                auto res = [&](int i0, int j, auto &r) {
+                    pred_col(m, pa, dcv, i0, 1, j, LW, LH, EA, EL, upA, upL, r);
+                    constexpr int CH = sizeof(r) / sizeof(r[0]);
+#pragma unroll
+                    for (int c = 0; c < CH; c++) r[c] = (int)src[(i0 + c) * sp + j] - r[c];
                 };
                 lds<float> *tl = (lds<float> *)(lds<char> *)(char *)S.ev.tmp;
                 e.resEnergy = grp_fwd_dct<W, H>(tl, T, T.txScale[TXS], res);

@@ -252,12 +252,16 @@ __device__ EvalOut grp_task_big(lds<RcShared> &S, lds<RcTeam> &X, const Img &I, 
     }
     if (lane == 0) { S.gup[gid()][0] = upA; S.gup[gid()][1] = upL; }
     const int dcv = S.dcv[p], alpha = c == 255 ? S.cflAlpha[p - 1] : 0, cavg = S.cflAvg;
-    auto res = [&](int i, int j) -> int {
-        int pr;
-        if (c == 255) pr = clip3(0, (1 << bd) - 1, dcv + round2signed(alpha * (S.cflL[i * W + j] - cavg), 6));
-        else if (m == DC_PRED) pr = dcv;
-        else pr = pred_px(m, pa, i, j, LW, LH, EA, EL, upA, upL);
-        return (int)src[i * sp + j] - pr;
+    auto res = [&](int i0, int j, auto &r) {
+        constexpr int CH = sizeof(r) / sizeof(r[0]);
+        if (c == 255) {
+#pragma unroll
+            for (int k = 0; k < CH; k++) r[k] = clip3(0, (1 << bd) - 1, dcv + round2signed(alpha * (S.cflL[(i0 + k) * W + j] - cavg), 6));
+        } else {
+            pred_col(m, pa, dcv, i0, 1, j, LW, LH, EA, EL, upA, upL, r);
+        }
+#pragma unroll
+        for (int k = 0; k < CH; k++) r[k] = (int)src[(i0 + k) * sp + j] - r[k];
     };
     const int txSz = tx_of(W, H);
     RcEvalIn e;
