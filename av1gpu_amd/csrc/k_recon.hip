@@ -175,11 +175,15 @@ __device__ void rc_edges(lds<RcShared> &S, const Img &I, int p, int px, int py, 
     }
 }
 
-// prediction of candidate c (255 = CfL) of plane p for a W x H tx block, by the group, into res = src - pred; the group's
-// edge buffers receive the prepared (filtered) edges of directional modes.  Returns nothing; S.gup[gid] = flags.
-__device__ void grp_residual(lds<RcShared> &S, lds<uint16_t> *gea, lds<uint16_t> *gel, int p, int c, int lw, int lh,
-                             const lds<uint16_t> *src, int sp, lds<int16_t> *res, int bd) {
-    const int W = 1 << lw, H = 1 << lh, lane = glane();
+// prediction of candidate c (255 = CfL) of plane p for a W x H tx block (W, H <= 16), by the group, into res = src -
+// pred (raster order: lane l makes samples l, l + 32, ..., i.e. column l % W, rows l / W + k * 32 / W, as one run of
+// pred_col); the group's edge buffers receive the prepared (filtered) edges of directional modes.  S.gup[gid] = flags.
+template <int W, int H>
+__device__ void grp_residual(lds<RcShared> &S, lds<uint16_t> *gea, lds<uint16_t> *gel, int p, int c, const lds<uint16_t> *src, int sp,
+                             lds<int16_t> *res, int bd) {
+    constexpr int lw = W == 4 ? 2 : W == 8 ? 3 : 4, lh = H == 4 ? 2 : H == 8 ? 3 : 4;
+    constexpr int NPX = W * H, CH = NPX >= GW ? NPX / GW : 1, DI = GW / W;
+    const int lane = glane();
     const int m = c == 255 ? DC_PRED : cand_mode(c), pa = pangle(m, c == 255 ? 0 : cand_delta(c));
     const lds<uint16_t> *EA = S.ea[p] + EDGE_OFF, *EL = S.el[p] + EDGE_OFF;
     int upA = 0, upL = 0;
@@ -193,13 +197,21 @@ __device__ void grp_residual(lds<RcShared> &S, lds<uint16_t> *gea, lds<uint16_t>
         EL = gel + EDGE_OFF;
     }
     const int dcv = S.dcv[p];
-    for (int q = lane; q < W * H; q += GW) {
-        const int i = q >> lw, j = q & (W - 1);
-        int pr;
-        if (c == 255) pr = clip3(0, (1 << bd) - 1, dcv + round2signed(S.cflAlpha[p - 1] * (S.cflL[i * W + j] - S.cflAvg), 6));
-        else if (m == DC_PRED) pr = dcv;
-        else pr = pred_px(m, pa, i, j, lw, lh, EA, EL, upA, upL);
-        res[q] = (int16_t)((int)src[i * sp + j] - pr);
+    if (lane < NPX) {
+        const int i0 = lane >> lw, j = lane & (W - 1);
+        int r[CH];
+        if (c == 255) {
+            const int alpha = S.cflAlpha[p - 1], avg = S.cflAvg;
+#pragma unroll
+            for (int k = 0; k < CH; k++) r[k] = clip3(0, (1 << bd) - 1, dcv + round2signed(alpha * (S.cflL[(i0 + k * DI) * W + j] - avg), 6));
+        } else {
+            pred_col(m, pa, dcv, i0, DI, j, lw, lh, EA, EL, upA, upL, r);
+        }
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            const int i = i0 + k * DI;
+            res[i * W + j] = (int16_t)((int)src[i * sp + j] - r[k]);
+        }
     }
     if (lane == 0) { S.gup[gid()][0] = upA; S.gup[gid()][1] = upL; }
     gsync();
@@ -459,7 +471,7 @@ __device__ __forceinline__ void wg_run_tasks(lds<RcShared> &S, const Img &I, con
         done = true;
         if (!lumaBits) {
             for (int t = g; t < S.nTask; t += RC_NG)
-                grp_residual(S, I.edgeFilter ? S.u.s.g[g].ea : nullptr, S.u.s.g[g].el, S.taskP[t], S.taskC[t], LW, LH, stage_src(S, S.taskP[t]), W,
+                grp_residual<W, H>(S, I.edgeFilter ? S.u.s.g[g].ea : nullptr, S.u.s.g[g].el, S.taskP[t], S.taskC[t], stage_src(S, S.taskP[t]), W,
                              S.u.s.res[t], I.bd);
             __syncthreads();
         }
@@ -488,7 +500,7 @@ __device__ __forceinline__ void wg_run_tasks(lds<RcShared> &S, const Img &I, con
             o = grp_task_big<W, H>(S, team_scr(S, g), I, T, p, c, stage_src(S, p), W, P);
         } else {
             if (!lumaBits)   // chroma: this task's own residual
-                grp_residual(S, I.edgeFilter ? S.u.s.g[g].ea : nullptr, S.u.s.g[g].el, p, c, LW, LH, stage_src(S, p), W, S.u.s.res[t], I.bd);
+                grp_residual<W, H>(S, I.edgeFilter ? S.u.s.g[g].ea : nullptr, S.u.s.g[g].el, p, c, stage_src(S, p), W, S.u.s.res[t], I.bd);
             o = grp_task<W, H>(S, S.u.s.g[g], I, p, c, tt, S.u.s.res[lumaBits ? S.taskK[t] : t], S.u.s.lev + t * (W * H), P);
         }
         float J;
@@ -527,7 +539,7 @@ __device__ void wg_luma_residuals(lds<RcShared> &S, const Img &I) {
     constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : 4, LH = H == 4 ? 2 : H == 8 ? 3 : 4;
     const int g = gid();
     if (g < S.nCand)
-        grp_residual(S, I.edgeFilter ? S.u.s.g[g].ea : nullptr, S.u.s.g[g].el, 0, S.candC[g], LW, LH, stage_src(S, 0), W, S.u.s.res[g], I.bd);
+        grp_residual<W, H>(S, I.edgeFilter ? S.u.s.g[g].ea : nullptr, S.u.s.g[g].el, 0, S.candC[g], stage_src(S, 0), W, S.u.s.res[g], I.bd);
     __syncthreads();
 }
 
