@@ -208,12 +208,12 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
                     const int c = S.cand[ci], m = cand_mode(c), pa = pangle(m, cand_delta(c));
                     int d[16];
 #pragma unroll
-                    for (int i = 0; i < 4; i++)
+                    for (int j = 0; j < 4; j++) {   // one run of pred_col per column of the 4x4
+                        int r[4];
+                        pred_col(m, pa, dcv, sy, 1, sx + j, LW, LH, A, L, 0, 0, r);
 #pragma unroll
-                        for (int j = 0; j < 4; j++) {
-                            const int p = m == DC_PRED ? dcv : pred_px(m, pa, sy + i, sx + j, LW, LH, A, L, 0, 0);
-                            d[i * 4 + j] = (int)src[(sy + i) * sp + sx + j] - p;
-                        }
+                        for (int i = 0; i < 4; i++) d[i * 4 + j] = (int)src[(sy + i) * sp + sx + j] - r[i];
+                    }
                     v = hadamard4(d);
                 }
                 v = seg_sumi<SEG>(v);

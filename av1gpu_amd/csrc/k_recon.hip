@@ -890,12 +890,12 @@ __device__ void rc_chroma(lds<RcShared> &S, const Batch &B, const SearchParams &
                 const lds<uint16_t> *A = S.ea[p] + EDGE_OFF, *L = S.el[p] + EDGE_OFF;
                 int d[16];
 #pragma unroll
-                for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) {   // one run of pred_col per column of the 4x4
+                    int r[4];
+                    pred_col(m, pa, S.dcv[p], sy, 1, sx + j, LW, LH, A, L, 0, 0, r);
 #pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const int pr = m == DC_PRED ? S.dcv[p] : pred_px(m, pa, sy + i, sx + j, LW, LH, A, L, 0, 0);
-                        d[i * 4 + j] = (int)s[(sy + i) * sp + sx + j] - pr;
-                    }
+                    for (int i = 0; i < 4; i++) d[i * 4 + j] = (int)s[(sy + i) * sp + sx + j] - r[i];
+                }
                 v = hadamard4(d);
             }
             v = seg_sumi<SEG>(v);
