@@ -76,23 +76,27 @@ __global__ void __launch_bounds__(64) k_ln_conv(const bf16_t *in, const bf16_t *
     const bf16_t *x = in + (size_t)item * HIN * HIN * CIN;
     float4_t acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
     for (int kb = 0; kb < KP; kb += 16) {
-        short4_t a;
+        short4_t a = {0, 0, 0, 0};
+        if constexpr (CIN % 4 == 0) {
+            // the lane's 4 taps are 4 consecutive channels of one input sample: one 8-byte load (NHWC, channel
+            // offset a multiple of 4, buffers 256-byte aligned)
+            const int k = kb + 4 * kq, t = k / CIN, ci = k % CIN, iy = 2 * oy - 1 + t / 3, ix = 2 * ox - 1 + t % 3;
+            if (k < K && iy >= 0 && iy < HIN && ix >= 0 && ix < HIN) a = *(const short4_t *)(x + ((size_t)iy * HIN + ix) * CIN + ci);
+        } else {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int k = kb + 4 * kq + j;
-            bf16_t v = 0;
-            if (k < K) {
-                const int t = k / CIN, ci = k % CIN, iy = 2 * oy - 1 + t / 3, ix = 2 * ox - 1 + t % 3;
-                if (iy >= 0 && iy < HIN && ix >= 0 && ix < HIN) v = x[((size_t)iy * HIN + ix) * CIN + ci];
+            for (int j = 0; j < 4; j++) {
+                const int k = kb + 4 * kq + j;
+                bf16_t v = 0;
+                if (k < K) {
+                    const int t = k / CIN, ci = k % CIN, iy = 2 * oy - 1 + t / 3, ix = 2 * ox - 1 + t % 3;
+                    if (iy >= 0 && iy < HIN && ix >= 0 && ix < HIN) v = x[((size_t)iy * HIN + ix) * CIN + ci];
+                }
+                a[j] = (short)v;
             }
-            a[j] = (short)v;
         }
-        short4_t b0, b1;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            b0[j] = (short)W[(size_t)(l & 15) * KP + kb + 4 * kq + j];
-            b1[j] = (short)W[(size_t)(16 + (l & 15)) * KP + kb + 4 * kq + j];
-        }
+        // the weight rows are zero-padded to KP (a multiple of 16): 4 consecutive taps, one 8-byte load each
+        const short4_t b0 = *(const short4_t *)(W + (size_t)(l & 15) * KP + kb + 4 * kq);
+        const short4_t b1 = *(const short4_t *)(W + (size_t)(16 + (l & 15)) * KP + kb + 4 * kq);
         acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b0, acc0, 0, 0, 0);
         acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b1, acc1, 0, 0, 0);
     }
