@@ -82,6 +82,17 @@ __global__ void __launch_bounds__(64) k_ln_conv(const bf16_t *in, const bf16_t *
             // offset a multiple of 4, buffers 256-byte aligned)
             const int k = kb + 4 * kq, t = k / CIN, ci = k % CIN, iy = 2 * oy - 1 + t / 3, ix = 2 * ox - 1 + t % 3;
             if (k < K && iy >= 0 && iy < HIN && ix >= 0 && ix < HIN) a = *(const short4_t *)(x + ((size_t)iy * HIN + ix) * CIN + ci);
+        } else if constexpr (CIN == 2) {
+            // two taps of two channels each: one 4-byte load per tap
+            typedef short short2_t __attribute__((ext_vector_type(2)));
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int k = kb + 4 * kq + 2 * h, t = k / 2, iy = 2 * oy - 1 + t / 3, ix = 2 * ox - 1 + t % 3;
+                short2_t v = {0, 0};
+                if (k < K && iy >= 0 && iy < HIN && ix >= 0 && ix < HIN) v = *(const short2_t *)(x + ((size_t)iy * HIN + ix) * 2);
+                a[2 * h] = v[0];
+                a[2 * h + 1] = v[1];
+            }
         } else {
 #pragma unroll
             for (int j = 0; j < 4; j++) {
