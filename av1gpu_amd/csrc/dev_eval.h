@@ -242,9 +242,11 @@ __device__ EvalOut grp_eval(RP res, FP tmp, QP q, const E &e, const TT &T) {
     constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : W == 32 ? 5 : 6, LH = H == 4 ? 2 : H == 8 ? 3 : H == 16 ? 4 : H == 32 ? 5 : 6;
     const int lane = glane();
     const int rk = row_kind(e.txType), ck = col_kind(e.txType), fud = flip_ud(e.txType), flr = flip_lr(e.txType);
+    // Outputs per item chosen so that a pass fills the group's 32 lanes where the block allows (at most 8 per item):
+    // each output is one fmaf chain over k ascending whichever lane computes it.
     // ---- column pass: item = (column j, OC consecutive outputs) ----
     {
-        constexpr int OC = TH < 8 ? TH : 8, NI = W * (TH / OC);
+        constexpr int OCF = W * TH / GW, OC = OCF < 1 ? 1 : OCF > 8 ? 8 : OCF, NI = W * (TH / OC);
         const auto *FcT = T.fwdT + T.fwdOff[ck][LH];
         const auto *Fc = T.fwd + T.fwdOff[ck][LH];
         for (int it = lane; it < NI; it += GW) {
@@ -269,30 +271,36 @@ __device__ EvalOut grp_eval(RP res, FP tmp, QP q, const E &e, const TT &T) {
         }
     }
     gsync();
-    // ---- row pass, in place: lane i owns row i (reads it whole, then writes its tw outputs) ----
+    // ---- row pass, in place: item = (row i, OR consecutive outputs); every lane reads its row whole before any writes ----
     {
+        constexpr int ORF = TW * TH / GW, OR = ORF < 1 ? 1 : ORF > TW ? TW : ORF, NR = TH * (TW / OR);
+        static_assert(NR <= GW, "one row item per lane");
         const auto *FrT = T.fwdT + T.fwdOff[rk][LW];
         const auto *Fr = T.fwd + T.fwdOff[rk][LW];
         const float scale = T.txScale[e.txSz];
-        if (lane < TH) {
-            float acc[TW];
-            auto *row = tmp + lane * TS;
+        const int i = lane % TH, o0 = (lane / TH) * OR;
+        auto *row = tmp + i * TS;
+        float acc[OR];
+        if (lane < NR) {
             if (rk == K_IDT) {
 #pragma unroll
-                for (int o = 0; o < TW; o++) acc[o] = Fr[o * W + o] * row[o];
+                for (int o = 0; o < OR; o++) acc[o] = Fr[(o0 + o) * W + o0 + o] * row[o0 + o];
             } else {
 #pragma unroll
-                for (int o = 0; o < TW; o++) acc[o] = 0.f;
+                for (int o = 0; o < OR; o++) acc[o] = 0.f;
 #pragma unroll 2
                 for (int k = 0; k < W; k++) {
                     const float x = row[k];
-                    const auto *f = FrT + k * W;
+                    const auto *f = FrT + k * W + o0;
 #pragma unroll
-                    for (int o = 0; o < TW; o++) acc[o] = fmaf(f[o], x, acc[o]);
+                    for (int o = 0; o < OR; o++) acc[o] = fmaf(f[o], x, acc[o]);
                 }
             }
+        }
+        gsync();
+        if (lane < NR) {
 #pragma unroll
-            for (int o = 0; o < TW; o++) row[o] = acc[o] * scale;
+            for (int o = 0; o < OR; o++) row[o0 + o] = acc[o] * scale;
         }
     }
     gsync();
