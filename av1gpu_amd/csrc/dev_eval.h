@@ -436,8 +436,9 @@ __device__ EvalOut half_eval(RP res, FP tmp, QP q, const E &e, const TT &T) {
     constexpr int LW = W == 4 ? 2 : W == 8 ? 3 : 4, LH = H == 4 ? 2 : H == 8 ? 3 : 4;
     const int lane = hlane();
     const int rk = row_kind(e.txType), ck = col_kind(e.txType), fud = flip_ud(e.txType), flr = flip_lr(e.txType);
+    // outputs per item chosen so that each pass fills the 16 lanes (as grp_eval does with 32)
     {
-        constexpr int OC = TH < 8 ? TH : 8, NI = W * (TH / OC);
+        constexpr int OCF = W * TH / 16, OC = OCF < 1 ? 1 : OCF > 8 ? 8 : OCF, NI = W * (TH / OC);
         const auto *FcT = T.fwdT + T.fwdOff[ck][LH];
         const auto *Fc = T.fwd + T.fwdOff[ck][LH];
         for (int it = lane; it < NI; it += 16) {
@@ -463,28 +464,34 @@ __device__ EvalOut half_eval(RP res, FP tmp, QP q, const E &e, const TT &T) {
     }
     gsync();
     {
+        constexpr int ORF = TW * TH / 16, OR = ORF < 1 ? 1 : ORF > TW ? TW : ORF, NR = TH * (TW / OR);
+        static_assert(NR <= 16, "one row item per lane");
         const auto *FrT = T.fwdT + T.fwdOff[rk][LW];
         const auto *Fr = T.fwd + T.fwdOff[rk][LW];
         const float scale = T.txScale[e.txSz];
-        if (lane < TH) {
-            float acc[TW];
-            auto *row = tmp + lane * TS;
+        const int i = lane % TH, o0 = (lane / TH) * OR;
+        auto *row = tmp + i * TS;
+        float acc[OR];
+        if (lane < NR) {
             if (rk == K_IDT) {
 #pragma unroll
-                for (int o = 0; o < TW; o++) acc[o] = Fr[o * W + o] * row[o];
+                for (int o = 0; o < OR; o++) acc[o] = Fr[(o0 + o) * W + o0 + o] * row[o0 + o];
             } else {
 #pragma unroll
-                for (int o = 0; o < TW; o++) acc[o] = 0.f;
+                for (int o = 0; o < OR; o++) acc[o] = 0.f;
 #pragma unroll 2
                 for (int k = 0; k < W; k++) {
                     const float x = row[k];
-                    const auto *f = FrT + k * W;
+                    const auto *f = FrT + k * W + o0;
 #pragma unroll
-                    for (int o = 0; o < TW; o++) acc[o] = fmaf(f[o], x, acc[o]);
+                    for (int o = 0; o < OR; o++) acc[o] = fmaf(f[o], x, acc[o]);
                 }
             }
+        }
+        gsync();   // every read of the rows precedes the in-place writes
+        if (lane < NR) {
 #pragma unroll
-            for (int o = 0; o < TW; o++) row[o] = acc[o] * scale;
+            for (int o = 0; o < OR; o++) row[o0 + o] = acc[o] * scale;
         }
     }
     gsync();
