@@ -177,6 +177,31 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
         // round holds at most two candidates (modes); larger blocks: item = (candidate, 4x4).  A candidate's items are
         // consecutive, so each segment of up to 32 lanes sums its Hadamard costs with shuffles before one LDS atomic.
         auto screen = [&](int c0, int c1) {
+            constexpr int NSY = NSUB / NSX;
+            if constexpr (NSUB >= 4 && NSUB <= 8 && NSY >= 2) {
+                // one candidate per round: lane l sits at (row, column) = ((l >> 2) & 3, l & 3) of every 4x4 its half
+                // (l >> 4) covers -- 4x4 columns h * NSX / 2 .. with all their 4x4 rows (NSX >= 2), or the 4x4 rows h,
+                // h + 2, .. of the one column -- predicted as vertical runs of pred_col; each 4x4's Hadamard runs
+                // across the half's 16 lanes, and the group sums the candidate's costs once
+                constexpr int CPH = NSX >= 2 ? NSX / 2 : 1, RUN = NSX >= 2 ? NSY : NSY / 2, DI = NSX >= 2 ? 4 : 8;
+                const int h = lane >> 4, r = (lane >> 2) & 3, cc = lane & 3;
+                for (int ci = c0; ci < c1; ci++) {
+                    const int c = S.cand[ci], m = cand_mode(c), pa = pangle(m, cand_delta(c));
+                    int v = 0;
+#pragma unroll
+                    for (int k = 0; k < CPH; k++) {
+                        const int sx = (NSX >= 2 ? h * CPH + k : 0) * 4 + cc, sy0 = (NSX >= 2 ? 0 : 4 * h) + r;
+                        int pr[RUN];
+                        pred_col(m, pa, dcv, sy0, DI, sx, LW, LH, A, L, 0, 0, pr);
+#pragma unroll
+                        for (int q = 0; q < RUN; q++) v += hadamard4_lanes((int)src[(sy0 + q * DI) * sp + sx] - pr[q]);
+                    }
+                    v = seg_sumi<GW>(v);
+                    if (lane == 0) S.satdI[ci] += v;
+                }
+                gsync();
+                return;
+            }
             if constexpr (NSUB <= 8) {
                 constexpr int PIX = NSUB * 16, SEG = PIX < GW ? PIX : GW;
                 const int items = (c1 - c0) * PIX;
