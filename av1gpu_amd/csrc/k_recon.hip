@@ -855,6 +855,40 @@ __device__ void rc_chroma(lds<RcShared> &S, const Batch &B, const SearchParams &
     // candidate's items are consecutive, so each segment of up to 32 lanes sums its costs with shuffles before one LDS
     // atomic.
     auto screen = [&](int c0, int c1) {
+        constexpr int NSY = NSUB / NSX;
+        if constexpr (NSUB <= 16 && NSY >= 2) {
+            // item = (candidate, plane, 4x4 column, pixel of the 4x4): each thread predicts a vertical run of pred_col
+            // through the column's NSY 4x4s at its (row, column) = ((t >> 2) & 3, t & 3); each 4x4's Hadamard runs
+            // across the 16 lanes holding it; a (candidate, plane) spans NSX x 16 consecutive threads (a whole wave
+            // for 16-wide blocks), so a round holds few modes
+            constexpr int TPC = NSX * 16, SEG = TPC < GW ? TPC : GW;
+            const int items = (c1 - c0) * 2 * TPC;
+            for (int base = 0; base < items; base += RC_NT) {
+                const int it = base + tid;
+                int d[NSY], ci = c0;
+#pragma unroll
+                for (int k = 0; k < NSY; k++) d[k] = 0;
+                if (it < items) {
+                    ci = c0 + it / (2 * TPC);
+                    const int rr = it % (2 * TPC), p = 1 + rr / TPC, q = rr % TPC;
+                    const int sx = (q >> 4) * 4 + (q & 3), sy0 = (q >> 2) & 3;
+                    const int c = S.uvc[ci], m = cand_mode(c), pa = pangle(m, cand_delta(c));
+                    const lds<uint16_t> *s = p == 1 ? srcU : srcV;
+                    const lds<uint16_t> *A = S.ea[p] + EDGE_OFF, *L = S.el[p] + EDGE_OFF;
+                    int pr[NSY];
+                    pred_col(m, pa, S.dcv[p], sy0, 4, sx, LW, LH, A, L, 0, 0, pr);
+#pragma unroll
+                    for (int k = 0; k < NSY; k++) d[k] = (int)s[(sy0 + 4 * k) * sp + sx] - pr[k];
+                }
+                int v = 0;
+#pragma unroll
+                for (int k = 0; k < NSY; k++) v += hadamard4_lanes(d[k]);
+                v = seg_sumi<SEG>(v);
+                if (it < items && (it & (SEG - 1)) == 0) atomicAdd(gen(&S.satdI[ci]), v);
+            }
+            __syncthreads();
+            return;
+        }
         if constexpr (NSUB <= 16) {
             constexpr int PIX = NSUB * 16;
             const int items = (c1 - c0) * PIX * 2;
