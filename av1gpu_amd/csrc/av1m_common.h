@@ -189,23 +189,32 @@ struct Inv {
         return a <= 64 ? av1t::Cos128_Lookup[a] : a <= 128 ? -av1t::Cos128_Lookup[128 - a]
              : a <= 192 ? -av1t::Cos128_Lookup[a - 128] : av1t::Cos128_Lookup[256 - a];
     }
-    static AV1M_HD void range(int64_t v, int bits, uint32_t &err, uint32_t f) {
-        if (v < -((int64_t)1 << (bits - 1)) || v > ((int64_t)1 << (bits - 1)) - 1) err |= f;
+    // 32-bit arithmetic (r <= 18: 8- and 10-bit).  While every value a step reads lies inside its range, the products
+    // (at most 2^17 x 2^14) and sums stay below 2^31 and the results are exact; the first value out of its range is
+    // then also exact, so it raises the range flag whose pass is discarded (the coefficients are halved and the
+    // transform runs again), and what the discarded pass computes after it does not matter.  mul / add wrap (unsigned
+    // arithmetic) so that such a pass has defined behaviour.
+    static AV1M_HD int32_t mul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
+    static AV1M_HD int32_t add(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+    static AV1M_HD int32_t sub(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
+    static AV1M_HD int32_t rnd12(int32_t v) { return add(v, 2048) >> 12; }
+    static AV1M_HD void range(int32_t v, int bits, uint32_t &err, uint32_t f) {
+        if (v < -(1 << (bits - 1)) || v > (1 << (bits - 1)) - 1) err |= f;
     }
     static AV1M_HD void B(int32_t *T, int a, int b, int ang, int flip, int r, uint32_t &err) {
-        const int64_t c = cos128(ang), s = cos128(ang - 64);
-        const int64_t x = round2l((int64_t)T[a] * c - (int64_t)T[b] * s, 12), y = round2l((int64_t)T[a] * s + (int64_t)T[b] * c, 12);
+        const int32_t c = cos128(ang), s = cos128(ang - 64);
+        const int32_t x = rnd12(sub(mul(T[a], c), mul(T[b], s))), y = rnd12(add(mul(T[a], s), mul(T[b], c)));
         range(x, r, err, 1u);
         range(y, r, err, 1u);
-        T[a] = (int32_t)(flip ? y : x);
-        T[b] = (int32_t)(flip ? x : y);
+        T[a] = flip ? y : x;
+        T[b] = flip ? x : y;
     }
     static AV1M_HD void H(int32_t *T, int a, int b, int flip, int r) {
         const int aa = flip ? b : a, bb = flip ? a : b;
-        const int64_t x = T[aa], y = T[bb], lo = -((int64_t)1 << (r - 1)), hi = ((int64_t)1 << (r - 1)) - 1;
-        const int64_t s = x + y, d = x - y;
-        T[aa] = (int32_t)(s < lo ? lo : s > hi ? hi : s);
-        T[bb] = (int32_t)(d < lo ? lo : d > hi ? hi : d);
+        const int32_t x = T[aa], y = T[bb], lo = -(1 << (r - 1)), hi = (1 << (r - 1)) - 1;
+        const int32_t s = add(x, y), d = sub(x, y);
+        T[aa] = s < lo ? lo : s > hi ? hi : s;
+        T[bb] = d < lo ? lo : d > hi ? hi : d;
     }
     static constexpr int brev(int bits, int x) {
         int t = 0;
@@ -368,21 +377,22 @@ struct Inv {
         }
     }
     static AV1M_HD void adst4(int32_t *T, int r, uint32_t &err) {
-        const int64_t S1 = 1321, S2 = 2482, S3 = 3344, S4 = 3803;
-        int64_t s0 = S1 * T[0], s1 = S2 * T[0], s2 = S3 * T[1], s3 = S4 * T[2], s4 = S1 * T[2], s5 = S2 * T[3], s6 = S4 * T[3];
-        const int64_t a7 = (int64_t)T[0] - T[2], b7 = a7 + T[3];
+        constexpr int32_t S1 = 1321, S2 = 2482, S3 = 3344, S4 = 3803;
+        int32_t s0 = mul(S1, T[0]), s1 = mul(S2, T[0]), s2 = mul(S3, T[1]), s3 = mul(S4, T[2]), s4 = mul(S1, T[2]), s5 = mul(S2, T[3]),
+                s6 = mul(S4, T[3]);
+        const int32_t a7 = sub(T[0], T[2]), b7 = add(a7, T[3]);
         range(a7, r + 1, err, 2u); range(b7, r, err, 2u);
         range(s0, r + 12, err, 2u); range(s1, r + 12, err, 2u); range(s2, r + 12, err, 2u); range(s3, r + 12, err, 2u);
         range(s4, r + 12, err, 2u); range(s5, r + 12, err, 2u); range(s6, r + 12, err, 2u);
-        s0 = s0 + s3; s1 = s1 - s4; s3 = s2; s2 = S3 * b7;
+        s0 = add(s0, s3); s1 = sub(s1, s4); s3 = s2; s2 = mul(S3, b7);
         range(s0, r + 12, err, 2u); range(s1, r + 12, err, 2u); range(s2, r + 12, err, 2u);
-        s0 = s0 + s5; s1 = s1 - s6;
+        s0 = add(s0, s5); s1 = sub(s1, s6);
         range(s0, r + 12, err, 2u); range(s1, r + 12, err, 2u);
-        int64_t x0 = s0 + s3, x1 = s1 + s3, x2 = s2, x3 = s0 + s1;
+        int32_t x0 = add(s0, s3), x1 = add(s1, s3), x2 = s2, x3 = add(s0, s1);
         range(x3, r + 12, err, 2u);
-        x3 = x3 - s3;
+        x3 = sub(x3, s3);
         range(x0, r + 12, err, 2u); range(x1, r + 12, err, 2u); range(x2, r + 12, err, 2u); range(x3, r + 12, err, 2u);
-        T[0] = (int32_t)round2l(x0, 12); T[1] = (int32_t)round2l(x1, 12); T[2] = (int32_t)round2l(x2, 12); T[3] = (int32_t)round2l(x3, 12);
+        T[0] = rnd12(x0); T[1] = rnd12(x1); T[2] = rnd12(x2); T[3] = rnd12(x3);
     }
     static AV1M_HD void adst8(int32_t *T, int r, uint32_t &err) {
         adst_in<3>(T);
@@ -427,10 +437,10 @@ struct Inv {
     template <int n> static AV1M_HD void idt(int32_t *T) {
 #pragma unroll
         for (int i = 0; i < (1 << n); i++) {
-            if (n == 2) T[i] = (int32_t)round2l((int64_t)T[i] * 5793, 12);
-            else if (n == 3) T[i] = T[i] * 2;
-            else if (n == 4) T[i] = (int32_t)round2l((int64_t)T[i] * 11586, 12);
-            else T[i] = T[i] * 4;
+            if (n == 2) T[i] = rnd12(mul(T[i], 5793));
+            else if (n == 3) T[i] = mul(T[i], 2);
+            else if (n == 4) T[i] = rnd12(mul(T[i], 11586));
+            else T[i] = mul(T[i], 4);
         }
     }
     template <int n> static AV1M_HD void run(int32_t *T, int kind, int r, uint32_t &err) {

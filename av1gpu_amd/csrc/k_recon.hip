@@ -296,7 +296,7 @@ __device__ __forceinline__ uint32_t inv_row(int i, int W, int th, int tw, const 
     for (int j = 0; j < N; j++) {
         const int v = (i < th && j < tw) ? q[i * tw + j] : 0;
         Tv[j] = v ? dequant(v, (i == 0 && j == 0) ? dcQ : acQ, dsh, bd) : 0;
-        if (rect) Tv[j] = (int32_t)round2l((int64_t)Tv[j] * 2896, 12);
+        if (rect) Tv[j] = Inv::rnd12(Inv::mul(Tv[j], 2896));   // |Tv| < 2^17: exact
     }
     uint32_t err = 0;
     Inv::run<LW>(Tv, rk, bd + 8, err);
