@@ -451,7 +451,7 @@ __device__ __forceinline__ const lds<uint16_t> *stage_src(const lds<RcShared> &S
 // J = fmaf(lambda, R + extra, D) with extra = luma mode bits (plane 0) or 0; S.taskJ / taskE.  Then S.win = the first
 // minimum over the tasks.  Blocks up to 16x16: luma tasks read the residual of their candidate (S.taskK), made by
 // wg_luma_residuals; chroma tasks make their own in slot t; levels of task t go to S.u.s.lev[t].  Larger blocks:
-// team g keeps the levels and prepared edges of its last task.
+// team g (tasks g, g + nTeam, ..., run in descending order) keeps the levels and prepared edges of task g.
 template <int W, int H>
 __device__ __forceinline__ void wg_run_tasks(lds<RcShared> &S, const Img &I, const DevTables &T, const SearchParams &P, float lambda, bool lumaBits,
                              bool pickWinner) {
@@ -493,7 +493,10 @@ __device__ __forceinline__ void wg_run_tasks(lds<RcShared> &S, const Img &I, con
         }
     }
     const int NGT = big ? S.nTeam : RC_NG;   // big tasks: the teams (groups 0 .. nTeam - 1)
-    if (!done && g < NGT) for (int t = g; t < S.nTask; t += NGT) {
+    // a team keeps the levels and prepared edges of its last task only: it runs its tasks in descending order, so the
+    // first tasks -- the analysis's best candidates, the likeliest winners -- are the ones kept
+    const int tFirst = big ? (g < S.nTask ? g + ((S.nTask - 1 - g) / NGT) * NGT : -1) : g, tStep = big ? -NGT : NGT;
+    if (!done && g < NGT) for (int t = tFirst; t >= 0 && t < S.nTask; t += tStep) {
         const int c = S.taskC[t], tt = S.taskT[t], p = S.taskP[t];
         EvalOut o;
         if constexpr (big) {
@@ -543,13 +546,13 @@ __device__ void wg_luma_residuals(lds<RcShared> &S, const Img &I) {
     __syncthreads();
 }
 
-// the levels of the winning task t, and (larger blocks) the team holding them with its prepared edges: its last task
-// is t, or it evaluates t again
+// the levels of the winning task t, and (larger blocks) the team holding them with its prepared edges: the team's
+// kept task is t (t < nTeam), or it evaluates t again
 template <int W, int H>
 __device__ __forceinline__ lds<int16_t> *wg_final_levels(lds<RcShared> &S, const Img &I, const DevTables &T, const SearchParams &P, int t, int &team) {
     team = (W > 16 || H > 16) ? t % S.nTeam : t % RC_NG;
     if constexpr (W > 16 || H > 16) {
-        if (t + S.nTeam < S.nTask) {   // overwritten by a later task of the same team
+        if (t >= S.nTeam) {   // overwritten by a task of the same team that ran after it
             const int p = S.taskP[t];
             if (gid() == team) grp_task_big<W, H>(S, team_scr(S, team), I, T, p, S.taskC[t], stage_src(S, p), W, P);
             __syncthreads();
