@@ -121,6 +121,7 @@ struct av1mi_encoder {
     int device = 0;
     DevTables *dT = nullptr;
     hipEvent_t ev[10] = {};
+    std::vector<hipEvent_t> dlEv;     // ends of the segments of the record download (host copies start behind them)
     int cus = 0, slots[2][2] = {{0, 0}, {0, 0}};   // compute units; resident closed-loop workgroups per CU [10-bit][variant]
     uint64_t clockHz = 100000000;     // wall_clock64 rate
     LnNet *ln = nullptr;              // learned pruning network (null: exhaustive search)
@@ -173,27 +174,48 @@ static void drop_plan(av1mi_encoder *e) {
     e->plan = av1mi_encoder::Plan();
 }
 
-// copy jobs (dst, src, bytes) on up to 16 host threads, in chunks (spreads the first-touch faults of fresh outputs)
-static void parallel_copy(const std::vector<std::tuple<void *, const void *, size_t>> &jobs) {
+// Copy jobs (dst, src, bytes) out of the pinned staging buffer on up to 16 host threads, in chunks (which spreads the
+// first-touch faults of fresh outputs), while the download into it is still running: the download is issued as
+// segments in staging order, segEnd[k] the end offset of segment k and segEv[k] the event after it; the chunks are
+// taken in staging order and each waits only for the segment that covers its end.
+static int parallel_copy(const std::vector<std::tuple<void *, const void *, size_t>> &jobs, const char *stage,
+                         const std::vector<size_t> &segEnd, const std::vector<hipEvent_t> &segEv) {
     constexpr size_t CH = (size_t)8 << 20;
-    std::vector<std::tuple<char *, const char *, size_t>> parts;
+    struct Part { char *d; const char *s; size_t n; size_t seg; };
+    std::vector<Part> parts;
     size_t total = 0;
     for (const auto &j : jobs) {
         char *d = (char *)std::get<0>(j);
-        const char *s = (const char *)std::get<1>(j);
-        for (size_t o = 0; o < std::get<2>(j); o += CH) parts.emplace_back(d + o, s + o, std::min(CH, std::get<2>(j) - o));
+        const char *src = (const char *)std::get<1>(j);
+        for (size_t o = 0; o < std::get<2>(j); o += CH) parts.push_back(Part{d + o, src + o, std::min(CH, std::get<2>(j) - o), 0});
         total += std::get<2>(j);
     }
+    for (auto &pt : parts) {
+        const size_t end = (size_t)(pt.s - stage) + pt.n;
+        pt.seg = (size_t)(std::lower_bound(segEnd.begin(), segEnd.end(), end) - segEnd.begin());
+        if (pt.seg >= segEnd.size()) return -1;
+    }
+    std::sort(parts.begin(), parts.end(), [](const Part &a, const Part &b) { return a.s < b.s; });
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
     const unsigned nt = (unsigned)std::min<size_t>(std::min(16u, hw), std::max<size_t>(1, total / CH));
     std::atomic<size_t> next{0};
+    std::atomic<int> bad{0};
     auto work = [&]() {
-        for (size_t k; (k = next.fetch_add(1)) < parts.size();) memcpy(std::get<0>(parts[k]), std::get<1>(parts[k]), std::get<2>(parts[k]));
+        size_t waited = 0;   // segments [0, waited) are known complete on this thread
+        for (size_t k; (k = next.fetch_add(1)) < parts.size();) {
+            const Part &pt = parts[k];
+            while (waited <= pt.seg) {
+                if (hipEventSynchronize(segEv[waited]) != hipSuccess) { bad = 1; return; }
+                waited++;
+            }
+            memcpy(pt.d, pt.s, pt.n);
+        }
     };
     std::vector<std::thread> th;
     for (unsigned t = 1; t < nt; t++) th.emplace_back(work);
     work();
     for (auto &t : th) t.join();
+    return bad ? -1 : 0;
 }
 
 extern "C" {
@@ -426,6 +448,7 @@ void av1mi_destroy(av1mi_encoder *e) {
     for (void *p : e->oldStages) (void)hipHostFree(p);
     for (auto &s : e->aux) if (s) (void)hipStreamDestroy(s);
     if (e->cap) (void)hipStreamDestroy(e->cap);
+    for (auto &v : e->dlEv) (void)hipEventDestroy(v);
     if (e->evFork) (void)hipEventDestroy(e->evFork);
     for (auto &v : e->evJoin) if (v) (void)hipEventDestroy(v);
     if (e->dT) (void)hipFree(e->dT);
@@ -890,7 +913,24 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
             enc->stageBytes = cap;
         }
         char *stg = (char *)enc->stage;
-        if (!dev) HCK(hipMemcpyAsync(stg, R.base, R.total, hipMemcpyDeviceToHost, st));
+        std::vector<size_t> segEnd;
+        std::vector<hipEvent_t> segEv;
+        auto segment = [&](size_t end) -> int {   // the download so far ends at staging offset end
+            if (segEv.size() == enc->dlEv.size()) {
+                hipEvent_t e;
+                if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return -1;
+                enc->dlEv.push_back(e);
+            }
+            segEv.push_back(enc->dlEv[segEv.size()]);
+            segEnd.push_back(end);
+            return hipEventRecord(segEv.back(), st) == hipSuccess ? 0 : -1;
+        };
+        constexpr size_t SEG = (size_t)64 << 20;
+        if (!dev)
+            for (size_t o = 0; o < R.total; o += SEG) {
+                HCK(hipMemcpyAsync(stg + o, (const char *)R.base + o, std::min(SEG, R.total - o), hipMemcpyDeviceToHost, st));
+                if (segment(std::min(o + SEG, R.total))) { snprintf(err, errlen, "HIP: download event"); return -1; }
+            }
         auto hostOf = [&](const void *dptr) { return stg + ((const char *)dptr - (const char *)R.base); };
         std::vector<std::tuple<void *, const void *, size_t>> jobs;
         size_t reconOff = dev ? 0 : R.total;
@@ -922,12 +962,14 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
                     HCK(hipMemcpyAsync(stg + reconOff, I.cdefOn ? I.cdefOut[p] : I.rec[p], b, hipMemcpyDeviceToHost, st));
                     jobs.emplace_back(o.recon[p], stg + reconOff, b);
                     reconOff += b;
+                    if (segment(reconOff)) { snprintf(err, errlen, "HIP: download event"); return -1; }
                 }
             }
         }
         (void)hipEventRecord(enc->ev[7], st);
+        const int cr = parallel_copy(jobs, stg, segEnd, segEv);
         HCK(hipStreamSynchronize(st));
-        parallel_copy(jobs);
+        if (cr) { snprintf(err, errlen, "internal: record copy"); return -1; }
         if (dev) {   // the packed arena passes to the caller as it is: frames back to back, frame-relative offsets
             memset(dev, 0, sizeof *dev);
             dev->n_frames = (uint32_t)n;
