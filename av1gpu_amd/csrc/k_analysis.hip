@@ -222,6 +222,62 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
                 gsync();
                 return;
             }
+            if constexpr (NSUB >= GW) {
+                // every candidate gives lane l the same 4x4s (l, l + 32, ...): each is loaded once and screened for all
+                // the candidates in turn, the group summing each candidate's costs (integers: any order)
+                for (int sb = lane; sb < NSUB; sb += GW) {
+                    const int sx = (sb % NSX) * 4, sy = (sb / NSX) * 4;
+                    int sv[16];
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+#pragma unroll
+                        for (int j = 0; j < 4; j++) sv[i * 4 + j] = src[(sy + i) * sp + sx + j];
+                    for (int ci = c0; ci < c1; ci++) {
+                        const int c = S.cand[ci], m = cand_mode(c), pa = pangle(m, cand_delta(c));
+                        int d[16];
+#pragma unroll
+                        for (int j = 0; j < 4; j++) {   // one run of pred_col per column of the 4x4
+                            int r[4];
+                            pred_col(m, pa, dcv, sy, 1, sx + j, LW, LH, A, L, 0, 0, r);
+#pragma unroll
+                            for (int i = 0; i < 4; i++) d[i * 4 + j] = sv[i * 4 + j] - r[i];
+                        }
+                        const int v = seg_sumi<GW>(hadamard4(d));
+                        if (lane == 0) S.satdI[ci] += v;
+                    }
+                }
+                gsync();
+                return;
+            }
+            if constexpr (NSUB == 16) {
+                // the same with 16 4x4s: lane l keeps 4x4 l % 16, two candidates a round (one per half)
+                const int sb = lane & 15, sx = (sb % NSX) * 4, sy = (sb / NSX) * 4;
+                int sv[16];
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) sv[i * 4 + j] = src[(sy + i) * sp + sx + j];
+                for (int cq = c0; cq < c1; cq += 2) {
+                    const int ci = cq + (lane >> 4);
+                    int d[16];
+#pragma unroll
+                    for (int q = 0; q < 16; q++) d[q] = 0;
+                    if (ci < c1) {
+                        const int c = S.cand[ci], m = cand_mode(c), pa = pangle(m, cand_delta(c));
+#pragma unroll
+                        for (int j = 0; j < 4; j++) {
+                            int r[4];
+                            pred_col(m, pa, dcv, sy, 1, sx + j, LW, LH, A, L, 0, 0, r);
+#pragma unroll
+                            for (int i = 0; i < 4; i++) d[i * 4 + j] = sv[i * 4 + j] - r[i];
+                        }
+                    }
+                    const int v = seg_sumi<16>(hadamard4(d));
+                    if ((lane & 15) == 0 && ci < c1) S.satdI[ci] += v;
+                }
+                gsync();
+                return;
+            }
             constexpr int SEG = NSUB < GW ? NSUB : GW;
             const int items = (c1 - c0) * NSUB;
             for (int base = 0; base < items; base += GW) {
