@@ -184,17 +184,22 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
                 // h + 2, .. of the one column -- predicted as vertical runs of pred_col; each 4x4's Hadamard runs
                 // across the half's 16 lanes, and the group sums the candidate's costs once
                 constexpr int CPH = NSX >= 2 ? NSX / 2 : 1, RUN = NSX >= 2 ? NSY : NSY / 2, DI = NSX >= 2 ? 4 : 8;
-                const int h = lane >> 4, r = (lane >> 2) & 3, cc = lane & 3;
+                const int h = lane >> 4, r = (lane >> 2) & 3, cc = lane & 3, sy0 = (NSX >= 2 ? 0 : 4 * h) + r;
+                int sv[CPH][RUN];   // the lane's source samples, the same for every candidate
+#pragma unroll
+                for (int k = 0; k < CPH; k++)
+#pragma unroll
+                    for (int q = 0; q < RUN; q++) sv[k][q] = src[(sy0 + q * DI) * sp + (NSX >= 2 ? h * CPH + k : 0) * 4 + cc];
                 for (int ci = c0; ci < c1; ci++) {
                     const int c = S.cand[ci], m = cand_mode(c), pa = pangle(m, cand_delta(c));
                     int v = 0;
 #pragma unroll
                     for (int k = 0; k < CPH; k++) {
-                        const int sx = (NSX >= 2 ? h * CPH + k : 0) * 4 + cc, sy0 = (NSX >= 2 ? 0 : 4 * h) + r;
+                        const int sx = (NSX >= 2 ? h * CPH + k : 0) * 4 + cc;
                         int pr[RUN];
                         pred_col(m, pa, dcv, sy0, DI, sx, LW, LH, A, L, 0, 0, pr);
 #pragma unroll
-                        for (int q = 0; q < RUN; q++) v += hadamard4_lanes((int)src[(sy0 + q * DI) * sp + sx] - pr[q]);
+                        for (int q = 0; q < RUN; q++) v += hadamard4_lanes(sv[k][q] - pr[q]);
                     }
                     v = seg_sumi<GW>(v);
                     if (lane == 0) S.satdI[ci] += v;
@@ -203,7 +208,15 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
                 return;
             }
             if constexpr (NSUB <= 8) {
-                constexpr int PIX = NSUB * 16, SEG = PIX < GW ? PIX : GW;
+                constexpr int PIX = NSUB * 16, SEG = PIX < GW ? PIX : GW, NPL = PIX > GW ? PIX / GW : 1;
+                // the pixel of item base + lane is (lane % SEG) + 32 k, k = (base / 32) % NPL, for every candidate: the
+                // lane's NPL source samples are loaded once
+                int sv[NPL];
+#pragma unroll
+                for (int k = 0; k < NPL; k++) {
+                    const int px = lane % SEG + GW * k, sb = px >> 4;
+                    sv[k] = src[((sb / NSX) * 4 + ((px >> 2) & 3)) * sp + (sb % NSX) * 4 + (px & 3)];
+                }
                 const int items = (c1 - c0) * PIX;
                 for (int base = 0; base < items; base += GW) {
                     const int it = base + lane;
@@ -214,7 +227,10 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
                         const int sx = (sb % NSX) * 4 + (px & 3), sy = (sb / NSX) * 4 + ((px >> 2) & 3);
                         const int c = S.cand[ci], m = cand_mode(c), pa = pangle(m, cand_delta(c));
                         const int p = m == DC_PRED ? dcv : pred_px(m, pa, sy, sx, LW, LH, A, L, 0, 0);
-                        v = (int)src[sy * sp + sx] - p;
+                        int sval = sv[0];
+#pragma unroll
+                        for (int k = 1; k < NPL; k++) if ((base / GW) % NPL == k) sval = sv[k];
+                        v = sval - p;
                     }
                     v = seg_sumi<SEG>(hadamard4_lanes(v));
                     if (it < items && (it & (SEG - 1)) == 0) atomicAdd(&S.satdI[ci], v);
