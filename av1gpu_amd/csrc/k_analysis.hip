@@ -1,13 +1,15 @@
 // Open-loop luma analysis and per-superblock partition DP.
 //
-// k_analysis<W, H>: one 32-lane group per (image, block position) of one shape, NG groups per workgroup.  Per block:
+// k_analysis<W, H>: 32-lane groups, NG per workgroup, each taking the (image, block position) items of one shape in
+// turn (grid stride, the grid capped at a multiple of what the device holds).  Per block:
 // edges from the SOURCE with Z-order availability, SATD screening of the 13 modes (integer totals through LDS atomics:
 // order free), angle deltas +-1..3 around the two best directional modes, top-K by SATD + lambda_s * mode bits, exact
 // RD (forward transform, quantizer, coefficient rate) of the top-K with DCT_DCT, then the other tx types of the set for
 // the winner.  Output: BlockRes {J, candidates best first, tx type}.  Reference: encoder/src/k_analysis.cuh:496-689.
 //
-// k_dp: one thread per 64x64 superblock: bottom-up DP over the 85 square nodes and 10 partition types with the
-// averaged partition bits, then the leaves in decode order.  Reference: encoder/src/k_dp.cuh:72-297.
+// k_dp: one wave per 64x64 superblock: bottom-up DP over the 85 square nodes and 10 partition types with the averaged
+// partition bits, a level's nodes across the lanes, then the leaves in decode order by prefix sums.  Reference:
+// encoder/src/k_dp.cuh:72-297.
 #include <type_traits>
 #include "dev_eval.h"
 
@@ -438,15 +440,36 @@ __global__ void k_an_blank(BlockRes *blk, uint32_t n) {
     }
 }
 
+// workgroups of a shape's launch: the kernels loop over the blocks (grid stride), so the grid is capped at
+// AN_WAVES_PER_SLOT times what the device holds at once -- most blocks of the small shapes are pruned by the learned
+// network and take a few instructions each, which one workgroup per 8 blocks spent on workgroup dispatch instead.
+// Measured on MI355X (analysis of the 256-image batch): 908 ms uncapped, 836 / 811 / 771 / 771 / 780 / 798 ms at 2 /
+// 4 / 16 / 32 / 64 / 128 (few workgroups balance the unevenly pruned blocks worse)
+constexpr int AN_WAVES_PER_SLOT = 16;
+template <typename K>
+static uint32_t an_grid_cap(K kernel, int threads) {
+    int dev = 0, cus = 0, occ = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, threads, 0) != hipSuccess || cus <= 0 || occ <= 0)
+        return 1u << 30;
+    return (uint32_t)cus * (uint32_t)occ * AN_WAVES_PER_SLOT;
+}
+
 template <int W, int H, int BS>
 static void launch_an(const Batch &B, const SearchParams &P, uint32_t n, int bd, const AnHooks &h, hipStream_t st) {
     if (!n) return;
     constexpr int NG = an_groups(W, H);
     const AnHook hk = h.hook ? h.hook[BS] : AnHook{};
     if (hk.list) hipLaunchKernelGGL(k_an_blank, dim3((n + 255) / 256), dim3(256), 0, st, B.blk + h.base[BS], n);
-    uint32_t grid = (n + NG - 1) / NG;
-    if (grid > (1u << 30)) grid = 1u << 30;
+    // the cap of this instance on the current device (host-side queries, per launch: encoders on several host threads
+    // share nothing here)
     const bool hooked = hk.ext || hk.list;
+    const uint32_t c = bd == 10 ? (hooked ? an_grid_cap(k_analysis<W, H, BS, uint16_t, true>, NG * 32)
+                                          : an_grid_cap(k_analysis<W, H, BS, uint16_t, false>, NG * 32))
+                                : (hooked ? an_grid_cap(k_analysis<W, H, BS, uint8_t, true>, NG * 32)
+                                          : an_grid_cap(k_analysis<W, H, BS, uint8_t, false>, NG * 32));
+    uint32_t grid = (n + NG - 1) / NG;
+    if (grid > c) grid = c;
     if (bd == 10) {
         if (hooked) hipLaunchKernelGGL((k_analysis<W, H, BS, uint16_t, true>), dim3(grid), dim3(NG * 32), 0, st, B, P, n, hk);
         else hipLaunchKernelGGL((k_analysis<W, H, BS, uint16_t, false>), dim3(grid), dim3(NG * 32), 0, st, B, P, n, hk);
