@@ -16,14 +16,20 @@
 namespace av1m {
 
 // Shapes at least 32 wide try DCT_DCT only: their evaluation takes the forward DCT with the residual made on the fly
-// from the prediction (grp_fwd_dct: wave-uniform matrix operands), so no residual array sits in LDS.
+// from the prediction (grp_fwd_dct: wave-uniform matrix operands), so no residual array sits in LDS.  The levels
+// overlay the coefficients (lev): the quantizer's round t writes levels 32 t .. 32 t + 31, i.e. the bytes of
+// coefficients 16 t .. 16 t + 15 of the row-major tmp, whose raster index is at most 16 t + 15 -- read in an earlier
+// round, or in this one before its store (each lane's store depends on its own load); nothing reads the coefficients
+// after the quantizer.  64 x 64 and 64 x 32: 19.7 KB per two-group workgroup, 8 per CU (2 waves per SIMD, the VGPR
+// limit) instead of 6.
 template <int W, int H>
 constexpr bool an_direct() { return W >= 32; }
 template <int W, int H>
 struct EvalScrDirect {
     static constexpr int TS = W + 1, TH = H > 32 ? 32 : H;
     float tmp[TH * TS];
-    int16_t q[(W > 32 ? 32 : W) * TH];
+    static_assert((W > 32 ? 32 : W) * TH * 2 <= TH * TS * 4, "the levels fit in the coefficients' space");
+    __device__ int16_t *lev() { return reinterpret_cast<int16_t *>(tmp); }
 };
 template <int W, int H>
 struct AnScr {
@@ -64,9 +70,11 @@ __device__ __forceinline__ int ol_decoded(int px, int py, int x, int y, const Ti
 // hide the LDS and table latency, which outweighs the spills that costs the small shapes (measured on MI355X, kernel
 // trace of the 256-image batch at targets 2 .. 6: 4 is the fastest for most shapes and 3 or 5 for the few below,
 // analysis 1119 -> 1096 ms).  The shapes of 32x32 and up are limited by LDS instead (the compiler notes that it cannot
-// reach the target there; the build silences it).
+// reach the target there; the build silences it).  The 64-wide shapes take 2: 256 VGPRs and no AGPRs (the two share
+// one 512-entry file per lane, so any AGPR beyond 256 VGPRs leaves one wave per SIMD); with the levels over the
+// coefficients that gave 64x64 94 -> 49 ms and 64x32 71 -> 56 ms on the 256-image batch.
 constexpr int an_wpe(int W, int H) {
-    return (W == 16 && H == 32) || (W == 32 && H == 64) ? 3 : (W == 8 && H == 16) || (W == 16 && H == 8) ? 5 : 4;
+    return W == 64 ? 2 : (W == 16 && H == 32) || (W == 32 && H == 64) ? 3 : (W == 8 && H == 16) || (W == 16 && H == 8) ? 5 : 4;
 }
 // HOOKS: the instance that reads an external screener's candidates and / or a block list (AnHook); the default instance
 // has neither, so their registers do not weigh on it.
@@ -375,7 +383,7 @@ __global__ void __launch_bounds__(an_groups(W, H) * 32) __attribute__((amdgpu_wa
                 e.resEnergy = grp_fwd_dct<W, H>(tl, T, T.txScale[TXS], res);
                 e.txType = DCT_DCT;
                 e.intraDir = m;
-                const EvalOut o = grp_qrate<W, H>(S.ev.tmp, W + 1, S.ev.q, e, T);
+                const EvalOut o = grp_qrate<W, H>(S.ev.tmp, W + 1, S.ev.lev(), e, T);
                 const float J = fmaf(lambda, (o.R + mbits(c)) + overhead, o.D);
                 if (J < bestJ) { bestJ = J; bestK = k; }
                 gsync();
