@@ -574,7 +574,7 @@ AV1M_HD void pred_col(int mode, int pa, int dcv, int i0, int di, int j, int lw, 
         for (int c = 0; c < CH; c++) {
             const int i = i0 + c * di, wi = wy[i], l = L[i];
             const int vv = wi * a + (256 - wi) * lb, hh = wj * l + (256 - wj) * ar;
-            o[c] = (mode == SMOOTH_PRED ? vv + hh : mode == SMOOTH_V_PRED ? 2 * vv : 2 * hh) + 256 >> 9;
+            o[c] = ((mode == SMOOTH_PRED ? vv + hh : mode == SMOOTH_V_PRED ? 2 * vv : 2 * hh) + 256) >> 9;
         }
     }
 }

@@ -67,14 +67,19 @@ __device__ __forceinline__ int ol_decoded(int px, int py, int x, int y, const Ti
 }
 
 // Occupancy target per shape, in waves per SIMD.  At 4 the compiler keeps a kernel under 128 VGPRs and the extra waves
-// hide the LDS and table latency, which outweighs the spills that costs the small shapes (measured on MI355X, kernel
-// trace of the 256-image batch at targets 2 .. 6: 4 is the fastest for most shapes and 3 or 5 for the few below,
-// analysis 1119 -> 1096 ms).  The shapes of 32x32 and up are limited by LDS instead (the compiler notes that it cannot
-// reach the target there; the build silences it).  The 64-wide shapes take 2: 256 VGPRs and no AGPRs (the two share
-// one 512-entry file per lane, so any AGPR beyond 256 VGPRs leaves one wave per SIMD); with the levels over the
-// coefficients that gave 64x64 94 -> 49 ms and 64x32 71 -> 56 ms on the 256-image batch.
+// hide the LDS and table latency, which outweighs the spills that costs most small shapes; the shapes listed at 3
+// spill 150-340 bytes per lane at 4 or 5 and run faster with 168 VGPRs (measured on MI355X, kernel trace of the
+// 256-image batch with every shape at each of the targets 2 .. 6, the fastest kept per shape: analysis 642 -> 626 ms;
+// 8x16 63 -> 56 ms).  The 64-wide shapes take 2: 256 VGPRs and no AGPRs (the two share one 512-entry file per lane,
+// so any AGPR beyond 256 VGPRs leaves one wave per SIMD); with the levels over the coefficients that gave 64x64
+// 94 -> 49 ms and 64x32 71 -> 56 ms on the same batch.  Where LDS allows fewer waves than the target the compiler
+// notes that it cannot reach it; the build silences that.
 constexpr int an_wpe(int W, int H) {
-    return W == 64 ? 2 : (W == 16 && H == 32) || (W == 32 && H == 64) ? 3 : (W == 8 && H == 16) || (W == 16 && H == 8) ? 5 : 4;
+    return W == 64 ? 2
+         : (W == 8 && H == 16) || (W == 16 && H == 8) || (W == 16 && H == 16) || (W == 16 && H == 32) || (W == 32 && H == 32) ||
+                   (W == 32 && H == 64)
+             ? 3
+             : 4;
 }
 // HOOKS: the instance that reads an external screener's candidates and / or a block list (AnHook); the default instance
 // has neither, so their registers do not weigh on it.
