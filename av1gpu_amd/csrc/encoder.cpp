@@ -12,6 +12,7 @@
 #include <sys/mman.h>
 #include <algorithm>
 #include <map>
+#include <memory>
 #include <atomic>
 #include <thread>
 #include <tuple>
@@ -39,6 +40,7 @@ size_t ln_scratch_bytes(uint32_t n);
 int recon_prepare(int bd);
 size_t recon_smem_bytes();
 void launch_pack(const Batch &B, const PackOff *dOff, RecBlock *oB, uint32_t *oBT, RecTxb *oT, int16_t *oC, hipStream_t st);
+void launch_sparse(const int16_t *c, uint64_t n, uint64_t *mask, uint32_t *cnt, uint64_t *off, int16_t *val, hipStream_t st);
 void launch_to10(const uint8_t *src, int sp, uint16_t *dst, int dp, int w, int h, int chroma, hipStream_t st);
 hipError_t noalloc_scratch(void **ptr, size_t *bytes);
 void launch_deblock(const Batch &B, int pass, const uint64_t *dStart, uint64_t units, int bd, hipStream_t st);
@@ -132,6 +134,8 @@ struct av1mi_encoder {
     size_t stageBytes = 0;
     std::vector<void *> oldStages;    // outgrown staging buffers, freed at destroy: hipHostFree would wait for the whole
                                       // device, i.e. for the other encoders' work in a multi-stream server
+    uint64_t *spTot = nullptr;        // pinned: the nonzero count of a call's sparse coefficient download
+    hipEvent_t spEv = nullptr;        // after that count's copy
     bool countWork = false;           // count the forward-transform FLOPs of the evaluations (av1mi_times.flops*)
     // analysis hooks (device pointers of the caller): external candidates per shape with their per-(shape, image) offsets,
     // block lists per shape
@@ -176,9 +180,10 @@ static void drop_plan(av1mi_encoder *e) {
 }
 
 // a fresh output buffer of at least 4 MiB asks for transparent huge pages (where the kernel offers them on request), so
-// its first touch by the copy faults once per 2 MiB instead of once per 4 KiB
-static void *out_alloc(size_t bytes) {
-    void *p = malloc(bytes);
+// its first touch by the copy faults once per 2 MiB instead of once per 4 KiB; zero: through calloc, whose large blocks
+// are fresh zero pages (nothing written before the first touch)
+static void *out_alloc(size_t bytes, bool zero = false) {
+    void *p = zero ? calloc(bytes, 1) : malloc(bytes);
     constexpr size_t HP = (size_t)2 << 20;
     if (p && bytes >= 2 * HP) {
         const uintptr_t a = ((uintptr_t)p + HP - 1) & ~(uintptr_t)(HP - 1), e = ((uintptr_t)p + bytes) & ~(uintptr_t)(HP - 1);
@@ -224,6 +229,97 @@ static int parallel_copy(const std::vector<std::tuple<void *, const void *, size
             memcpy(pt.d, pt.s, pt.n);
         }
     };
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; t++) th.emplace_back(work);
+    work();
+    for (auto &t : th) t.join();
+    return bad ? -1 : 0;
+}
+
+// The frames' host coefficient arrays, made while the device works: allocated zeroed at an upper bound of the count
+// (every superblock's area in each plane: a tx block stores at most its area and lies in its superblock) and faulted
+// in on up to 8 host threads, so that after the download only the nonzero values remain to be written.  take() hands a
+// frame's array over, shrunk to its count; what is not taken is freed with the object.
+struct CoefPrefault {
+    std::vector<int16_t *> buf;
+    std::vector<uint64_t> cap;
+    std::vector<std::thread> th;
+    void start(const std::vector<Img> &imgs) {
+        const size_t n = imgs.size();
+        buf.assign(n, nullptr);
+        cap.resize(n);
+        std::vector<size_t> order(n);
+        for (size_t i = 0; i < n; i++) {
+            const Img &I = imgs[i];
+            const uint64_t a = (uint64_t)I.sbCols * 64 * I.sbRows * 64;
+            cap[i] = a + (I.mono ? 0 : 2 * (a >> (2 * I.ss)));
+            order[i] = i;
+        }
+        std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cap[a] > cap[b]; });
+        auto next = std::make_shared<std::atomic<size_t>>(0);
+        const unsigned nt = (unsigned)std::min<size_t>(std::min(8u, std::max(1u, std::thread::hardware_concurrency())), n);
+        for (unsigned t = 0; t < nt; t++)
+            th.emplace_back([this, order, next]() {
+                for (size_t k; (k = next->fetch_add(1)) < order.size();) {
+                    const size_t i = order[k], bytes = cap[i] * 2;
+                    volatile char *p = (volatile char *)out_alloc(bytes, true);
+                    if (!p) continue;
+                    for (size_t o = 0; o < bytes; o += 4096) p[o] = 0;
+                    buf[i] = (int16_t *)p;
+                }
+            });
+    }
+    void join() {
+        for (auto &t : th) t.join();
+        th.clear();
+    }
+    int16_t *take(size_t i, uint64_t count) {
+        join();
+        if (i >= buf.size() || !buf[i] || count > cap[i]) return nullptr;
+        int16_t *p = (int16_t *)realloc(buf[i], std::max<uint64_t>(count, 1) * 2);
+        if (p) buf[i] = nullptr;
+        return p;
+    }
+    ~CoefPrefault() {
+        join();
+        for (int16_t *p : buf) free(p);
+    }
+};
+
+// The frames' coefficients from the sparse download (k_sparse_pack's layout: mask word 8 s + e has bit l set for a
+// nonzero coefficient 512 s + 8 l + e of the batch's packed array; the values follow in the order (s, e, l), the run
+// of 32 spans r from off[r]).  Frame i owns the packed coefficients [c0[i], c0[i + 1]) and its array dst[i] is zeroed,
+// so only the nonzero values are written.  Groups of 8 runs go to up to 16 host threads; -1 if a group's values do not
+// end at the next group's offset.
+static int sparse_expand(const uint64_t *mask, const uint64_t *off, const int16_t *val, uint64_t n, const std::vector<uint64_t> &c0,
+                         const std::vector<int16_t *> &dst) {
+    constexpr uint64_t SPAN = 512, RUN = 32, GROUP = 8;
+    const uint64_t nSpan = (n + SPAN - 1) / SPAN, nRun = (nSpan + RUN - 1) / RUN, nJob = (nRun + GROUP - 1) / GROUP;
+    if (!nJob) return 0;
+    std::atomic<uint64_t> next{0};
+    std::atomic<int> bad{0};
+    const auto frameOf = [&](uint64_t g) { return (size_t)(std::upper_bound(c0.begin(), c0.end(), g) - c0.begin()) - 1; };
+    auto work = [&]() {
+        for (uint64_t j; (j = next.fetch_add(1)) < nJob;) {
+            const uint64_t r0 = j * GROUP, r1 = std::min(r0 + GROUP, nRun), s1 = std::min(r1 * RUN, nSpan);
+            uint64_t v = off[r0];
+            size_t f = frameOf(r0 * RUN * SPAN);
+            for (uint64_t s = r0 * RUN; s < s1; s++) {
+                const uint64_t lo = s * SPAN;
+                while (c0[f + 1] <= lo) f++;
+                const bool whole = lo + SPAN <= c0[f + 1];   // the span lies in frame f
+                for (int e = 0; e < 8; e++)
+                    for (uint64_t m = mask[s * 8 + e]; m; m &= m - 1) {
+                        const uint64_t g = lo + 8 * (uint64_t)__builtin_ctzll(m) + (uint64_t)e;
+                        const size_t h = whole ? f : frameOf(g);
+                        dst[h][g - c0[h]] = val[v++];
+                    }
+            }
+            if (v != off[r1]) bad = 1;
+        }
+    };
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nt = (unsigned)std::min<uint64_t>(std::min(16u, hw), nJob);
     std::vector<std::thread> th;
     for (unsigned t = 1; t < nt; t++) th.emplace_back(work);
     work();
@@ -459,6 +555,8 @@ void av1mi_destroy(av1mi_encoder *e) {
     ln_delete(e->ln);
     if (e->stage) (void)hipHostFree(e->stage);
     for (void *p : e->oldStages) (void)hipHostFree(p);
+    if (e->spTot) (void)hipHostFree(e->spTot);
+    if (e->spEv) (void)hipEventDestroy(e->spEv);
     for (auto &s : e->aux) if (s) (void)hipStreamDestroy(s);
     if (e->cap) (void)hipStreamDestroy(e->cap);
     for (auto &v : e->dlEv) (void)hipEventDestroy(v);
@@ -567,6 +665,8 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
         nSb += (uint32_t)(I.sbCols * I.sbRows);
         for (int k = 0; k < I.sbCols * I.sbRows; k++) sbImg.push_back((uint32_t)i);
     }
+    CoefPrefault pf;   // host outputs: their coefficient arrays are made while the device works
+    if (!dev && !na) pf.start(imgs);
     uint64_t blkTotal = 0;
     for (int s = 0; s < NSHAPES; s++) {
         const uint64_t shapeBase = blkTotal;
@@ -871,6 +971,12 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
     imgBlk[n] = gBlk; imgTxb[n] = gTxb; imgCoef[n] = gCoef;
     Arena R;
     RecBlock *rB; uint32_t *rBT; RecTxb *rT; int16_t *rC;
+    // host outputs: the coefficients go down in sparse form (launch_sparse: spSpan spans of 512, spRun runs of 32 spans)
+    const bool sparse = !na && !dev && gCoef > 0;
+    const uint64_t spSpan = (gCoef + 511) / 512, spRun = (spSpan + 31) / 32;
+    uint64_t *sMask = nullptr, *sOff = nullptr;
+    uint32_t *sCnt = nullptr;
+    int16_t *sVal = nullptr;
     if (na) {   // the caller's output buffers
         const av1mi_noalloc_out &o = *na->o;
         if (gBlk > o.cap_blocks || gTxb > o.cap_txb || gCoef > o.cap_coeffs) {
@@ -882,6 +988,7 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
         rB = (RecBlock *)o.blocks; rBT = o.block_txb; rT = (RecTxb *)o.txb; rC = o.coeffs;
     } else {
         R.add(rB, gBlk); R.add(rBT, gBlk); R.add(rT, gTxb); R.add(rC, gCoef);
+        if (sparse) { R.add(sMask, spSpan * 8); R.add(sCnt, spRun); R.add(sOff, spRun + 1); R.add(sVal, gCoef); }
         HCK(R.alloc(st));
     }
     HCK(hipMemcpyAsync(dPO, po.data(), (size_t)nSb * sizeof(PackOff), hipMemcpyHostToDevice, st));
@@ -889,6 +996,14 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
     launch_pack(B, dPO, rB, rBT, rT, rC, st);
     HCK(hipGetLastError());
     (void)hipEventRecord(enc->ev[6], st);
+    if (sparse) {   // masks and values of the nonzero coefficients, and their count to the host
+        if (!enc->spTot) HCK(hipHostMalloc((void **)&enc->spTot, 64, hipHostMallocDefault));
+        if (!enc->spEv) HCK(hipEventCreateWithFlags(&enc->spEv, hipEventDisableTiming));
+        launch_sparse(rC, gCoef, sMask, sCnt, sOff, sVal, st);
+        HCK(hipGetLastError());
+        HCK(hipMemcpyAsync(enc->spTot, sOff + spRun, 8, hipMemcpyDeviceToHost, st));
+        HCK(hipEventRecord(enc->spEv, st));
+    }
     if (na) {   // the records are in the caller's buffers; the reconstruction goes to its planes, if it gave any
         for (int i = 0; i < n; i++) {
             const Img &I = imgs[i];
@@ -915,7 +1030,13 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
         size_t reconBytes = 0;
         if (wantRecon)
             for (int i = 0; i < n; i++) reconBytes += ((size_t)imgs[i].aw * imgs[i].ah + 2 * (size_t)(imgs[i].aw >> imgs[i].ss) * (imgs[i].ah >> imgs[i].ss)) * bps;
-        const size_t need = (dev ? 0 : R.total) + reconBytes;
+        // staging: the records before the coefficients at their arena offsets; then, for the sparse form, the masks,
+        // the runs' offsets and room for every coefficient's value; then the reconstructions
+        const size_t prefix = sparse ? (size_t)((const char *)rC - (const char *)R.base) : R.total;
+        const size_t maskBytes = spSpan * 8 * sizeof(uint64_t);
+        const size_t stMask = prefix, stOff = stMask + maskBytes, stVal = (stOff + (spRun + 1) * 8 + 255) & ~(size_t)255;
+        const size_t recEnd = dev ? 0 : sparse ? (stVal + gCoef * 2 + 255) & ~(size_t)255 : R.total;
+        const size_t need = recEnd + reconBytes;
         if (need > enc->stageBytes) {
             HCK(hipStreamSynchronize(st));
             if (enc->stage) enc->oldStages.push_back(enc->stage);
@@ -939,14 +1060,30 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
             return hipEventRecord(segEv.back(), st) == hipSuccess ? 0 : -1;
         };
         constexpr size_t SEG = (size_t)64 << 20;
-        if (!dev)
-            for (size_t o = 0; o < R.total; o += SEG) {
-                HCK(hipMemcpyAsync(stg + o, (const char *)R.base + o, std::min(SEG, R.total - o), hipMemcpyDeviceToHost, st));
-                if (segment(std::min(o + SEG, R.total))) { snprintf(err, errlen, "HIP: download event"); return -1; }
+        auto download = [&](size_t at, const void *src, size_t bytes) -> hipError_t {   // to staging offset at, in segments
+            for (size_t o = 0; o < bytes; o += SEG) {
+                const hipError_t e = hipMemcpyAsync(stg + at + o, (const char *)src + o, std::min(SEG, bytes - o), hipMemcpyDeviceToHost, st);
+                if (e != hipSuccess) return e;
+                if (segment(at + std::min(o + SEG, bytes))) return hipErrorUnknown;
             }
+            return hipSuccess;
+        };
+        uint64_t nnz = 0;
+        if (!dev) {
+            HCK(download(0, R.base, prefix));
+            if (sparse) {
+                HCK(download(stMask, sMask, maskBytes));
+                HCK(download(stOff, sOff, (spRun + 1) * 8));
+                HCK(hipEventSynchronize(enc->spEv));
+                nnz = *enc->spTot;
+                if (nnz > gCoef) { snprintf(err, errlen, "internal: sparse coefficient count %llu", (unsigned long long)nnz); return -1; }
+                HCK(download(stVal, sVal, nnz * 2));
+            }
+        }
         auto hostOf = [&](const void *dptr) { return stg + ((const char *)dptr - (const char *)R.base); };
         std::vector<std::tuple<void *, const void *, size_t>> jobs;
-        size_t reconOff = dev ? 0 : R.total;
+        size_t reconOff = recEnd;
+        std::vector<int16_t *> coefDst;
         for (int i = 0; i < n; i++) {
             const Img &I = imgs[i];
             av1mi_frame_out &o = out[i];
@@ -960,11 +1097,13 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
                 o.blocks = (av1mi_block *)malloc(std::max<size_t>(o.n_blocks, 1) * 16);
                 o.block_txb = (uint32_t *)malloc(((size_t)o.n_blocks + 1) * 4);
                 o.txb = (av1mi_txb *)out_alloc(std::max<size_t>(o.n_txb, 1) * 16);
-                o.coeffs = (int16_t *)out_alloc(std::max<uint64_t>(o.n_coeffs, 1) * 2);
+                o.coeffs = sparse ? pf.take((size_t)i, o.n_coeffs) : nullptr;
+                if (!o.coeffs) o.coeffs = (int16_t *)out_alloc(std::max<uint64_t>(o.n_coeffs, 1) * 2, sparse);
                 jobs.emplace_back(o.blocks, hostOf(rB + imgBlk[i]), (size_t)o.n_blocks * 16);
                 jobs.emplace_back(o.block_txb, hostOf(rBT + imgBlk[i]), (size_t)o.n_blocks * 4);
                 jobs.emplace_back(o.txb, hostOf(rT + imgTxb[i]), (size_t)o.n_txb * 16);
-                jobs.emplace_back(o.coeffs, hostOf(rC + imgCoef[i]), (size_t)o.n_coeffs * 2);
+                if (sparse) coefDst.push_back(o.coeffs);
+                else jobs.emplace_back(o.coeffs, hostOf(rC + imgCoef[i]), (size_t)o.n_coeffs * 2);
             }
             if (wantRecon) {
                 for (int p = 0; p < 3; p++) {
@@ -983,6 +1122,12 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
         const int cr = parallel_copy(jobs, stg, segEnd, segEv);
         HCK(hipStreamSynchronize(st));
         if (cr) { snprintf(err, errlen, "internal: record copy"); return -1; }
+        if (sparse && (sparse_expand((const uint64_t *)(stg + stMask), (const uint64_t *)(stg + stOff), (const int16_t *)(stg + stVal), gCoef,
+                                     imgCoef, coefDst) ||
+                       ((const uint64_t *)(stg + stOff))[spRun] != nnz)) {
+            snprintf(err, errlen, "internal: sparse coefficients");
+            return -1;
+        }
         if (dev) {   // the packed arena passes to the caller as it is: frames back to back, frame-relative offsets
             memset(dev, 0, sizeof *dev);
             dev->n_frames = (uint32_t)n;
