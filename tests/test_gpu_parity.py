@@ -72,6 +72,23 @@ def test_gpu_batch_is_independent_of_composition(enc):
     assert t.blocks == sum(len(a.blocks) for a in alone)
 
 
+def test_gpu_flat_frame_in_batch_equals_oracle(enc, oracle):
+    """The host records' coefficients come down as nonzero masks and values: a flat frame at qindex 255 (no nonzero
+    level at all) between textured frames whose coefficient counts end inside a mask span equals the oracle, each frame
+    in the batch and alone."""
+    W, H = 136, 72
+    flat = [np.full((H, W), 128, np.uint8), np.full((H // 2, W // 2), 128, np.uint8), np.full((H // 2, W // 2), 128, np.uint8)]
+    specs = [(A.synth_image(1090, 96, 40), 96, 40, 120), (flat, W, H, 255), (A.synth_image(1091, 72, 56), 72, 56, 60)]
+    frames = [A.DeviceFrame(p, w, h, qindex=q) for p, w, h, q in specs]
+    together, t = enc.encode(frames)
+    assert int(np.count_nonzero(together[1].coeffs)) == 0 and len(together[1].coeffs) > 0
+    assert any(len(g.coeffs) % 512 for g in together)
+    for (p, w, h, q), f, g in zip(specs, frames, together):
+        print("frame %dx%d q%d, %d coefficients" % (w, h, q, len(g.coeffs)), flush=True)
+        _same(g, oracle.encode(p, w, h, qindex=q, threads=8))
+        _same(enc.encode([f])[0][0], oracle.encode(p, w, h, qindex=q, threads=8))
+
+
 def _decodes(ref, g, W, H, q):
     dec, avif, info = ref.encode(dict(blocks=g.blocks, block_txb=g.block_txb, txb=g.txb, coeffs=g.coeffs), W, H, q, avif=True)
     for p in range(3):
