@@ -1030,12 +1030,20 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
         size_t reconBytes = 0;
         if (wantRecon)
             for (int i = 0; i < n; i++) reconBytes += ((size_t)imgs[i].aw * imgs[i].ah + 2 * (size_t)(imgs[i].aw >> imgs[i].ss) * (imgs[i].ah >> imgs[i].ss)) * bps;
+        // the sparse form's value count: the copies below queue behind its kernels on this stream anyway, so waiting
+        // for it here costs the device nothing and sizes the staging to what is downloaded
+        uint64_t nnz = 0;
+        if (sparse) {
+            HCK(hipEventSynchronize(enc->spEv));
+            nnz = *enc->spTot;
+            if (nnz > gCoef) { snprintf(err, errlen, "internal: sparse coefficient count %llu", (unsigned long long)nnz); return -1; }
+        }
         // staging: the records before the coefficients at their arena offsets; then, for the sparse form, the masks,
-        // the runs' offsets and room for every coefficient's value; then the reconstructions
+        // the runs' offsets and the values; then the reconstructions
         const size_t prefix = sparse ? (size_t)((const char *)rC - (const char *)R.base) : R.total;
         const size_t maskBytes = spSpan * 8 * sizeof(uint64_t);
         const size_t stMask = prefix, stOff = stMask + maskBytes, stVal = (stOff + (spRun + 1) * 8 + 255) & ~(size_t)255;
-        const size_t recEnd = dev ? 0 : sparse ? (stVal + gCoef * 2 + 255) & ~(size_t)255 : R.total;
+        const size_t recEnd = dev ? 0 : sparse ? (stVal + nnz * 2 + 255) & ~(size_t)255 : R.total;
         const size_t need = recEnd + reconBytes;
         if (need > enc->stageBytes) {
             HCK(hipStreamSynchronize(st));
@@ -1068,15 +1076,11 @@ static int encode_impl(av1mi_encoder *enc, const av1mi_frame_in *in, int n, av1m
             }
             return hipSuccess;
         };
-        uint64_t nnz = 0;
         if (!dev) {
             HCK(download(0, R.base, prefix));
             if (sparse) {
                 HCK(download(stMask, sMask, maskBytes));
                 HCK(download(stOff, sOff, (spRun + 1) * 8));
-                HCK(hipEventSynchronize(enc->spEv));
-                nnz = *enc->spTot;
-                if (nnz > gCoef) { snprintf(err, errlen, "internal: sparse coefficient count %llu", (unsigned long long)nnz); return -1; }
                 HCK(download(stVal, sVal, nnz * 2));
             }
         }
