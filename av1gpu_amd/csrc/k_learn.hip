@@ -66,57 +66,71 @@ __global__ void __launch_bounds__(256) k_ln_tiles(const uint8_t *tiles, const ui
 // ---- convolution forward on MFMA: one wave computes 16 output positions x 32 channels of one superblock ----
 // A (16 x 16): lane l holds positions l % 16, taps kb + 4 (l / 16) + 0..3; B (16 x 16): taps likewise, channel l % 16;
 // D: lane l holds channel l % 16 of positions 4 (l / 16) + 0..3.
-template <int CIN, int HIN>
+// One wave computes TPW consecutive tiles of 16 output positions of one item; the weights of every 16-tap block sit in
+// registers for all of them (loaded once per wave), and each tile's MFMA operands and order are the same as with one
+// tile per wave.
+template <int CIN, int HIN, int TPW>
 __global__ void __launch_bounds__(64) k_ln_conv(const bf16_t *in, const bf16_t *W, const float *bias, bf16_t *out, uint32_t n) {
-    constexpr int HOUT = HIN / 2, NPOS = HOUT * HOUT, K = 9 * CIN, KP = (K + 15) / 16 * 16, TILES = NPOS / 16;
-    const uint32_t item = blockIdx.x / TILES, tile = blockIdx.x % TILES;
+    constexpr int HOUT = HIN / 2, NPOS = HOUT * HOUT, K = 9 * CIN, KP = (K + 15) / 16 * 16, TILES = NPOS / 16, KB = KP / 16;
+    constexpr int GROUPS = TILES / TPW;
+    static_assert(TILES % TPW == 0, "whole groups of tiles");
+    const uint32_t item = blockIdx.x / GROUPS, grp = blockIdx.x % GROUPS;
     if (item >= n) return;
-    const int l = threadIdx.x, row = l & 15, kq = l >> 4;
-    const int pos = (int)tile * 16 + row, oy = pos / HOUT, ox = pos % HOUT;
+    const int l = threadIdx.x, row = l & 15, kq = l >> 4, c = l & 15;
     const bf16_t *x = in + (size_t)item * HIN * HIN * CIN;
-    float4_t acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-    for (int kb = 0; kb < KP; kb += 16) {
-        short4_t a = {0, 0, 0, 0};
-        if constexpr (CIN % 4 == 0) {
-            // the lane's 4 taps are 4 consecutive channels of one input sample: one 8-byte load (NHWC, channel
-            // offset a multiple of 4, buffers 256-byte aligned)
-            const int k = kb + 4 * kq, t = k / CIN, ci = k % CIN, iy = 2 * oy - 1 + t / 3, ix = 2 * ox - 1 + t % 3;
-            if (k < K && iy >= 0 && iy < HIN && ix >= 0 && ix < HIN) a = *(const short4_t *)(x + ((size_t)iy * HIN + ix) * CIN + ci);
-        } else if constexpr (CIN == 2) {
-            // two taps of two channels each: one 4-byte load per tap
-            typedef short short2_t __attribute__((ext_vector_type(2)));
+    // the weight rows are zero-padded to KP (a multiple of 16): 4 consecutive taps, one 8-byte load each
+    short4_t b0[KB], b1[KB];
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int k = kb + 4 * kq + 2 * h, t = k / 2, iy = 2 * oy - 1 + t / 3, ix = 2 * ox - 1 + t % 3;
-                short2_t v = {0, 0};
-                if (k < K && iy >= 0 && iy < HIN && ix >= 0 && ix < HIN) v = *(const short2_t *)(x + ((size_t)iy * HIN + ix) * 2);
-                a[2 * h] = v[0];
-                a[2 * h + 1] = v[1];
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int k = kb + 4 * kq + j;
-                bf16_t v = 0;
-                if (k < K) {
-                    const int t = k / CIN, ci = k % CIN, iy = 2 * oy - 1 + t / 3, ix = 2 * ox - 1 + t % 3;
-                    if (iy >= 0 && iy < HIN && ix >= 0 && ix < HIN) v = x[((size_t)iy * HIN + ix) * CIN + ci];
-                }
-                a[j] = (short)v;
-            }
-        }
-        // the weight rows are zero-padded to KP (a multiple of 16): 4 consecutive taps, one 8-byte load each
-        const short4_t b0 = *(const short4_t *)(W + (size_t)(l & 15) * KP + kb + 4 * kq);
-        const short4_t b1 = *(const short4_t *)(W + (size_t)(16 + (l & 15)) * KP + kb + 4 * kq);
-        acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b0, acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b1, acc1, 0, 0, 0);
+    for (int kb = 0; kb < KB; kb++) {
+        b0[kb] = *(const short4_t *)(W + (size_t)c * KP + 16 * kb + 4 * kq);
+        b1[kb] = *(const short4_t *)(W + (size_t)(16 + c) * KP + 16 * kb + 4 * kq);
     }
+    const float bias0 = bias[c], bias1 = bias[16 + c];
     bf16_t *o = out + (size_t)item * NPOS * LN_C;
+    for (int tt = 0; tt < TPW; tt++) {
+        const int tile = (int)grp * TPW + tt, pos = tile * 16 + row, oy = pos / HOUT, ox = pos % HOUT;
+        float4_t acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const int p = (int)tile * 16 + 4 * kq + r, c = l & 15;
-        o[(size_t)p * LN_C + c] = to_bf16(fmaxf(acc0[r] + bias[c], 0.f));
-        o[(size_t)p * LN_C + 16 + c] = to_bf16(fmaxf(acc1[r] + bias[16 + c], 0.f));
+        for (int kbi = 0; kbi < KB; kbi++) {
+            const int kb = 16 * kbi;
+            short4_t a = {0, 0, 0, 0};
+            if constexpr (CIN % 4 == 0) {
+                // the lane's 4 taps are 4 consecutive channels of one input sample: one 8-byte load (NHWC, channel
+                // offset a multiple of 4, buffers 256-byte aligned)
+                const int k = kb + 4 * kq, t = k / CIN, ci = k % CIN, iy = 2 * oy - 1 + t / 3, ix = 2 * ox - 1 + t % 3;
+                if (k < K && iy >= 0 && iy < HIN && ix >= 0 && ix < HIN) a = *(const short4_t *)(x + ((size_t)iy * HIN + ix) * CIN + ci);
+            } else if constexpr (CIN == 2) {
+                // two taps of two channels each: one 4-byte load per tap
+                typedef short short2_t __attribute__((ext_vector_type(2)));
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int k = kb + 4 * kq + 2 * h, t = k / 2, iy = 2 * oy - 1 + t / 3, ix = 2 * ox - 1 + t % 3;
+                    short2_t v = {0, 0};
+                    if (k < K && iy >= 0 && iy < HIN && ix >= 0 && ix < HIN) v = *(const short2_t *)(x + ((size_t)iy * HIN + ix) * 2);
+                    a[2 * h] = v[0];
+                    a[2 * h + 1] = v[1];
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int k = kb + 4 * kq + j;
+                    bf16_t v = 0;
+                    if (k < K) {
+                        const int t = k / CIN, ci = k % CIN, iy = 2 * oy - 1 + t / 3, ix = 2 * ox - 1 + t % 3;
+                        if (iy >= 0 && iy < HIN && ix >= 0 && ix < HIN) v = x[((size_t)iy * HIN + ix) * CIN + ci];
+                    }
+                    a[j] = (short)v;
+                }
+            }
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b0[kbi], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b1[kbi], acc1, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int p = tile * 16 + 4 * kq + r;
+            o[(size_t)p * LN_C + c] = to_bf16(fmaxf(acc0[r] + bias0, 0.f));
+            o[(size_t)p * LN_C + 16 + c] = to_bf16(fmaxf(acc1[r] + bias1, 0.f));
+        }
     }
 }
 
@@ -373,10 +387,11 @@ static hipError_t ln_alloc_acts(LnNet &N, uint32_t n) {
 // forward of n items whose inputs are in a.x0
 static void ln_forward(const LnNet &N, const LnActs &a, uint32_t n, hipStream_t st) {
     const float *w = N.w;
-    hipLaunchKernelGGL((k_ln_conv<2, 64>), dim3(n * 64), dim3(64), 0, st, a.x0, N.wb + ln_bf_off(0), w + LN_W_OFF[0] + 32 * LN_KP1, a.a1, n);
-    hipLaunchKernelGGL((k_ln_conv<32, 32>), dim3(n * 16), dim3(64), 0, st, a.a1, N.wb + ln_bf_off(1), w + LN_W_OFF[1] + 32 * LN_KP, a.a2, n);
-    hipLaunchKernelGGL((k_ln_conv<32, 16>), dim3(n * 4), dim3(64), 0, st, a.a2, N.wb + ln_bf_off(2), w + LN_W_OFF[2] + 32 * LN_KP, a.a3, n);
-    hipLaunchKernelGGL((k_ln_conv<32, 8>), dim3(n), dim3(64), 0, st, a.a3, N.wb + ln_bf_off(3), w + LN_W_OFF[3] + 32 * LN_KP, a.a4, n);
+    // grids: items x (tiles / tiles per wave); 64, 16, 4 and 1 tiles per item
+    hipLaunchKernelGGL((k_ln_conv<2, 64, 8>), dim3(n * 8), dim3(64), 0, st, a.x0, N.wb + ln_bf_off(0), w + LN_W_OFF[0] + 32 * LN_KP1, a.a1, n);
+    hipLaunchKernelGGL((k_ln_conv<32, 32, 4>), dim3(n * 4), dim3(64), 0, st, a.a1, N.wb + ln_bf_off(1), w + LN_W_OFF[1] + 32 * LN_KP, a.a2, n);
+    hipLaunchKernelGGL((k_ln_conv<32, 16, 4>), dim3(n), dim3(64), 0, st, a.a2, N.wb + ln_bf_off(2), w + LN_W_OFF[2] + 32 * LN_KP, a.a3, n);
+    hipLaunchKernelGGL((k_ln_conv<32, 8, 1>), dim3(n), dim3(64), 0, st, a.a3, N.wb + ln_bf_off(3), w + LN_W_OFF[3] + 32 * LN_KP, a.a4, n);
     hipLaunchKernelGGL(k_ln_fc, dim3((n + LN_FC_IT - 1) / LN_FC_IT), dim3(128), 0, st, a.a4, w + LN_FC_OFF, w + LN_FC_OFF + (size_t)LN_OUT * LN_FEAT,
                        a.logits, n);
 }
