@@ -87,6 +87,7 @@ __global__ void __launch_bounds__(64) k_ln_conv(const bf16_t *in, const bf16_t *
     }
     const float bias0 = bias[c], bias1 = bias[16 + c];
     bf16_t *o = out + (size_t)item * NPOS * LN_C;
+    __shared__ __attribute__((aligned(16))) bf16_t stage[16 * LN_C];
     for (int tt = 0; tt < TPW; tt++) {
         const int tile = (int)grp * TPW + tt, pos = tile * 16 + row, oy = pos / HOUT, ox = pos % HOUT;
         float4_t acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
@@ -125,12 +126,16 @@ __global__ void __launch_bounds__(64) k_ln_conv(const bf16_t *in, const bf16_t *
             acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b0[kbi], acc0, 0, 0, 0);
             acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b1[kbi], acc1, 0, 0, 0);
         }
+        // the tile's 16 positions x 32 channels are 1 KB of consecutive output (NHWC): staged in LDS, then one
+        // 16-byte store per lane
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int p = tile * 16 + 4 * kq + r;
-            o[(size_t)p * LN_C + c] = to_bf16(fmaxf(acc0[r] + bias0, 0.f));
-            o[(size_t)p * LN_C + 16 + c] = to_bf16(fmaxf(acc1[r] + bias1, 0.f));
+            stage[(4 * kq + r) * LN_C + c] = to_bf16(fmaxf(acc0[r] + bias0, 0.f));
+            stage[(4 * kq + r) * LN_C + 16 + c] = to_bf16(fmaxf(acc1[r] + bias1, 0.f));
         }
+        __syncthreads();
+        *(uint4 *)(o + (size_t)tile * 16 * LN_C + 8 * l) = *(const uint4 *)(stage + 8 * l);
+        __syncthreads();
     }
 }
 
