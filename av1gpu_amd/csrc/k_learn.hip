@@ -66,75 +66,149 @@ __global__ void __launch_bounds__(256) k_ln_tiles(const uint8_t *tiles, const ui
 // ---- convolution forward on MFMA: one wave computes 16 output positions x 32 channels of one superblock ----
 // A (16 x 16): lane l holds positions l % 16, taps kb + 4 (l / 16) + 0..3; B (16 x 16): taps likewise, channel l % 16;
 // D: lane l holds channel l % 16 of positions 4 (l / 16) + 0..3.
-// One wave computes TPW consecutive tiles of 16 output positions of one item; the weights of every 16-tap block sit in
-// registers for all of them (loaded once per wave), and each tile's MFMA operands and order are the same as with one
-// tile per wave.
+// Geometry of a 3x3 stride-2 convolution layer (zero padding) over HIN x HIN x CIN NHWC activations: 16-position
+// tiles of the HOUT x HOUT output, K = 9 CIN taps in KB blocks of 16 (the weight rows are zero-padded to KP).
+template <int CIN, int HIN>
+struct LnConv {
+    static constexpr int HOUT = HIN / 2, NPOS = HOUT * HOUT, K = 9 * CIN, KP = (K + 15) / 16 * 16, TILES = NPOS / 16, KB = KP / 16;
+};
+
+// the weights of every 16-tap block for lane l (4 consecutive taps of output channels l & 15 and 16 + (l & 15))
+template <int CIN, int HIN>
+__device__ __forceinline__ void conv_weights(const bf16_t *W, int l, short4_t (&b0)[LnConv<CIN, HIN>::KB], short4_t (&b1)[LnConv<CIN, HIN>::KB]) {
+    using G = LnConv<CIN, HIN>;
+    const int c = l & 15, kq = l >> 4;
+#pragma unroll
+    for (int kb = 0; kb < G::KB; kb++) {
+        b0[kb] = *(const short4_t *)(W + (size_t)c * G::KP + 16 * kb + 4 * kq);
+        b1[kb] = *(const short4_t *)(W + (size_t)(16 + c) * G::KP + 16 * kb + 4 * kq);
+    }
+}
+
+// One tile: 16 output positions x 32 channels as two 16x16x16 BF16 MFMA accumulations over the tap blocks in
+// ascending order; lane l carries position tile * 16 + (l & 15) of the A operand and taps 4 (l >> 4) .. + 3 of each
+// block.  x is the item's input, in global memory or LDS.
+template <int CIN, int HIN>
+__device__ __forceinline__ void conv_tile(const bf16_t *x, const short4_t (&b0)[LnConv<CIN, HIN>::KB],
+                                          const short4_t (&b1)[LnConv<CIN, HIN>::KB], int tile, int l, float4_t &acc0, float4_t &acc1) {
+    using G = LnConv<CIN, HIN>;
+    constexpr int K = G::K;
+    const int row = l & 15, kq = l >> 4, pos = tile * 16 + row, oy = pos / G::HOUT, ox = pos % G::HOUT;
+    acc0 = float4_t{0.f, 0.f, 0.f, 0.f};
+    acc1 = float4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kbi = 0; kbi < G::KB; kbi++) {
+        const int kb = 16 * kbi;
+        short4_t a = {0, 0, 0, 0};
+        if constexpr (CIN % 4 == 0) {
+            // the lane's 4 taps are 4 consecutive channels of one input sample: one 8-byte load (NHWC, channel
+            // offset a multiple of 4, buffers 256-byte aligned)
+            const int k = kb + 4 * kq, t = k / CIN, ci = k % CIN, iy = 2 * oy - 1 + t / 3, ix = 2 * ox - 1 + t % 3;
+            if (k < K && iy >= 0 && iy < HIN && ix >= 0 && ix < HIN) a = *(const short4_t *)(x + ((size_t)iy * HIN + ix) * CIN + ci);
+        } else if constexpr (CIN == 2) {
+            // two taps of two channels each: one 4-byte load per tap
+            typedef short short2_t __attribute__((ext_vector_type(2)));
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int k = kb + 4 * kq + 2 * h, t = k / 2, iy = 2 * oy - 1 + t / 3, ix = 2 * ox - 1 + t % 3;
+                short2_t v = {0, 0};
+                if (k < K && iy >= 0 && iy < HIN && ix >= 0 && ix < HIN) v = *(const short2_t *)(x + ((size_t)iy * HIN + ix) * 2);
+                a[2 * h] = v[0];
+                a[2 * h + 1] = v[1];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int k = kb + 4 * kq + j;
+                bf16_t v = 0;
+                if (k < K) {
+                    const int t = k / CIN, ci = k % CIN, iy = 2 * oy - 1 + t / 3, ix = 2 * ox - 1 + t % 3;
+                    if (iy >= 0 && iy < HIN && ix >= 0 && ix < HIN) v = x[((size_t)iy * HIN + ix) * CIN + ci];
+                }
+                a[j] = (short)v;
+            }
+        }
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b0[kbi], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b1[kbi], acc1, 0, 0, 0);
+    }
+}
+
+// a tile's outputs (bias, ReLU, BF16) at their NHWC places in dst, the 16 positions' 32 channels from tile * 512
+__device__ __forceinline__ void conv_tile_out(bf16_t *dst, int tile, int l, const float4_t &acc0, const float4_t &acc1, float bias0,
+                                              float bias1) {
+    const int c = l & 15, kq = l >> 4;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int p = tile * 16 + 4 * kq + r;
+        dst[(size_t)p * LN_C + c] = to_bf16(fmaxf(acc0[r] + bias0, 0.f));
+        dst[(size_t)p * LN_C + 16 + c] = to_bf16(fmaxf(acc1[r] + bias1, 0.f));
+    }
+}
+
+// One wave computes TPW consecutive tiles of one item, the weights in registers for all of them (loaded once per
+// wave); each tile's outputs (1 KB of consecutive NHWC activations) are staged in LDS and written with one 16-byte
+// store per lane.
 template <int CIN, int HIN, int TPW>
 __global__ void __launch_bounds__(64) k_ln_conv(const bf16_t *in, const bf16_t *W, const float *bias, bf16_t *out, uint32_t n) {
-    constexpr int HOUT = HIN / 2, NPOS = HOUT * HOUT, K = 9 * CIN, KP = (K + 15) / 16 * 16, TILES = NPOS / 16, KB = KP / 16;
-    constexpr int GROUPS = TILES / TPW;
-    static_assert(TILES % TPW == 0, "whole groups of tiles");
+    using G = LnConv<CIN, HIN>;
+    constexpr int GROUPS = G::TILES / TPW;
+    static_assert(G::TILES % TPW == 0, "whole groups of tiles");
     const uint32_t item = blockIdx.x / GROUPS, grp = blockIdx.x % GROUPS;
     if (item >= n) return;
-    const int l = threadIdx.x, row = l & 15, kq = l >> 4, c = l & 15;
+    const int l = threadIdx.x, c = l & 15;
     const bf16_t *x = in + (size_t)item * HIN * HIN * CIN;
-    // the weight rows are zero-padded to KP (a multiple of 16): 4 consecutive taps, one 8-byte load each
-    short4_t b0[KB], b1[KB];
-#pragma unroll
-    for (int kb = 0; kb < KB; kb++) {
-        b0[kb] = *(const short4_t *)(W + (size_t)c * KP + 16 * kb + 4 * kq);
-        b1[kb] = *(const short4_t *)(W + (size_t)(16 + c) * KP + 16 * kb + 4 * kq);
-    }
+    short4_t b0[G::KB], b1[G::KB];
+    conv_weights<CIN, HIN>(W, l, b0, b1);
     const float bias0 = bias[c], bias1 = bias[16 + c];
-    bf16_t *o = out + (size_t)item * NPOS * LN_C;
+    bf16_t *o = out + (size_t)item * G::NPOS * LN_C;
     __shared__ __attribute__((aligned(16))) bf16_t stage[16 * LN_C];
     for (int tt = 0; tt < TPW; tt++) {
-        const int tile = (int)grp * TPW + tt, pos = tile * 16 + row, oy = pos / HOUT, ox = pos % HOUT;
-        float4_t acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kbi = 0; kbi < KB; kbi++) {
-            const int kb = 16 * kbi;
-            short4_t a = {0, 0, 0, 0};
-            if constexpr (CIN % 4 == 0) {
-                // the lane's 4 taps are 4 consecutive channels of one input sample: one 8-byte load (NHWC, channel
-                // offset a multiple of 4, buffers 256-byte aligned)
-                const int k = kb + 4 * kq, t = k / CIN, ci = k % CIN, iy = 2 * oy - 1 + t / 3, ix = 2 * ox - 1 + t % 3;
-                if (k < K && iy >= 0 && iy < HIN && ix >= 0 && ix < HIN) a = *(const short4_t *)(x + ((size_t)iy * HIN + ix) * CIN + ci);
-            } else if constexpr (CIN == 2) {
-                // two taps of two channels each: one 4-byte load per tap
-                typedef short short2_t __attribute__((ext_vector_type(2)));
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const int k = kb + 4 * kq + 2 * h, t = k / 2, iy = 2 * oy - 1 + t / 3, ix = 2 * ox - 1 + t % 3;
-                    short2_t v = {0, 0};
-                    if (k < K && iy >= 0 && iy < HIN && ix >= 0 && ix < HIN) v = *(const short2_t *)(x + ((size_t)iy * HIN + ix) * 2);
-                    a[2 * h] = v[0];
-                    a[2 * h + 1] = v[1];
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int k = kb + 4 * kq + j;
-                    bf16_t v = 0;
-                    if (k < K) {
-                        const int t = k / CIN, ci = k % CIN, iy = 2 * oy - 1 + t / 3, ix = 2 * ox - 1 + t % 3;
-                        if (iy >= 0 && iy < HIN && ix >= 0 && ix < HIN) v = x[((size_t)iy * HIN + ix) * CIN + ci];
-                    }
-                    a[j] = (short)v;
-                }
-            }
-            acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b0[kbi], acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b1[kbi], acc1, 0, 0, 0);
-        }
-        // the tile's 16 positions x 32 channels are 1 KB of consecutive output (NHWC): staged in LDS, then one
-        // 16-byte store per lane
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            stage[(4 * kq + r) * LN_C + c] = to_bf16(fmaxf(acc0[r] + bias0, 0.f));
-            stage[(4 * kq + r) * LN_C + 16 + c] = to_bf16(fmaxf(acc1[r] + bias1, 0.f));
-        }
+        const int tile = (int)grp * TPW + tt;
+        float4_t acc0, acc1;
+        conv_tile<CIN, HIN>(x, b0, b1, tile, l, acc0, acc1);
+        conv_tile_out(stage, 0, l, acc0, acc1, bias0, bias1);
         __syncthreads();
         *(uint4 *)(o + (size_t)tile * 16 * LN_C + 8 * l) = *(const uint4 *)(stage + 8 * l);
+        __syncthreads();
+    }
+}
+
+// The first two layers for one item per 256-thread workgroup, the first layer's 32 x 32 x 32 output (64 KB) kept in
+// LDS: the four waves take 16 of its 64 tiles each, then 4 of the second layer's 16 tiles each, reading it from LDS.
+// The tiles' arithmetic is conv_tile's, so the second layer's output equals k_ln_conv<2, 64> then k_ln_conv<32, 32>.
+__global__ void __launch_bounds__(256) k_ln_conv12(const bf16_t *in, const bf16_t *W1, const float *bias1, const bf16_t *W2,
+                                                   const float *bias2, bf16_t *out, uint32_t n) {
+    using G1 = LnConv<2, 64>;
+    using G2 = LnConv<32, 32>;
+    const uint32_t item = blockIdx.x;
+    if (item >= n) return;
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6, c = l & 15;
+    __shared__ __attribute__((aligned(16))) bf16_t a1[G1::NPOS * LN_C];
+    __shared__ __attribute__((aligned(16))) bf16_t stage[4][16 * LN_C];
+    {
+        const bf16_t *x = in + (size_t)item * 64 * 64 * 2;
+        short4_t b0[G1::KB], b1[G1::KB];
+        conv_weights<2, 64>(W1, l, b0, b1);
+        const float bb0 = bias1[c], bb1 = bias1[16 + c];
+        for (int tt = 0; tt < G1::TILES / 4; tt++) {
+            const int tile = w * (G1::TILES / 4) + tt;
+            float4_t acc0, acc1;
+            conv_tile<2, 64>(x, b0, b1, tile, l, acc0, acc1);
+            conv_tile_out(a1, tile, l, acc0, acc1, bb0, bb1);
+        }
+    }
+    __syncthreads();
+    short4_t b0[G2::KB], b1[G2::KB];
+    conv_weights<32, 32>(W2, l, b0, b1);
+    const float bb0 = bias2[c], bb1 = bias2[16 + c];
+    bf16_t *o = out + (size_t)item * G2::NPOS * LN_C;
+    for (int tt = 0; tt < G2::TILES / 4; tt++) {
+        const int tile = w * (G2::TILES / 4) + tt;
+        float4_t acc0, acc1;
+        conv_tile<32, 32>(a1, b0, b1, tile, l, acc0, acc1);
+        conv_tile_out(stage[w], 0, l, acc0, acc1, bb0, bb1);
+        __syncthreads();
+        *(uint4 *)(o + (size_t)tile * 16 * LN_C + 8 * l) = *(const uint4 *)(stage[w] + 8 * l);
         __syncthreads();
     }
 }
@@ -390,11 +464,18 @@ static hipError_t ln_alloc_acts(LnNet &N, uint32_t n) {
 }
 
 // forward of n items whose inputs are in a.x0
-static void ln_forward(const LnNet &N, const LnActs &a, uint32_t n, hipStream_t st) {
+// fused: the first two layers in one launch (k_ln_conv12) without writing the first layer's activations, for
+// inference; training and the activation check read them
+static void ln_forward(const LnNet &N, const LnActs &a, uint32_t n, hipStream_t st, bool fused = false) {
     const float *w = N.w;
     // grids: items x (tiles / tiles per wave); 64, 16, 4 and 1 tiles per item
-    hipLaunchKernelGGL((k_ln_conv<2, 64, 8>), dim3(n * 8), dim3(64), 0, st, a.x0, N.wb + ln_bf_off(0), w + LN_W_OFF[0] + 32 * LN_KP1, a.a1, n);
-    hipLaunchKernelGGL((k_ln_conv<32, 32, 4>), dim3(n * 4), dim3(64), 0, st, a.a1, N.wb + ln_bf_off(1), w + LN_W_OFF[1] + 32 * LN_KP, a.a2, n);
+    if (fused) {
+        hipLaunchKernelGGL(k_ln_conv12, dim3(n), dim3(256), 0, st, a.x0, N.wb + ln_bf_off(0), w + LN_W_OFF[0] + 32 * LN_KP1, N.wb + ln_bf_off(1),
+                           w + LN_W_OFF[1] + 32 * LN_KP, a.a2, n);
+    } else {
+        hipLaunchKernelGGL((k_ln_conv<2, 64, 8>), dim3(n * 8), dim3(64), 0, st, a.x0, N.wb + ln_bf_off(0), w + LN_W_OFF[0] + 32 * LN_KP1, a.a1, n);
+        hipLaunchKernelGGL((k_ln_conv<32, 32, 4>), dim3(n * 4), dim3(64), 0, st, a.a1, N.wb + ln_bf_off(1), w + LN_W_OFF[1] + 32 * LN_KP, a.a2, n);
+    }
     hipLaunchKernelGGL((k_ln_conv<32, 16, 4>), dim3(n), dim3(64), 0, st, a.a2, N.wb + ln_bf_off(2), w + LN_W_OFF[2] + 32 * LN_KP, a.a3, n);
     hipLaunchKernelGGL((k_ln_conv<32, 8, 1>), dim3(n), dim3(64), 0, st, a.a3, N.wb + ln_bf_off(3), w + LN_W_OFF[3] + 32 * LN_KP, a.a4, n);
     hipLaunchKernelGGL(k_ln_fc, dim3((n + LN_FC_IT - 1) / LN_FC_IT), dim3(128), 0, st, a.a4, w + LN_FC_OFF, w + LN_FC_OFF + (size_t)LN_OUT * LN_FEAT,
@@ -442,7 +523,7 @@ int ln_masks(const LnNet &N, void *scratch, const Img *dImg, const uint32_t *dSb
     for (uint32_t s0 = 0; s0 < nSb; s0 += chunk) {
         const uint32_t n = nSb - s0 < chunk ? nSb - s0 : chunk;
         hipLaunchKernelGGL(k_ln_gather, dim3(n), dim3(256), 0, st, dImg, dSbImg, s0, n, a.x0);
-        ln_forward(N, a, n, st);
+        ln_forward(N, a, n, st, true);
         hipLaunchKernelGGL(k_ln_decide, dim3((n + 63) / 64), dim3(64), 0, st, a.logits, dImg, dSbImg, s0, n, conf, dMask);
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
