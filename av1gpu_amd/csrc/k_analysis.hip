@@ -459,13 +459,18 @@ __global__ void k_an_blank(BlockRes *blk, uint32_t n) {
 // Measured on MI355X (analysis of the 256-image batch): 908 ms uncapped, 836 / 811 / 771 / 771 / 780 / 798 ms at 2 /
 // 4 / 16 / 32 / 64 / 128 (few workgroups balance the unevenly pruned blocks worse)
 constexpr int AN_WAVES_PER_SLOT = 16;
+// twice that for the shapes that measured faster at 32 (kernel trace of the same batch, every shape at 8 / 16 / 32:
+// 4x4 29.2 -> 27.8 ms, 64x32 55.7 -> 54.6, 64x64 49.4 -> 48.6, 4x8, 8x4, 16x32, 32x64 0.5-0.9 ms each)
+constexpr int an_slot_mult(int W, int H) {
+    return (W == 4 && H <= 8) || (W == 8 && H == 4) || (W == 16 && H == 32) || (W == 32 && H == 64) || (W == 64 && H >= 32) ? 2 : 1;
+}
 template <typename K>
-static uint32_t an_grid_cap(K kernel, int threads) {
+static uint32_t an_grid_cap(K kernel, int threads, int mult) {
     int dev = 0, cus = 0, occ = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, threads, 0) != hipSuccess || cus <= 0 || occ <= 0)
         return 1u << 30;
-    return (uint32_t)cus * (uint32_t)occ * AN_WAVES_PER_SLOT;
+    return (uint32_t)cus * (uint32_t)occ * AN_WAVES_PER_SLOT * (uint32_t)mult;
 }
 
 template <int W, int H, int BS>
@@ -477,10 +482,11 @@ static void launch_an(const Batch &B, const SearchParams &P, uint32_t n, int bd,
     // the cap of this instance on the current device (host-side queries, per launch: encoders on several host threads
     // share nothing here)
     const bool hooked = hk.ext || hk.list;
-    const uint32_t c = bd == 10 ? (hooked ? an_grid_cap(k_analysis<W, H, BS, uint16_t, true>, NG * 32)
-                                          : an_grid_cap(k_analysis<W, H, BS, uint16_t, false>, NG * 32))
-                                : (hooked ? an_grid_cap(k_analysis<W, H, BS, uint8_t, true>, NG * 32)
-                                          : an_grid_cap(k_analysis<W, H, BS, uint8_t, false>, NG * 32));
+    constexpr int M = an_slot_mult(W, H);
+    const uint32_t c = bd == 10 ? (hooked ? an_grid_cap(k_analysis<W, H, BS, uint16_t, true>, NG * 32, M)
+                                          : an_grid_cap(k_analysis<W, H, BS, uint16_t, false>, NG * 32, M))
+                                : (hooked ? an_grid_cap(k_analysis<W, H, BS, uint8_t, true>, NG * 32, M)
+                                          : an_grid_cap(k_analysis<W, H, BS, uint8_t, false>, NG * 32, M));
     uint32_t grid = (n + NG - 1) / NG;
     if (grid > c) grid = c;
     if (bd == 10) {
